@@ -1,0 +1,537 @@
+"""CPU oracle for the boundary-integral hot path of QuantumBilliards.jl.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module, and only as the checker
+(or the timed CPU baseline).  The product path (quantumbilliards.jl_amd/) never
+imports it.
+
+It restates, in NumPy/SciPy, the reference's algorithm for:
+  * geometry inputs and the Kress node rule
+      src/solvers/sweepmethods/cfie/cfie.jl:424, :461-475, :535-561, :822-832
+      src/solvers/sweepmethods/dlp/dlp_kress.jl:359-386
+      src/billiards/{circle.jl:69-75, ellipse.jl:69-79, star_with_hole.jl:16-28,
+                     circle_with_circle_holes.jl:43-54}
+  * geometry cache and the Kress R matrix
+      src/billiards/geometry/boundarypoints.jl:310-365, :823-884
+  * block caches / panel lookup
+      src/chebyshev/chebyshev_euclidian_helmholtz_cfie_kress.jl:362-448
+      src/chebyshev/chebyshev_euclidian_helmholtz_dlp_kress.jl:119-182
+      src/chebyshev/chebyshev_bessels.jl:574-577
+  * Chebyshev tables (breaks, Lobatto samples, DCT-I fit, Clenshaw)
+      src/chebyshev/chebyshev_core.jl:67-103, :114-314
+      src/chebyshev/chebyshev_bessels.jl:331-372, :461-504
+    with the samples taken from scipy.special.hankel1 / jv, which wrap the
+    same AMOS routines (zbesh/zbesj) that SpecialFunctions.besselh/besselj wrap.
+  * CFIE-Kress and DLP-Kress assembly
+      src/chebyshev/chebyshev_euclidian_helmholtz_cfie_kress.jl:543-665, :1431-1450
+      src/chebyshev/chebyshev_euclidian_helmholtz_dlp_kress.jl:702-769, :1086-1124
+  * the exact-Hankel (direct, real k) twins
+      src/solvers/sweepmethods/cfie/cfie_kress.jl:416-510
+      src/solvers/sweepmethods/dlp/dlp_kress.jl:802-828, :906-932
+  * the Beyn contour loop
+      src/solvers/acceleratedmethods/beyn.jl:257-322, :364-397
+    with LAPACK (scipy.linalg) for LU / solve / SVD / eig, as the reference
+    uses LAPACK through Julia's LinearAlgebra.
+
+How each part is pinned (besides the reference's own sources) is stated in
+tests/test_oracle.py: closed forms (circle: circulant T(z) whose spectrum is
+given by J_m H_m products; Beyn eigenvalues = zeros of J_m), brute-force
+Kress R, DCT-I exactness, mpmath Bessel values, and LAPACK residuals.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.special as sp
+
+TWO_PI = 2.0 * math.pi
+INV_TWO_PI = 1.0 / TWO_PI
+EULER = 0.57721566490153286061
+EULER_OVER_PI = EULER / math.pi
+H_CUT_SMALL = 1e-3   # chebyshev_bessels.jl:238
+H_CUT = 0.2          # chebyshev_bessels.jl:239
+
+# --------------------------------------------------------------------------
+# Geometry (curves parametrised by t in [0, 2pi); derivatives w.r.t. t)
+# --------------------------------------------------------------------------
+
+
+def curve_circle(R=1.0, cx=0.0, cy=0.0):
+    """make_circle / CircleSegment (circle.jl:69-75, circlesegment.jl:16-18,175-182)."""
+    def f(t):
+        c, s = np.cos(t), np.sin(t)
+        return (R * c + cx, R * s + cy, -R * s, R * c, -R * c, -R * s)
+    return f, TWO_PI * R
+
+
+def curve_ellipse(a=1.0, b=0.6):
+    """make_full_ellipse (ellipse.jl:69-79): r(u) = (a cos 2pi u, b sin 2pi u)."""
+    def f(t):
+        c, s = np.cos(t), np.sin(t)
+        return (a * c, b * s, -a * s, b * c, -a * c, -b * s)
+    L = 4.0 * a * sp.ellipe(1.0 - (b / a) ** 2) if a >= b else 4.0 * b * sp.ellipe(1.0 - (a / b) ** 2)
+    return f, L
+
+
+def curve_star(R=1.0, eps=0.2, m=5):
+    """make_star_polar_boundary (star_with_hole.jl:16-28): r = R(1 + eps cos(m t))."""
+    def f(t):
+        c, s = np.cos(t), np.sin(t)
+        r = R * (1.0 + eps * np.cos(m * t))
+        r1 = -R * eps * m * np.sin(m * t)
+        r2 = -R * eps * m * m * np.cos(m * t)
+        x, y = r * c, r * s
+        dx = r1 * c - r * s
+        dy = r1 * s + r * c
+        ddx = r2 * c - 2.0 * r1 * s - r * c
+        ddy = r2 * s + 2.0 * r1 * c - r * s
+        return (x, y, dx, dy, ddx, ddy)
+    # perimeter by high-order periodic trapezoid (spectrally exact)
+    tt = np.arange(4096) * (TWO_PI / 4096)
+    _, _, dx, dy, _, _ = f(tt)
+    L = float(np.sum(np.hypot(dx, dy)) * (TWO_PI / 4096))
+    return f, L
+
+
+@dataclass
+class Comp:
+    """One boundary component, the fields of BoundaryPointsCFIE the path reads
+    (boundarypoints.jl:487-502)."""
+    x: np.ndarray
+    y: np.ndarray
+    dx: np.ndarray
+    dy: np.ndarray
+    ddx: np.ndarray
+    ddy: np.ndarray
+    ts: np.ndarray
+    ws: np.ndarray
+
+
+def kress_nodes(fcurve, N: int) -> Comp:
+    """_evaluate_points(::CFIE_kress) node rule (cfie.jl:424, :535-561):
+    t_j = 2pi (j - 1/2)/N, ws = 2pi/N, derivatives rescaled to d/dt."""
+    j = np.arange(1, N + 1, dtype=np.float64)
+    ts = TWO_PI * (j - 0.5) / N
+    x, y, dx, dy, ddx, ddy = fcurve(ts)
+    return Comp(x, y, dx, dy, ddx, ddy, ts, np.full(N, TWO_PI / N))
+
+
+def reverse_comp(c: Comp) -> Comp:
+    """_reverse_component_orientation (cfie.jl:461-475): reverse xy, tangent
+    reversed and negated, tangent_2 reversed, ts and ws reversed."""
+    return Comp(c.x[::-1].copy(), c.y[::-1].copy(), -c.dx[::-1], -c.dy[::-1],
+                c.ddx[::-1].copy(), c.ddy[::-1].copy(), c.ts[::-1].copy(), c.ws[::-1].copy())
+
+
+def n_points(k: float, L: float, b: float, min_pts: int = 20) -> int:
+    """N = max(min_pts, round(k L b / 2pi)) bumped to even (cfie.jl:537-549)."""
+    N = max(min_pts, int(round(k * L * b / TWO_PI)))
+    return N + (N % 2)
+
+
+def make_geometry(kind: str, Ns, **kw):
+    """Components for the workloads: 'circle', 'ellipse', 'star', 'circle_hole'."""
+    if kind == "circle":
+        return [kress_nodes(curve_circle(kw.get("R", 1.0))[0], Ns[0])]
+    if kind == "ellipse":
+        return [kress_nodes(curve_ellipse(kw.get("a", 1.0), kw.get("b", 0.6))[0], Ns[0])]
+    if kind == "star":
+        return [kress_nodes(curve_star(kw.get("R", 1.0), kw.get("eps", 0.2), kw.get("m", 5))[0], Ns[0])]
+    if kind == "circle_hole":
+        outer = kress_nodes(curve_circle(kw.get("R", 1.0))[0], Ns[0])
+        rh, hx, hy = kw.get("hole", (0.3, 0.35, 0.0))
+        hole = reverse_comp(kress_nodes(curve_circle(rh, hx, hy)[0], Ns[1]))
+        return [outer, hole]
+    raise ValueError(kind)
+
+
+# --------------------------------------------------------------------------
+# Kress R and the geometry cache
+# --------------------------------------------------------------------------
+
+
+def kress_R_col(N: int) -> np.ndarray:
+    """First column of the Kress circulant (boundarypoints.jl:310-365):
+    even N=2n: R_d = -(4pi/N) sum_{m=1}^{n-1} cos(2pi m d/N)/m - (4pi/N^2)(-1)^d,
+    odd N=2n+1: R_d = -(4pi/N) sum_{m=1}^{n} cos(2pi m d/N)/m.
+    R[i,j] = R_col[(i-j) mod N]."""
+    d = np.arange(N)
+    n = N // 2 if N % 2 == 0 else (N - 1) // 2
+    mmax = n - 1 if N % 2 == 0 else n
+    acc = np.zeros(N)
+    for m in range(1, mmax + 1):
+        acc += np.cos(TWO_PI * ((m * d) % N) / N) / m
+    R = -(2.0 * TWO_PI / N) * acc
+    if N % 2 == 0:
+        R -= (2.0 * TWO_PI / (N * N)) * np.where(d % 2 == 0, 1.0, -1.0)
+    return R
+
+
+def kress_R(N: int) -> np.ndarray:
+    col = kress_R_col(N)
+    i = np.arange(N)
+    return col[(i[:, None] - i[None, :]) % N]
+
+
+@dataclass
+class GeomCache:
+    """cfie_geom_cache (boundarypoints.jl:854-884)."""
+    R: np.ndarray
+    invR: np.ndarray
+    inner: np.ndarray
+    logterm: np.ndarray
+    speed: np.ndarray
+    kappa: np.ndarray
+
+
+def geom_cache(c: Comp) -> GeomCache:
+    DX = c.x[:, None] - c.x[None, :]
+    DY = c.y[:, None] - c.y[None, :]
+    R = np.hypot(DX, DY)
+    np.fill_diagonal(R, 1.0)
+    invR = 1.0 / R
+    np.fill_diagonal(invR, 0.0)
+    inner = c.dy[None, :] * DX - c.dx[None, :] * DY
+    DT = c.ts[:, None] - c.ts[None, :]
+    with np.errstate(divide="ignore"):
+        lt = np.log(4.0 * np.sin(DT / 2.0) ** 2)
+    np.fill_diagonal(lt, 0.0)
+    speed = np.sqrt(c.dx ** 2 + c.dy ** 2)
+    kappa = INV_TWO_PI * (-(c.dx * c.ddy - c.dy * c.ddx) / (c.dx ** 2 + c.dy ** 2))
+    return GeomCache(R, invR, inner, lt, speed, kappa)
+
+
+# --------------------------------------------------------------------------
+# Chebyshev tables
+# --------------------------------------------------------------------------
+
+
+def breaks_uniform(rmin: float, rmax: float, npan: int) -> np.ndarray:
+    """_breaks_uniform (chebyshev_core.jl:95-103)."""
+    h = (rmax - rmin) / npan
+    b = np.arange(npan + 1) * h + rmin
+    b[-1] = rmax
+    return b
+
+
+def lobatto_t(M: int) -> np.ndarray:
+    """cospi(j/M), j = 0..M."""
+    j = np.arange(M + 1)
+    t = np.cos(np.pi * j / M)
+    t[0] = 1.0
+    t[-1] = -1.0
+    if M % 2 == 0:
+        t[M // 2] = 0.0
+    return t
+
+
+def chebfit(f: np.ndarray) -> np.ndarray:
+    """_chebfit! (chebyshev_core.jl:67-80) on the last axis: DCT-I with
+    endpoint half weights, c_0 halved."""
+    M = f.shape[-1] - 1
+    j = np.arange(M + 1)
+    C = np.cos(np.pi * np.outer(j, j) / M)   # [m, j]
+    w = np.ones(M + 1)
+    w[0] = w[-1] = 0.5
+    c = (2.0 / M) * np.einsum("...j,mj->...m", f * w, C)
+    c[..., 0] *= 0.5
+    return c
+
+
+def build_table(kind: str, nu: int, k: complex, rmin: float, rmax: float, npan: int, M: int):
+    """plan_h (bessels:461-471) / plan_j (bessels:494-504): coefficient array
+    [npan, M+1] of H_nu^(1)(k r) or J_nu(k r) on uniform panels."""
+    br = breaks_uniform(rmin, rmax, npan)
+    a, b = br[:-1, None], br[1:, None]
+    t = lobatto_t(M)[None, :]
+    r = ((b + a) + (b - a) * t) / 2.0
+    z = complex(k) * r
+    f = sp.hankel1(nu, z) if kind == "h" else sp.jv(nu, z)
+    return chebfit(f)
+
+
+def clenshaw(c: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """_cheb_clenshaw (chebyshev_core.jl:287-314); c[..., m] per point."""
+    M = c.shape[-1] - 1
+    b1 = np.zeros(t.shape, np.complex128)
+    b2 = np.zeros(t.shape, np.complex128)
+    u = 2.0 * t
+    for m in range(M, 0, -1):
+        b0 = (u * b1 - b2) + c[..., m]
+        b2 = b1
+        b1 = b0
+    return (t * b1 - b2) + c[..., 0]
+
+
+@dataclass
+class Panels:
+    rmin: float
+    rmax: float
+    npan: int
+    M: int
+
+    @property
+    def dr(self):
+        return (self.rmax - self.rmin) / self.npan
+
+    def locate(self, r: np.ndarray):
+        """_find_panel_uniform (bessels:574-577) + t = (2r-(b+a))/(b-a)
+        (cfie-kress cheb :437-444).  Returns 0-based panel index."""
+        invdr = 1.0 / self.dr
+        p = np.floor((r - self.rmin) * invdr).astype(np.int64) + 1
+        p = np.clip(p, 1, self.npan)
+        h = (self.rmax - self.rmin) / self.npan
+        a = (p - 1) * h + self.rmin
+        b = np.where(p == self.npan, self.rmax, p * h + self.rmin)
+        t = (2.0 * r - (b + a)) / (b - a)
+        return p - 1, t
+
+
+@dataclass
+class ChebParams:
+    n_panels_h: int = 15000
+    M_h: int = 5
+    n_panels_j: int = 3000
+    M_j: int = 5
+
+
+def _rrange_blocks(comps):
+    """Global padded [rmin, rmax] over all component blocks
+    (cfie-kress cheb :374-418; dlp-kress cheb :135-147)."""
+    rmin, rmax = np.inf, 0.0
+    allx = [c.x for c in comps]
+    ally = [c.y for c in comps]
+    eps = np.finfo(np.float64).eps
+    for a in range(len(comps)):
+        for b in range(len(comps)):
+            R = np.hypot(allx[a][:, None] - allx[b][None, :], ally[a][:, None] - ally[b][None, :])
+            if a == b:
+                np.fill_diagonal(R, 0.0)
+            m = R > eps
+            rmin = min(rmin, float(np.min(R[m])) * 0.95)
+            rmax = max(rmax, float(np.max(R[m])) * 1.05)
+    return rmin, rmax
+
+
+def _hankel_direct(z):
+    """Direct branch of h0_h1_multi_ks_at_r! (bessels:1190-1206): |z|<1e-3
+    small series, else AMOS."""
+    return sp.hankel1(0, z), sp.hankel1(1, z)
+
+
+# --------------------------------------------------------------------------
+# Assembly
+# --------------------------------------------------------------------------
+
+
+def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None):
+    """construct_boundary_matrices! for CFIE_kress (cheb-CFIE:1431-1450 ->
+    _all_k_nosymm_CFIE_chebyshev! :543-665).  Returns [nz, N, N] with
+    A(z) = I - (D + i z S), row = target, column = source."""
+    zs = np.asarray(zs, dtype=np.complex128)
+    rmin, rmax = _rrange_blocks(comps)
+    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+    offs = np.concatenate([[0], np.cumsum([len(c.x) for c in comps])])
+    N = int(offs[-1])
+    out = np.zeros((len(zs), N, N), np.complex128)
+    caches = [geom_cache(c) for c in comps]
+    for q, k in enumerate(zs):
+        if tables is None:
+            H0 = build_table("h", 0, k, rmin, rmax, cp.n_panels_h, cp.M_h)
+            H1 = build_table("h", 1, k, rmin, rmax, cp.n_panels_h, cp.M_h)
+            J0 = build_table("j", 0, k, 0.0, rmax, cp.n_panels_j, cp.M_j)
+            J1 = build_table("j", 1, k, 0.0, rmax, cp.n_panels_j, cp.M_j)
+        else:
+            H0, H1, J0, J1 = tables[q]
+        aL1, aL2 = -k * INV_TWO_PI, 0.5j * k
+        aM1, aM2 = -INV_TWO_PI, 0.5j
+        ik = 1j * k
+        for a, ca in enumerate(comps):
+            for b, cb in enumerate(comps):
+                ro, co = offs[a], offs[b]
+                if a == b:
+                    G = caches[a]
+                    n = len(ca.x)
+                    Rk = kress_R(n)
+                    r = G.R
+                    pidx, th = ph.locate(r)
+                    pjx, tj = pj.locate(r)
+                    h0 = clenshaw(H0[pidx], th)
+                    h1 = clenshaw(H1[pidx], th)
+                    j0 = clenshaw(J0[pjx], tj)
+                    j1 = clenshaw(J1[pjx], tj)
+                    z = k * r
+                    az = np.abs(z)
+                    dm = (az < H_CUT)
+                    if np.any(dm & ~np.eye(n, dtype=bool)):
+                        hh0, hh1 = _hankel_direct(z[dm])
+                        h0[dm], h1[dm] = hh0, hh1
+                    sj = G.speed[None, :]
+                    wj = cb.ws[None, :]
+                    cD = G.inner * G.invR
+                    L1 = aL1 * j1
+                    L2 = aL2 * h1
+                    M1 = aM1 * j0
+                    M2 = aM2 * h0
+                    lt = G.logterm
+                    D = (Rk * cD) * L1 + (wj * cD) * L2 - (wj * lt * cD) * L1
+                    S = (Rk * sj) * M1 + (wj * sj) * M2 - (wj * sj * lt) * M1
+                    A = -(D + ik * S)
+                    # diagonal (cheb-CFIE :585-591)
+                    s = G.speed
+                    w = ca.ws
+                    dval = w * G.kappa
+                    m1 = aM1 * s
+                    m2 = ((0.5j - EULER_OVER_PI) - INV_TWO_PI * np.log((k * k / 4.0) * (s * s))) * s
+                    sval = np.diag(Rk) * m1 + w * m2
+                    A[np.arange(n), np.arange(n)] = 1.0 - (dval + ik * sval)
+                    out[q, ro:ro + n, co:co + n] = A
+                else:
+                    DX = ca.x[:, None] - cb.x[None, :]
+                    DY = ca.y[:, None] - cb.y[None, :]
+                    r = np.hypot(DX, DY)
+                    invr = np.where(r > np.finfo(float).eps, 1.0 / r, 0.0)
+                    inn = cb.dy[None, :] * DX - cb.dx[None, :] * DY
+                    pidx, th = ph.locate(r)
+                    h0 = clenshaw(H0[pidx], th)
+                    h1 = clenshaw(H1[pidx], th)
+                    z = k * r
+                    dm = np.abs(z) < H_CUT
+                    if np.any(dm):
+                        hh0, hh1 = _hankel_direct(z[dm])
+                        h0[dm], h1[dm] = hh0, hh1
+                    sjb = np.sqrt(cb.dx ** 2 + cb.dy ** 2)[None, :]
+                    wj = cb.ws[None, :]
+                    cD = wj * inn * invr
+                    cS = wj * sjb
+                    A = -(cD * aL2 * h1 + ik * (cS * aM2 * h0))
+                    out[q, ro:ro + len(ca.x), co:co + len(cb.x)] = A
+    return out
+
+
+def dlp_assemble(comp: Comp, zs, cp: ChebParams = ChebParams(), tables=None):
+    """construct_boundary_matrices! for DLP_kress (cheb-DLP:1110-1124 ->
+    :702-769 then F = I - D, :1086-1095)."""
+    zs = np.asarray(zs, dtype=np.complex128)
+    G = geom_cache(comp)
+    n = len(comp.x)
+    Rk = kress_R(n)
+    R = G.R.copy()
+    np.fill_diagonal(R, 0.0)
+    m = R > np.finfo(float).eps
+    rmin = float(np.min(R[m])) * 0.95
+    rmax = float(np.max(R[m])) * 1.05
+    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+    pidx, th = ph.locate(G.R)
+    pjx, tj = pj.locate(G.R)
+    out = np.zeros((len(zs), n, n), np.complex128)
+    cD = G.inner * G.invR
+    w = comp.ws
+    for q, k in enumerate(zs):
+        if tables is None:
+            H1 = build_table("h", 1, k, rmin, rmax, cp.n_panels_h, cp.M_h)
+            J1 = build_table("j", 1, k, 0.0, rmax, cp.n_panels_j, cp.M_j)
+        else:
+            H1, J1 = tables[q]
+        h1 = clenshaw(H1[pidx], th)
+        j1 = clenshaw(J1[pjx], tj)
+        l1 = (-k * INV_TWO_PI) * j1
+        ht = (0.5j * k) * h1
+        D = (Rk * cD) * l1 + (w[None, :] * cD) * ht - (w[None, :] * G.logterm * cD) * l1
+        D[np.arange(n), np.arange(n)] = w * G.kappa
+        out[q] = np.eye(n) - D
+    return out
+
+
+def cfie_direct_real(comps, k: float):
+    """Exact-Hankel CFIE at real k (cfie_kress.jl:416-510), one component
+    (the twin of the Chebyshev path, with J_n = real(H_n) for real k)."""
+    assert len(comps) == 1
+    c = comps[0]
+    G = geom_cache(c)
+    n = len(c.x)
+    Rk = kress_R(n)
+    r = G.R
+    h0 = sp.hankel1(0, k * r)
+    h1 = sp.hankel1(1, k * r)
+    j0, j1 = h0.real, h1.real
+    cD = G.inner * G.invR
+    wj = c.ws[None, :]
+    sj = G.speed[None, :]
+    aL1, aL2 = -k * INV_TWO_PI, 0.5j * k
+    D = (Rk * cD) * (aL1 * j1) + (wj * cD) * (aL2 * h1) - (wj * G.logterm * cD) * (aL1 * j1)
+    S = (Rk * sj) * (-INV_TWO_PI * j0) + (wj * sj) * (0.5j * h0) - (wj * sj * G.logterm) * (-INV_TWO_PI * j0)
+    A = -(D + 1j * k * S)
+    s = G.speed
+    dval = c.ws * G.kappa
+    m2 = ((0.5j - EULER_OVER_PI) - INV_TWO_PI * np.log((k * k / 4.0) * (s * s))) * s
+    sval = np.diag(Rk) * (-INV_TWO_PI * s) + c.ws * m2
+    A[np.arange(n), np.arange(n)] = 1.0 - (dval + 1j * k * sval)
+    return A
+
+
+# --------------------------------------------------------------------------
+# Beyn contour loop
+# --------------------------------------------------------------------------
+
+
+def contour(k0: complex, R: float, nq: int):
+    """Nodes and weights (beyn.jl:258-261)."""
+    th = TWO_PI * np.arange(nq) / nq
+    e = np.exp(1j * th)
+    return k0 + R * e, (R / nq) * e
+
+
+def beyn_moments(Ts, zj, wj, V):
+    """LU + solve + axpy loop (beyn.jl:270-285)."""
+    A0 = np.zeros(V.shape, np.complex128)
+    A1 = np.zeros(V.shape, np.complex128)
+    for T, z, w in zip(Ts, zj, wj):
+        lu = sla.lu_factor(T, check_finite=False)
+        X = sla.lu_solve(lu, V, check_finite=False)
+        A0 += w * X
+        A1 += (w * z) * X
+    return A0, A1
+
+
+def beyn_B(A0, A1, svd_tol):
+    """Thin SVD, absolute rank threshold, B = U_k^H A1 W_k Sigma_k^{-1}
+    (beyn.jl:286-321)."""
+    U, S, Wh = np.linalg.svd(A0, full_matrices=False)
+    rk = int(np.count_nonzero(S >= svd_tol))
+    if rk == 0:
+        return np.zeros((0, 0), np.complex128), U[:, :0], S, rk
+    Uk = U[:, :rk]
+    Wk = Wh.conj().T[:, :rk]
+    B = Uk.conj().T @ ((A1 @ Wk) / S[:rk][None, :])
+    return B, Uk, S, rk
+
+
+def beyn_solve(assemble, k0: complex, R: float, nq: int, V: np.ndarray, svd_tol: float):
+    """construct_B_matrix + eigen!(B) (beyn.jl:257-322, :386-397), without the
+    rank-saturation retry.  `assemble(zs)` returns [nq, N, N]."""
+    zj, wj = contour(k0, R, nq)
+    Ts = assemble(zj)
+    A0, A1 = beyn_moments(Ts, zj, wj, V)
+    B, Uk, S, rk = beyn_B(A0, A1, svd_tol)
+    lam = np.linalg.eigvals(B) if rk > 0 else np.zeros(0, np.complex128)
+    return lam, S, rk, A0, A1
+
+
+def jm_zeros_in_disk(k0: float, R: float, mmax: int = 200):
+    """Dirichlet circle eigenvalues (unit radius) inside |k-k0|<=R: zeros of
+    J_m, with multiplicity 2 for m>0."""
+    out = []
+    for m in range(mmax):
+        zs = sp.jn_zeros(m, 60)
+        for z in zs:
+            if abs(z - k0) <= R:
+                out += [z] * (1 if m == 0 else 2)
+        if zs[0] > k0 + R:
+            break
+    return np.sort(np.array(out))
