@@ -1,0 +1,136 @@
+/* C ABI of libqbhip: the MI355X (gfx950) boundary-integral hot path of
+ * QuantumBilliards.jl — DLP-/CFIE-Kress Chebyshev assembly and the Beyn
+ * contour loop.
+ *
+ * Conventions
+ *  - Plain C types only.  Complex numbers are interleaved (re, im) doubles,
+ *    layout-identical to Julia's ComplexF64 and C's double _Complex.
+ *  - Matrices are column-major with leading dimension N (Julia's layout).
+ *  - Every call returns QB_OK (0) or a negative QB_E* code; the message of the
+ *    last failing call on a context is returned by qb_last_error(ctx).
+ *  - A context owns one HIP stream on one device.  Calls on one context must
+ *    not overlap; separate contexts are independent (no global state).
+ */
+#ifndef QB_H
+#define QB_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qb_ctx qb_ctx;
+typedef struct qb_geom qb_geom;
+typedef struct qb_window qb_window;
+
+enum {
+  QB_OK = 0,
+  QB_E_ARG = -1,      /* invalid argument (sizes, pointers, parameters)    */
+  QB_E_HIP = -2,      /* HIP runtime or kernel launch failure              */
+  QB_E_NOMEM = -3,    /* device allocation failed / over the memory budget */
+  QB_E_STATE = -4,    /* object used in the wrong state                    */
+  QB_E_NUMERIC = -5   /* numerical failure reported by a kernel            */
+};
+
+/* operators: the two Kress solver families of the reference */
+enum { QB_CFIE_KRESS = 1, QB_DLP_KRESS = 2 };
+
+/* precision of the Beyn LU/solves */
+enum { QB_FP64 = 0, QB_MIXED = 1 };
+
+/* flags for qb_assemble */
+enum { QB_OUT_DEVICE = 1 };
+
+/* Chebyshev table sizes: the n_panels_h/M_h/n_panels_j/M_j keywords of
+ * construct_boundary_matrices! (chebyshev_euclidian_helmholtz_cfie_kress.jl:1431,
+ * chebyshev_euclidian_helmholtz_dlp_kress.jl:1110). */
+typedef struct {
+  int32_t n_panels_h, M_h, n_panels_j, M_j;
+} qb_cheb_params;
+
+/* Per-window statistics returned by qb_window_create / qb_window_moments. */
+typedef struct {
+  int64_t N;
+  int32_t nq;
+  int32_t r;
+  int32_t precision;
+  int32_t refine_iters_max;     /* max refinement steps over nodes (mixed)   */
+  int32_t nodes_fp64_fallback;  /* nodes refactored in FP64 (mixed)          */
+  double refine_resid_max;      /* max final normwise backward error (mixed) */
+  double t_assemble_ms, t_factor_ms, t_solve_ms;
+  double rmin, rmax;            /* padded distance range of the tables       */
+} qb_stats;
+
+/* Context (one device, one stream).  Replaces no reference function: the
+ * reference runs in one Julia process with no device. */
+int qb_ctx_create(int device, qb_ctx** out);
+void qb_ctx_destroy(qb_ctx* ctx);
+const char* qb_last_error(const qb_ctx* ctx);
+/* Bytes currently held on the device by this context's objects. */
+int64_t qb_ctx_device_bytes(const qb_ctx* ctx);
+
+/* Boundary discretisation, one or more periodic components, holes already
+ * orientation-reversed by the caller.  Replaces the inputs the reference
+ * builds in evaluate_points (cfie.jl:822-832, dlp_kress.jl:563-606) and the
+ * CFIEGeomCache / Kress R it derives from them (boundarypoints.jl:310-365,
+ * :854-884): x, y = xy; dx, dy = tangent; ddx, ddy = tangent_2; ts, ws as in
+ * BoundaryPointsCFIE (boundarypoints.jl:487-502), all components concatenated.
+ * Each component must carry the uniform Kress grid (ts spaced 2pi/N_c). */
+int qb_geom_create(qb_ctx* ctx, int32_t ncomp, const int64_t* n_per_comp,
+                   const double* x, const double* y, const double* dx,
+                   const double* dy, const double* ddx, const double* ddy,
+                   const double* ts, const double* ws, qb_geom** out);
+void qb_geom_destroy(qb_geom* g);
+int64_t qb_geom_size(const qb_geom* g);
+
+/* construct_boundary_matrices!(Tbufs, solver, pts, zj; n_panels_h, M_h,
+ * n_panels_j, M_j) for CFIE_kress (cheb-CFIE:1431-1450) and DLP_kress
+ * (cheb-DLP:1110-1124).  Writes nz dense N x N complex128 matrices
+ * out[q*N*N + i + j*N] = T(z_q)[i,j] (overwrites, like the reference).  With
+ * flags & QB_OUT_DEVICE, out is a device pointer; otherwise host memory.
+ * rminmax (may be NULL) receives the padded table range [rmin, rmax]. */
+int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
+                int64_t nz, const qb_cheb_params* cheb, double* out,
+                int32_t flags, double* rminmax);
+
+/* The first half of construct_B_matrix (beyn.jl:257-277): contour nodes
+ * z_j = k0 + R e^{i 2pi j/nq}, assembly of T(z_j), LU of every node.  The
+ * factors stay resident in the window so the moment solve can be repeated
+ * with more probes (the r -> 2r retry of beyn.jl:291-309).  precision:
+ * QB_FP64 (plain complex128 path) or QB_MIXED (complex64 LU on MFMA + FP64
+ * iterative refinement against the resident FP64 T). */
+int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
+                     double k0_im, double R, int32_t nq,
+                     const qb_cheb_params* cheb, int32_t precision,
+                     qb_window** out, qb_stats* stats);
+void qb_window_destroy(qb_window* w);
+/* Device bytes a window of this size would hold (the admission footprint). */
+int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r);
+
+/* The moment loop of construct_B_matrix (beyn.jl:278-285): X_j = T(z_j)^{-1} V,
+ * A0 = sum_j w_j X_j, A1 = sum_j w_j z_j X_j, w_j = (R/nq) e^{i 2pi j/nq}.
+ * V, A0, A1: host N x r complex128.  refine_tol: relative correction at
+ * which mixed-precision refinement stops (ignored for QB_FP64). */
+int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0,
+                      double* A1, double refine_tol, qb_stats* stats);
+
+/* Device evaluation of H0^(1), H1^(1), J0, J1 at n complex points (the values
+ * the reference takes from SpecialFunctions.besselh/besselj when sampling its
+ * tables, chebyshev_bessels.jl:331-372).  Host arrays of n complex each. */
+int qb_bessel_eval(qb_ctx* ctx, const double* z, int64_t n, double* h0,
+                   double* h1, double* j0, double* j1);
+
+/* Table-size autotuning of chebyshev_params (chebyshev_optimal_panelization.jl:221-262):
+ * grow panels x1.5 (or M += 2 every 5th iteration) until the max abs error of
+ * every H0/H1/J0/J1 table over `samples` uniform radii is below tol.
+ * cheb holds the initial sizes and receives the chosen ones; errs (may be
+ * NULL) receives the 4 worst errors.  Returns QB_OK also when max_iter is
+ * reached without meeting tol (best effort, like the reference). */
+int qb_cheb_tune(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
+                 int64_t nz, double tol, int32_t samples, int32_t max_iter,
+                 qb_cheb_params* cheb, double* errs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

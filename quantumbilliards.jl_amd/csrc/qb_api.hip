@@ -1,0 +1,837 @@
+// C ABI of libqbhip (declared in include/qb.h): contexts, resident geometry,
+// construct_boundary_matrices!, and the Beyn window (assembly + LU of every
+// contour node, then the moment solves).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+#include "qb_internal.h"
+#include "qb_linalg.h"
+
+// ---------------------------------------------------------------------------
+// errors and allocation
+// ---------------------------------------------------------------------------
+int qb_fail(qb_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int qb_dev_alloc(qb_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return QB_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return qb_fail(ctx, QB_E_NOMEM,
+                   "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                       hipGetErrorString(e));
+  }
+  ctx->bytes += (int64_t)bytes;
+  return QB_OK;
+}
+
+void qb_dev_free(qb_ctx* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  ctx->bytes -= (int64_t)bytes;
+}
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+extern "C" {
+
+int qb_ctx_create(int device, qb_ctx** out) {
+  if (!out) return QB_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return QB_E_HIP;
+  if (device < 0 || device >= n) return QB_E_ARG;
+  if (hipSetDevice(device) != hipSuccess) return QB_E_HIP;
+  qb_ctx* c = new qb_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return QB_E_HIP;
+  }
+  *out = c;
+  return QB_OK;
+}
+
+void qb_ctx_destroy(qb_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* qb_last_error(const qb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int64_t qb_ctx_device_bytes(const qb_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+// Kress R first column and log term per component: one thread per (comp, d).
+// Even N_c = 2n: R_d = -(4 pi / N_c) sum_{m=1}^{n-1} cos(2 pi m d / N_c) / m
+//                      - (4 pi / N_c^2) (-1)^d;
+// odd N_c = 2n+1: no Nyquist term, m = 1..n (boundarypoints.jl:310-344).
+// lt_d = log(4 sin^2(pi d / N_c)), lt_0 = 0 (boundarypoints.jl:878-879).
+__global__ void k_kress_cols(int64_t N, int ncomp, DevGeom g, double* rk, double* lt) {
+  int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= N) return;
+  int c = 0;
+  while (c + 1 < ncomp && gi >= g.off[c + 1]) ++c;
+  int64_t o = g.off[c], nc = g.off[c + 1] - o;
+  int64_t d = gi - o;
+  int64_t mmax = (nc % 2 == 0) ? nc / 2 - 1 : (nc - 1) / 2;
+  double acc = 0.0;
+  for (int64_t m = 1; m <= mmax; ++m) {
+    int64_t md = (m * d) % nc;
+    acc += cospi(2.0 * (double)md / (double)nc) / (double)m;
+  }
+  const double two_pi = 6.283185307179586476925;
+  double R = -(2.0 * two_pi / (double)nc) * acc;
+  if (nc % 2 == 0) R -= (2.0 * two_pi / ((double)nc * (double)nc)) * ((d & 1) ? -1.0 : 1.0);
+  rk[gi] = R;
+  if (d == 0) {
+    lt[gi] = 0.0;
+  } else {
+    double s = sinpi((double)d / (double)nc);
+    lt[gi] = log(4.0 * s * s);
+  }
+}
+
+// Padded range of pairwise distances (cheb-CFIE:374-418, cheb-DLP:135-147):
+// same-component pairs skip i == j, all pairs skip r <= eps.
+__global__ void k_rrange(DevGeom g, int only_first, unsigned long long* mm) {
+  int64_t N = only_first ? g.off[1] : g.N;
+  int64_t j = blockIdx.y;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double rmn = 1e300, rmx = 0.0;
+  if (i < N && i != j) {
+    double r = hypot(g.x[i] - g.x[j], g.y[i] - g.y[j]);
+    if (r > 2.220446049250313e-16) {
+      rmn = r;
+      rmx = r;
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    rmn = fmin(rmn, __shfl_xor(rmn, off));
+    rmx = fmax(rmx, __shfl_xor(rmx, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMin(&mm[0], (unsigned long long)__double_as_longlong(rmn));
+    atomicMax(&mm[1], (unsigned long long)__double_as_longlong(rmx));
+  }
+}
+
+static double bits2d(unsigned long long b) {
+  double v;
+  memcpy(&v, &b, sizeof(v));
+  return v;
+}
+
+static int rrange(qb_ctx* ctx, const DevGeom& d, int only_first, double* rmin, double* rmax) {
+  unsigned long long* mm = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&mm, 2 * sizeof(unsigned long long));
+  if (rc) return rc;
+  double big = 1e300;
+  unsigned long long init[2];
+  memcpy(&init[0], &big, 8);
+  init[1] = 0ull;
+  QB_HIP(ctx, hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  int64_t N = only_first ? d.off[1] : d.N;
+  dim3 grid((unsigned)((N + 255) / 256), (unsigned)N);
+  hipLaunchKernelGGL(k_rrange, grid, dim3(256), 0, ctx->stream, d, only_first, mm);
+  QB_HIP(ctx, hipGetLastError());
+  unsigned long long h[2];
+  QB_HIP(ctx, hipMemcpyAsync(h, mm, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  QB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  qb_dev_free(ctx, mm, 2 * sizeof(unsigned long long));
+  *rmin = 0.95 * bits2d(h[0]);
+  *rmax = 1.05 * bits2d(h[1]);
+  return QB_OK;
+}
+
+extern "C" int qb_geom_create(qb_ctx* ctx, int32_t ncomp, const int64_t* n_per_comp,
+                              const double* x, const double* y, const double* dx, const double* dy,
+                              const double* ddx, const double* ddy, const double* ts,
+                              const double* ws, qb_geom** out) {
+  if (!ctx || !out) return QB_E_ARG;
+  *out = nullptr;
+  if (ncomp < 1 || ncomp > QB_MAX_COMP)
+    return qb_fail(ctx, QB_E_ARG, "ncomp must be in [1, 8]");
+  if (!n_per_comp || !x || !y || !dx || !dy || !ddx || !ddy || !ts || !ws)
+    return qb_fail(ctx, QB_E_ARG, "null geometry array");
+  int64_t N = 0;
+  for (int c = 0; c < ncomp; ++c) {
+    if (n_per_comp[c] < 4) return qb_fail(ctx, QB_E_ARG, "each component needs >= 4 nodes");
+    N += n_per_comp[c];
+  }
+  if (N > 65535) return qb_fail(ctx, QB_E_ARG, "N must be <= 65535");
+  // uniform Kress grid per component (ts spaced +-2pi/N_c)
+  int64_t o = 0;
+  for (int c = 0; c < ncomp; ++c) {
+    int64_t nc = n_per_comp[c];
+    double h = 2.0 * M_PI / (double)nc;
+    for (int64_t i = 0; i + 1 < nc; ++i) {
+      double d = fabs(ts[o + i + 1] - ts[o + i]);
+      if (fabs(d - h) > 1e-9 * (1.0 + h))
+        return qb_fail(ctx, QB_E_ARG,
+                       "component " + std::to_string(c) +
+                           ": ts is not the uniform Kress grid (graded/corner variants are not "
+                           "supported)");
+    }
+    o += nc;
+  }
+  (void)hipSetDevice(ctx->device);
+  qb_geom* g = new qb_geom();
+  g->ctx = ctx;
+  // host-side per-node quantities (O(N))
+  std::vector<double> speed(N), kappa(N);
+  std::vector<int32_t> comp(N);
+  o = 0;
+  for (int c = 0; c < ncomp; ++c) {
+    for (int64_t i = 0; i < n_per_comp[c]; ++i) comp[o + i] = c;
+    o += n_per_comp[c];
+  }
+  for (int64_t i = 0; i < N; ++i) {
+    speed[i] = sqrt(dx[i] * dx[i] + dy[i] * dy[i]);
+    double num = -(dx[i] * ddy[i] - dy[i] * ddx[i]);
+    double den = dx[i] * dx[i] + dy[i] * dy[i];
+    kappa[i] = (1.0 / (2.0 * M_PI)) * (num / den);
+  }
+  // one allocation: 9 double arrays + int32 comp
+  size_t nd = 9 * (size_t)N;
+  size_t bytes = nd * sizeof(double) + (size_t)N * sizeof(int32_t);
+  int rc = qb_dev_alloc(ctx, &g->buf, bytes);
+  if (rc) {
+    delete g;
+    return rc;
+  }
+  g->bytes = (int64_t)bytes;
+  double* b = (double*)g->buf;
+  double* dX = b;
+  double* dY = b + N;
+  double* dDX = b + 2 * N;
+  double* dDY = b + 3 * N;
+  double* dSp = b + 4 * N;
+  double* dW = b + 5 * N;
+  double* dK = b + 6 * N;
+  double* dRk = b + 7 * N;
+  double* dLt = b + 8 * N;
+  int32_t* dC = (int32_t*)(b + 9 * N);
+  auto up = [&](double* d, const double* h) {
+    return hipMemcpyAsync(d, h, N * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  };
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) e = up(dX, x);
+  if (e == hipSuccess) e = up(dY, y);
+  if (e == hipSuccess) e = up(dDX, dx);
+  if (e == hipSuccess) e = up(dDY, dy);
+  if (e == hipSuccess) e = up(dSp, speed.data());
+  if (e == hipSuccess) e = up(dW, ws);
+  if (e == hipSuccess) e = up(dK, kappa.data());
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dC, comp.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) {
+    qb_geom_destroy(g);
+    return qb_fail(ctx, QB_E_HIP, std::string("geometry upload: ") + hipGetErrorString(e));
+  }
+  DevGeom& d = g->d;
+  d.N = N;
+  d.ncomp = ncomp;
+  d.off[0] = 0;
+  for (int c = 0; c < ncomp; ++c) d.off[c + 1] = d.off[c] + n_per_comp[c];
+  for (int c = ncomp + 1; c <= QB_MAX_COMP; ++c) d.off[c] = N;
+  d.x = dX;
+  d.y = dY;
+  d.dx = dDX;
+  d.dy = dDY;
+  d.speed = dSp;
+  d.ws = dW;
+  d.kappa = dK;
+  d.comp = dC;
+  d.rk = dRk;
+  d.lt = dLt;
+  hipLaunchKernelGGL(k_kress_cols, dim3((unsigned)((N + 127) / 128)), dim3(128), 0, ctx->stream,
+                     N, ncomp, d, dRk, dLt);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    qb_geom_destroy(g);
+    return qb_fail(ctx, QB_E_HIP, std::string("kress columns: ") + hipGetErrorString(e));
+  }
+  rc = rrange(ctx, d, 0, &g->rmin_geom, &g->rmax_geom);
+  if (!rc) rc = rrange(ctx, d, 1, &g->rmin_dlp, &g->rmax_dlp);
+  if (rc) {
+    qb_geom_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return QB_OK;
+}
+
+extern "C" void qb_geom_destroy(qb_geom* g) {
+  if (!g) return;
+  qb_dev_free(g->ctx, g->buf, (size_t)g->bytes);
+  delete g;
+}
+
+extern "C" int64_t qb_geom_size(const qb_geom* g) { return g ? g->d.N : 0; }
+
+// ---------------------------------------------------------------------------
+// assembly
+// ---------------------------------------------------------------------------
+static void op_range(const qb_geom* g, int op, double* rmin, double* rmax) {
+  if (op == QB_DLP_KRESS) {
+    *rmin = g->rmin_dlp;
+    *rmax = g->rmax_dlp;
+  } else {
+    *rmin = g->rmin_geom;
+    *rmax = g->rmax_geom;
+  }
+}
+
+static int check_cheb(qb_ctx* ctx, const qb_cheb_params* c) {
+  if (!c) return qb_fail(ctx, QB_E_ARG, "null Chebyshev parameters");
+  if (c->n_panels_h < 1 || c->n_panels_j < 1 || c->M_h < 2 || c->M_j < 2 || c->M_h > 16 ||
+      c->M_j > 16)
+    return qb_fail(ctx, QB_E_ARG, "Chebyshev parameters out of range (panels >= 1, 2 <= M <= 16)");
+  return QB_OK;
+}
+
+extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z, int64_t nz,
+                           const qb_cheb_params* cheb, double* out, int32_t flags,
+                           double* rminmax) {
+  if (!ctx) return QB_E_ARG;
+  if (!g || !z || !out || nz < 0) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1)
+    return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  int rc = check_cheb(ctx, cheb);
+  if (rc) return rc;
+  if (nz == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  double rmin, rmax;
+  op_range(g, op, &rmin, &rmax);
+  if (rminmax) {
+    rminmax[0] = rmin;
+    rminmax[1] = rmax;
+  }
+  int64_t N = g->d.N;
+  cd* zd = nullptr;
+  rc = qb_dev_alloc(ctx, (void**)&zd, nz * sizeof(cd));
+  if (rc) return rc;
+  DevTables t;
+  cd* od = nullptr;
+  size_t obytes = (size_t)nz * N * N * sizeof(cd);
+  bool on_dev = (flags & QB_OUT_DEVICE) != 0;
+  hipError_t e = hipMemcpyAsync(zd, z, nz * sizeof(cd), hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  if (!rc) rc = qb_tables_build(ctx, (const double*)zd, nz, rmin, rmax, *cheb, true, &t);
+  if (!rc) {
+    if (on_dev)
+      od = (cd*)out;
+    else
+      rc = qb_dev_alloc(ctx, (void**)&od, obytes);
+  }
+  if (!rc) rc = qb_assemble_dev(ctx, g, op, zd, nz, t, od, nullptr);
+  if (!rc && !on_dev) {
+    e = hipMemcpyAsync(out, od, obytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  }
+  if (!rc) {
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  } else {
+    (void)hipStreamSynchronize(ctx->stream);
+  }
+  if (!on_dev && od) qb_dev_free(ctx, od, obytes);
+  qb_tables_free(ctx, &t);
+  qb_dev_free(ctx, zd, nz * sizeof(cd));
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Beyn window
+// ---------------------------------------------------------------------------
+struct qb_window {
+  qb_ctx* ctx = nullptr;
+  const qb_geom* g = nullptr;
+  int op = 0, nq = 0, precision = 0;
+  int64_t N = 0;
+  double k0re = 0, k0im = 0, R = 0;
+  std::vector<cd> zj, wj, wzj;  // host copies
+  cd* z_dev = nullptr;
+  cd* w_dev = nullptr;   // [w_j | w_j z_j]
+  cd* T64 = nullptr;     // FP64 factors (QB_FP64) or the FP64 matrices (QB_MIXED)
+  cf* LU32 = nullptr;    // complex64 factors (QB_MIXED)
+  int* ipiv = nullptr;   // [nq][N]
+  int* perm = nullptr;   // [nq][N]
+  int* info = nullptr;   // [nq]
+  double* tnorm = nullptr;  // [nq] ||T_j||_inf (QB_MIXED)
+  double* scratch_d = nullptr;  // [3 nq] per-node norms
+  std::vector<int> fb_nodes;    // nodes refactored in FP64
+  cd* fb_lu = nullptr;          // [fb][N][N]
+  int* fb_ipiv = nullptr;
+  int* fb_perm = nullptr;
+  size_t fb_cap = 0;
+  qb_stats st{};
+};
+
+static size_t win_bytes_core(int64_t N, int nq, int precision) {
+  size_t NN = (size_t)N * N;
+  size_t b = (size_t)nq * NN * sizeof(cd);
+  if (precision == QB_MIXED) b += (size_t)nq * NN * sizeof(cf);
+  b += 2 * (size_t)nq * N * sizeof(int) + (size_t)nq * sizeof(int);
+  return b;
+}
+
+extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r) {
+  size_t b = win_bytes_core(N, nq, precision);
+  size_t nv = (size_t)N * r;
+  // moment work buffers: X and R (complex128) per node, a complex64 copy
+  // per node in mixed mode, V, A0, A1
+  b += 2 * (size_t)nq * nv * sizeof(cd) + 3 * nv * sizeof(cd);
+  if (precision == QB_MIXED) b += (size_t)nq * nv * sizeof(cf);
+  return (int64_t)b;
+}
+
+static void win_free(qb_window* w) {
+  qb_ctx* ctx = w->ctx;
+  size_t NN = (size_t)w->N * w->N;
+  qb_dev_free(ctx, w->z_dev, w->nq * sizeof(cd));
+  qb_dev_free(ctx, w->w_dev, 2 * w->nq * sizeof(cd));
+  qb_dev_free(ctx, w->T64, (size_t)w->nq * NN * sizeof(cd));
+  qb_dev_free(ctx, w->LU32, (size_t)w->nq * NN * sizeof(cf));
+  qb_dev_free(ctx, w->ipiv, (size_t)w->nq * w->N * sizeof(int));
+  qb_dev_free(ctx, w->perm, (size_t)w->nq * w->N * sizeof(int));
+  qb_dev_free(ctx, w->info, w->nq * sizeof(int));
+  qb_dev_free(ctx, w->tnorm, w->nq * sizeof(double));
+  qb_dev_free(ctx, w->scratch_d, 3 * w->nq * sizeof(double));
+  qb_dev_free(ctx, w->fb_lu, w->fb_cap * NN * sizeof(cd));
+  qb_dev_free(ctx, w->fb_ipiv, w->fb_cap * w->N * sizeof(int));
+  qb_dev_free(ctx, w->fb_perm, w->fb_cap * w->N * sizeof(int));
+}
+
+extern "C" void qb_window_destroy(qb_window* w) {
+  if (!w) return;
+  (void)hipSetDevice(w->ctx->device);
+  (void)hipStreamSynchronize(w->ctx->stream);
+  win_free(w);
+  delete w;
+}
+
+#define QB_TRY(x)        \
+  do {                   \
+    int _rc = (x);       \
+    if (_rc) goto fail;  \
+  } while (0)
+
+extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
+                                double k0_im, double R, int32_t nq, const qb_cheb_params* cheb,
+                                int32_t precision, qb_window** out, qb_stats* stats) {
+  if (!ctx) return QB_E_ARG;
+  if (!g || !out) return qb_fail(ctx, QB_E_ARG, "null argument");
+  *out = nullptr;
+  if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1)
+    return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  if (nq < 1 || nq > 4096) return qb_fail(ctx, QB_E_ARG, "nq must be in [1, 4096]");
+  if (!(R > 0.0)) return qb_fail(ctx, QB_E_ARG, "contour radius must be > 0");
+  if (precision != QB_FP64 && precision != QB_MIXED)
+    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64 or QB_MIXED");
+  int rc = check_cheb(ctx, cheb);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  qb_window* w = new qb_window();
+  w->ctx = ctx;
+  w->g = g;
+  w->op = op;
+  w->nq = nq;
+  w->precision = precision;
+  w->N = g->d.N;
+  w->k0re = k0_re;
+  w->k0im = k0_im;
+  w->R = R;
+  const int64_t N = w->N;
+  const size_t NN = (size_t)N * N;
+  double t0 = now_ms(), t1, t2;
+  DevTables tab;
+  std::vector<cd> wbuf(2 * nq);
+  double rmin, rmax;
+  op_range(g, op, &rmin, &rmax);
+  // contour nodes and weights (beyn.jl:258-261)
+  w->zj.resize(nq);
+  w->wj.resize(nq);
+  w->wzj.resize(nq);
+  for (int j = 0; j < nq; ++j) {
+    long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)nq;
+    double c = (double)cosl(th), s = (double)sinl(th);
+    cd e = mk(c, s);
+    w->zj[j] = mk(k0_re + R * c, k0_im + R * s);
+    w->wj[j] = (R / (double)nq) * e;
+    w->wzj[j] = w->wj[j] * w->zj[j];
+    wbuf[j] = w->wj[j];
+    wbuf[nq + j] = w->wzj[j];
+  }
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->z_dev, nq * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->w_dev, 2 * nq * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->T64, (size_t)nq * NN * sizeof(cd)));
+  if (precision == QB_MIXED) {
+    QB_TRY(qb_dev_alloc(ctx, (void**)&w->LU32, (size_t)nq * NN * sizeof(cf)));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&w->tnorm, nq * sizeof(double)));
+  }
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->ipiv, (size_t)nq * N * sizeof(int)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->perm, (size_t)nq * N * sizeof(int)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->info, nq * sizeof(int)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->scratch_d, 3 * nq * sizeof(double)));
+  if (hipMemcpyAsync(w->z_dev, w->zj.data(), nq * sizeof(cd), hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(w->w_dev, wbuf.data(), 2 * nq * sizeof(cd), hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess ||
+      hipMemsetAsync(w->info, 0, nq * sizeof(int), ctx->stream) != hipSuccess) {
+    rc = qb_fail(ctx, QB_E_HIP, "window upload failed");
+    goto fail;
+  }
+  // tables + assembly (construct_boundary_matrices!, beyn.jl:264)
+  QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nq, rmin, rmax, *cheb, true, &tab));
+  QB_TRY(qb_assemble_dev(ctx, g, op, w->z_dev, nq, tab, w->T64,
+                         precision == QB_MIXED ? w->LU32 : nullptr));
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    rc = qb_fail(ctx, QB_E_HIP, "assembly failed");
+    goto fail;
+  }
+  qb_tables_free(ctx, &tab);
+  t1 = now_ms();
+  // LU of every node (beyn.jl:270-277)
+  if (precision == QB_FP64) {
+    QB_TRY(qb_getrf_batched<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
+  } else {
+    QB_TRY(qb_node_rownorm(ctx, w->T64, N, nq, w->tnorm));
+    QB_TRY(qb_getrf_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
+  }
+  QB_TRY(qb_perm_from_ipiv(ctx, w->ipiv, w->perm, (int)N, nq));
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    rc = qb_fail(ctx, QB_E_HIP, "factorisation failed");
+    goto fail;
+  }
+  t2 = now_ms();
+  w->st.N = N;
+  w->st.nq = nq;
+  w->st.precision = precision;
+  w->st.t_assemble_ms = t1 - t0;
+  w->st.t_factor_ms = t2 - t1;
+  w->st.rmin = rmin;
+  w->st.rmax = rmax;
+  if (stats) *stats = w->st;
+  *out = w;
+  return QB_OK;
+fail:
+  (void)hipStreamSynchronize(ctx->stream);
+  qb_tables_free(ctx, &tab);
+  if (rc == QB_OK) rc = QB_E_HIP;
+  win_free(w);
+  delete w;
+  return rc;
+}
+
+// FP64 refactorisation of the nodes listed in w->fb_nodes that are not yet
+// factored (mixed-precision fallback when refinement does not converge).
+static int win_fp64_fallback(qb_window* w, const std::vector<int>& nodes) {
+  qb_ctx* ctx = w->ctx;
+  const int64_t N = w->N;
+  const size_t NN = (size_t)N * N;
+  size_t need = w->fb_nodes.size() + nodes.size();
+  if (need > w->fb_cap) {
+    size_t cap = need;
+    cd* lu = nullptr;
+    int *ip = nullptr, *pm = nullptr;
+    int rc = qb_dev_alloc(ctx, (void**)&lu, cap * NN * sizeof(cd));
+    if (!rc) rc = qb_dev_alloc(ctx, (void**)&ip, cap * N * sizeof(int));
+    if (!rc) rc = qb_dev_alloc(ctx, (void**)&pm, cap * N * sizeof(int));
+    if (rc) {
+      qb_dev_free(ctx, lu, cap * NN * sizeof(cd));
+      qb_dev_free(ctx, ip, cap * N * sizeof(int));
+      return rc;
+    }
+    if (w->fb_cap) {
+      size_t old = w->fb_nodes.size();
+      QB_HIP(ctx, hipMemcpyAsync(lu, w->fb_lu, old * NN * sizeof(cd), hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+      QB_HIP(ctx, hipMemcpyAsync(ip, w->fb_ipiv, old * N * sizeof(int), hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+      QB_HIP(ctx, hipMemcpyAsync(pm, w->fb_perm, old * N * sizeof(int), hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+      QB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      qb_dev_free(ctx, w->fb_lu, w->fb_cap * NN * sizeof(cd));
+      qb_dev_free(ctx, w->fb_ipiv, w->fb_cap * N * sizeof(int));
+      qb_dev_free(ctx, w->fb_perm, w->fb_cap * N * sizeof(int));
+    }
+    w->fb_lu = lu;
+    w->fb_ipiv = ip;
+    w->fb_perm = pm;
+    w->fb_cap = cap;
+  }
+  for (int node : nodes) {
+    size_t slot = w->fb_nodes.size();
+    QB_HIP(ctx, hipMemcpyAsync(w->fb_lu + slot * NN, w->T64 + (size_t)node * NN, NN * sizeof(cd),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    int rc = qb_getrf_batched<cd>(ctx, w->fb_lu + slot * NN, (int)N, N, (int64_t)NN,
+                                  w->fb_ipiv + slot * N, N, nullptr, 1);
+    if (rc) return rc;
+    rc = qb_perm_from_ipiv(ctx, w->fb_ipiv + slot * N, w->fb_perm + slot * N, (int)N, 1);
+    if (rc) return rc;
+    w->fb_nodes.push_back(node);
+  }
+  return QB_OK;
+}
+
+extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0, double* A1,
+                                 double refine_tol, qb_stats* stats) {
+  if (!w) return QB_E_ARG;
+  qb_ctx* ctx = w->ctx;
+  if (!V || !A0 || !A1) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (r < 1 || r > w->N) return qb_fail(ctx, QB_E_ARG, "r must be in [1, N]");
+  (void)hipSetDevice(ctx->device);
+  const int64_t N = w->N;
+  const int nq = w->nq;
+  const size_t NN = (size_t)N * N;
+  const int64_t nv = N * (int64_t)r;
+  double t0 = now_ms();
+  cd *Vd = nullptr, *X = nullptr, *Rr = nullptr, *A0d = nullptr, *A1d = nullptr;
+  cf* D32 = nullptr;
+  int rc = QB_OK;
+  int iters_max = 0;
+  double resid_max = 0.0;
+  std::vector<double> h3(3 * nq);
+  QB_TRY(qb_dev_alloc(ctx, (void**)&Vd, nv * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&X, (size_t)nq * nv * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, nv * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, nv * sizeof(cd)));
+  if (hipMemcpyAsync(Vd, V, nv * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+    rc = qb_fail(ctx, QB_E_HIP, "V upload failed");
+    goto fail;
+  }
+  if (w->precision == QB_FP64) {
+    // X_j = P_j V, then the two triangular solves (ldiv!, beyn.jl:281)
+    QB_TRY(qb_gather_rows_cd(ctx, Vd, 0, w->perm, X, (int)N, r, nq));
+    QB_TRY(qb_trsm_lu_batched<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, X, r, N, nv, nq));
+  } else {
+    // complex64 solve + FP64 iterative refinement against the resident T64
+    QB_TRY(qb_dev_alloc(ctx, (void**)&Rr, (size_t)nq * nv * sizeof(cd)));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&D32, (size_t)nq * nv * sizeof(cf)));
+    if (hipMemsetAsync(X, 0, (size_t)nq * nv * sizeof(cd), ctx->stream) != hipSuccess) {
+      rc = qb_fail(ctx, QB_E_HIP, "memset failed");
+      goto fail;
+    }
+    QB_TRY(qb_bcast(ctx, Vd, Rr, nv, nq));
+    std::vector<char> done(nq, 0);
+    std::vector<double> prev_corr(nq, 1e300);
+    std::vector<int> stagnant;
+    const int maxit = 8;
+    int it = 0;
+    for (; it < maxit; ++it) {
+      QB_TRY(qb_gather_rows_cd2cf(ctx, Rr, nv, w->perm, D32, (int)N, r, nq));
+      QB_TRY(qb_trsm_lu_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, D32, r, N, nv, nq));
+      QB_TRY(qb_add_cf_to_cd(ctx, D32, X, (int64_t)nq * nv));
+      QB_TRY(qb_node_maxabs32(ctx, D32, nv, nq, w->scratch_d));
+      QB_TRY(qb_node_maxabs(ctx, X, nv, nq, w->scratch_d + nq));
+      // R = V - T X  (FP64)
+      QB_TRY(qb_bcast(ctx, Vd, Rr, nv, nq));
+      QB_TRY(gemm_sub<cd>(ctx, (int)N, r, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N, nv,
+                          nq));
+      QB_TRY(qb_node_maxabs(ctx, Rr, nv, nq, w->scratch_d + 2 * nq));
+      if (hipMemcpyAsync(h3.data(), w->scratch_d, 3 * nq * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        rc = qb_fail(ctx, QB_E_HIP, "refinement norms failed");
+        goto fail;
+      }
+      bool all = true;
+      for (int j = 0; j < nq; ++j) {
+        double corr = h3[j] / (h3[nq + j] > 0 ? h3[nq + j] : 1.0);
+        if (!(corr <= refine_tol)) {
+          all = false;
+          if (it >= 2 && !(corr < 0.5 * prev_corr[j])) done[j] = 2;  // stagnating
+        } else if (done[j] == 0) {
+          done[j] = 1;
+        }
+        prev_corr[j] = corr;
+      }
+      if (all) break;
+      bool any_active = false;
+      for (int j = 0; j < nq; ++j)
+        if (done[j] == 0) any_active = true;
+      if (!any_active) break;
+    }
+    iters_max = it + 1 > maxit ? maxit : it + 1;
+    // nodes that did not converge: FP64 refactorisation and solve
+    for (int j = 0; j < nq; ++j) {
+      double corr = h3[j] / (h3[nq + j] > 0 ? h3[nq + j] : 1.0);
+      if (!(corr <= refine_tol)) stagnant.push_back(j);
+    }
+    if (!stagnant.empty()) {
+      std::vector<int> fresh;
+      for (int j : stagnant)
+        if (std::find(w->fb_nodes.begin(), w->fb_nodes.end(), j) == w->fb_nodes.end())
+          fresh.push_back(j);
+      QB_TRY(win_fp64_fallback(w, fresh));
+      for (int j : stagnant) {
+        size_t slot = std::find(w->fb_nodes.begin(), w->fb_nodes.end(), j) - w->fb_nodes.begin();
+        cd* Xj = X + (size_t)j * nv;
+        QB_TRY(qb_gather_rows_cd(ctx, Vd, 0, w->fb_perm + slot * N, Xj, (int)N, r, 1));
+        QB_TRY(qb_trsm_lu_batched<cd>(ctx, w->fb_lu + slot * NN, (int)N, N, (int64_t)NN, Xj, r,
+                                      N, nv, 1));
+      }
+    }
+    {
+      std::vector<double> tn(nq);
+      if (hipMemcpyAsync(tn.data(), w->tnorm, nq * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        rc = qb_fail(ctx, QB_E_HIP, "norm download failed");
+        goto fail;
+      }
+      for (int j = 0; j < nq; ++j) {
+        if (std::find(stagnant.begin(), stagnant.end(), j) != stagnant.end()) continue;
+        double be = h3[2 * nq + j] / (tn[j] * h3[nq + j] + 1e-300);
+        resid_max = std::max(resid_max, be);
+      }
+    }
+    w->st.nodes_fp64_fallback = (int)w->fb_nodes.size();
+  }
+  QB_TRY(qb_moments(ctx, X, nv, nq, w->w_dev, w->w_dev + nq, A0d, A1d));
+  if (hipMemcpyAsync(A0, A0d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipMemcpyAsync(A1, A1d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    rc = qb_fail(ctx, QB_E_HIP, "moment download failed");
+    goto fail;
+  }
+  w->st.r = r;
+  w->st.refine_iters_max = iters_max;
+  w->st.refine_resid_max = resid_max;
+  w->st.t_solve_ms = now_ms() - t0;
+  if (stats) *stats = w->st;
+fail:
+  (void)hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, Vd, nv * sizeof(cd));
+  qb_dev_free(ctx, X, (size_t)nq * nv * sizeof(cd));
+  qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));
+  qb_dev_free(ctx, D32, (size_t)nq * nv * sizeof(cf));
+  qb_dev_free(ctx, A0d, nv * sizeof(cd));
+  qb_dev_free(ctx, A1d, nv * sizeof(cd));
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Bessel evaluation and table tuning
+// ---------------------------------------------------------------------------
+extern "C" int qb_bessel_eval(qb_ctx* ctx, const double* z, int64_t n, double* h0, double* h1,
+                              double* j0, double* j1) {
+  if (!ctx) return QB_E_ARG;
+  if (n < 0 || (n > 0 && (!z || !h0 || !h1 || !j0 || !j1)))
+    return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (n == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  cd *zd = nullptr, *od = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&zd, n * sizeof(cd));
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&od, 4 * n * sizeof(cd));
+  std::vector<cd> ho;
+  if (!rc && hipMemcpyAsync(zd, z, n * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    rc = qb_fail(ctx, QB_E_HIP, "upload failed");
+  if (!rc) rc = qb_bessel_eval_dev(ctx, zd, n, od);
+  if (!rc) {
+    ho.resize(4 * n);
+    if (hipMemcpyAsync(ho.data(), od, 4 * n * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = qb_fail(ctx, QB_E_HIP, "download failed");
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  if (!rc) {
+    for (int64_t i = 0; i < n; ++i) {
+      h0[2 * i] = ho[4 * i].re;
+      h0[2 * i + 1] = ho[4 * i].im;
+      h1[2 * i] = ho[4 * i + 1].re;
+      h1[2 * i + 1] = ho[4 * i + 1].im;
+      j0[2 * i] = ho[4 * i + 2].re;
+      j0[2 * i + 1] = ho[4 * i + 2].im;
+      j1[2 * i] = ho[4 * i + 3].re;
+      j1[2 * i + 1] = ho[4 * i + 3].im;
+    }
+  }
+  qb_dev_free(ctx, zd, n * sizeof(cd));
+  qb_dev_free(ctx, od, 4 * n * sizeof(cd));
+  return rc;
+}
+
+extern "C" int qb_cheb_tune(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
+                            int64_t nz, double tol, int32_t samples, int32_t max_iter,
+                            qb_cheb_params* cheb, double* errs) {
+  if (!ctx) return QB_E_ARG;
+  if (!g || !z || !cheb || nz < 1) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (samples < 2 || max_iter < 1 || !(tol > 0))
+    return qb_fail(ctx, QB_E_ARG, "need samples >= 2, max_iter >= 1, tol > 0");
+  int rc = check_cheb(ctx, cheb);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  // rmin_cheb = min_j 0.2/|z_j| (optimal_panelization.jl:222)
+  double rmin_cheb = 1e300;
+  for (int64_t q = 0; q < nz; ++q) rmin_cheb = std::min(rmin_cheb, 0.2 / hypot(z[2 * q], z[2 * q + 1]));
+  double rmin, rmax;
+  op_range(g, op, &rmin, &rmax);
+  double rmin_h = std::max(rmin_cheb, rmin);
+  cd* zd = nullptr;
+  rc = qb_dev_alloc(ctx, (void**)&zd, nz * sizeof(cd));
+  if (rc) return rc;
+  if (hipMemcpyAsync(zd, z, nz * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+    qb_dev_free(ctx, zd, nz * sizeof(cd));
+    return qb_fail(ctx, QB_E_HIP, "upload failed");
+  }
+  qb_cheb_params cp = *cheb;
+  double e4[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  for (int it = 1; it <= max_iter; ++it) {
+    if (cp.M_h > 16 || cp.M_j > 16) break;
+    DevTables t;
+    rc = qb_tables_build(ctx, (const double*)zd, nz, rmin_h, rmax, cp, true, &t);
+    if (!rc) rc = qb_cheb_check_dev(ctx, t, zd, nz, samples, e4);
+    (void)hipStreamSynchronize(ctx->stream);
+    qb_tables_free(ctx, &t);
+    if (rc) break;
+    *cheb = cp;
+    bool okH = e4[0] < tol && e4[1] < tol;
+    bool okJ = e4[2] < tol && e4[3] < tol;
+    if (okH && okJ) break;
+    if (it == max_iter) break;
+    if (!okH) {
+      if (it % 5 == 0)
+        cp.M_h += 2;
+      else
+        cp.n_panels_h = (int32_t)ceil(1.5 * cp.n_panels_h);
+    }
+    if (!okJ) {
+      if (it % 5 == 0)
+        cp.M_j += 2;
+      else
+        cp.n_panels_j = (int32_t)ceil(1.5 * cp.n_panels_j);
+    }
+  }
+  if (errs)
+    for (int c = 0; c < 4; ++c) errs[c] = e4[c];
+  qb_dev_free(ctx, zd, nz * sizeof(cd));
+  return rc;
+}

@@ -1,0 +1,90 @@
+// Internal declarations of libqbhip (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "qb_complex.h"
+#include "../../include/qb.h"
+
+#define QB_MAX_COMP 8
+
+// Boundary geometry resident on the device.  Node arrays are concatenated
+// over components; the Kress R matrix and the log term are circulant per
+// component, so each is stored as one length-N_c vector indexed by
+// (i - j) mod N_c.
+struct DevGeom {
+  int64_t N;
+  int32_t ncomp;
+  int64_t off[QB_MAX_COMP + 1];  // component offsets (0-based), off[ncomp] = N
+  const double* x;
+  const double* y;
+  const double* dx;
+  const double* dy;
+  const double* speed;
+  const double* ws;
+  const double* kappa;  // curvature / 2pi (boundarypoints.jl:881-882)
+  const int32_t* comp;  // component id per node
+  const double* rk;     // Kress R circulant columns, concatenated (length N)
+  const double* lt;     // log(4 sin^2((t_i - t_j)/2)) circulant columns (length N)
+};
+
+struct qb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t bytes = 0;  // device bytes held by objects of this context
+};
+
+struct qb_geom {
+  qb_ctx* ctx = nullptr;
+  DevGeom d{};
+  void* buf = nullptr;  // one allocation holding all arrays
+  int64_t bytes = 0;
+  double rmin_geom = 0, rmax_geom = 0;  // padded (x0.95, x1.05) min / max distance
+  double rmin_dlp = 0, rmax_dlp = 0;    // same over the first component only
+};
+
+// Chebyshev tables of one or more wavenumbers, resident on the device.
+// H: [nz][np_h][M_h+1] of {H0 coeff, H1 coeff}; J: [nz][np_j][M_j+1] of
+// {J0 coeff, J1 coeff}.  Panels are uniform on [rmin_h, rmax] (Hankel) and
+// [0, rmax] (J), built exactly like plan_h / plan_j.
+struct DevTables {
+  int64_t nz = 0;
+  int32_t np_h = 0, M_h = 0, np_j = 0, M_j = 0;
+  double rmin_h = 0, rmax = 0;
+  cd* H = nullptr;
+  cd* J = nullptr;
+};
+
+// error helpers ------------------------------------------------------------
+int qb_fail(qb_ctx* ctx, int code, const std::string& msg);
+#define QB_HIP(ctx, expr)                                                   \
+  do {                                                                      \
+    hipError_t _e = (expr);                                                 \
+    if (_e != hipSuccess)                                                   \
+      return qb_fail(ctx, QB_E_HIP, std::string(#expr) + ": " +             \
+                                        hipGetErrorString(_e));             \
+  } while (0)
+
+int qb_dev_alloc(qb_ctx* ctx, void** p, size_t bytes);
+void qb_dev_free(qb_ctx* ctx, void* p, size_t bytes);
+
+// tables (qb_tables.hip) ----------------------------------------------------
+int qb_tables_build(qb_ctx* ctx, const double* z_dev_or_host, int64_t nz,
+                    double rmin_h, double rmax, const qb_cheb_params& cp,
+                    bool want_h0j0, DevTables* t);
+void qb_tables_free(qb_ctx* ctx, DevTables* t);
+void qb_lobatto_host(int M, double* t, double* C);  // nodes cospi(j/M), DCT matrix
+
+// assembly (qb_assemble.hip) -------------------------------------------------
+// Writes T(z_q) for q in [0, nz) into out64 (ld = N, stride N*N), and, if
+// out32 != nullptr, a complex64 copy into out32.
+int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
+                    int64_t nz, const DevTables& t, cd* out64, cf* out32);
+
+// bessel (qb_tables.hip)
+int qb_bessel_eval_dev(qb_ctx* ctx, const cd* z, int64_t n, cd* out4);
+int qb_cheb_check_dev(qb_ctx* ctx, const DevTables& t, const cd* z_dev,
+                      int64_t nz, int32_t samples, double* err4_host);

@@ -1,0 +1,31 @@
+// Batched complex LU / triangular solves / helpers (qb_linalg.hip).
+#pragma once
+#include "qb_internal.h"
+
+template <typename T>
+int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA, const T* B,
+             int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch);
+template <typename T>
+int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
+                     int* info, int batch);
+// Triangular solves only (the row permutation is applied separately, fused
+// into the gather that fills B).
+template <typename T>
+int qb_trsm_lu_batched(qb_ctx* ctx, const T* LU, int n, int64_t lda, int64_t sA, T* B, int nrhs,
+                       int64_t ldb, int64_t sB, int batch);
+
+int qb_perm_from_ipiv(qb_ctx* ctx, const int* ipiv, int* perm, int n, int batch);
+// dst[b][i + c*n] = src[(b*sb)][perm[b][i] + c*n] for a permuted broadcast /
+// copy of right-hand sides (src stride 0 broadcasts one block to all nodes).
+int qb_gather_rows_cd(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm, cd* dst, int n,
+                      int nrhs, int batch);
+int qb_gather_rows_cd2cf(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm, cf* dst,
+                         int n, int nrhs, int batch);
+int qb_add_cf_to_cd(qb_ctx* ctx, const cf* d, cd* x, int64_t n);
+int qb_cd_to_cf(qb_ctx* ctx, const cd* x, cf* y, int64_t n);
+int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int nb);
+int qb_moments(qb_ctx* ctx, const cd* X, int64_t nv, int nq, const cd* w, const cd* wz, cd* A0,
+               cd* A1);
+int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int nb, double* out_dev);
+int qb_node_maxabs32(qb_ctx* ctx, const cf* X, int64_t n_per, int nb, double* out_dev);
+int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev);

@@ -1,0 +1,668 @@
+// Batched dense complex linear algebra for the Beyn contour loop:
+// blocked LU with partial pivoting (replaces lu!(T; check=false), i.e.
+// LAPACK zgetrf, beyn.jl:270-277), the two triangular solves with the row
+// permutation (ldiv!, zgetrs, beyn.jl:281), the FP64 residual of the
+// mixed-precision refinement, and the moment sums (axpy!, beyn.jl:282-283).
+//
+// Every routine works on a batch of matrices with a fixed stride (one matrix
+// per contour node), so one launch covers all nodes of a window.  Matrices
+// are column-major, complex interleaved.  T = cf (complex64) or cd
+// (complex128).
+//
+// Structure (right-looking, NB = 128):
+//   panel (recursive, leaf width 32: single-workgroup column elimination)
+//   -> row swaps on the other columns -> unit-lower TRSM of the block row
+//   -> trailing update C -= A*B on MFMA.
+// The trailing update and every GEMM inside the recursions run in
+// k_gemm_sub, which splits complex operands into real/imaginary planes in
+// LDS and issues 4 real 16x16x4 MFMAs per complex product block
+// (v_mfma_f32_16x16x4_f32 for complex64, v_mfma_f64_16x16x4_f64 for
+// complex128) with accumulation in the operand precision.
+#include "qb_internal.h"
+#include "qb_linalg.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Scal;
+template <>
+struct Scal<cf> {
+  typedef float S;
+  typedef floatx4 V4;
+};
+template <>
+struct Scal<cd> {
+  typedef double S;
+  typedef doublex4 V4;
+};
+
+__device__ __forceinline__ floatx4 mfma16(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// GEMM: C[b] -= A[b] * B[b]  (A: M x K, B: K x N, C: M x N)
+// ---------------------------------------------------------------------------
+// Workgroup tile BM x BN, k-tile BK, 4 waves as 2 x 2, wave tile
+// (BM/2) x (BN/2) built from 16 x 16 MFMA blocks.  MFMA 16x16x4 operand
+// layout (one value per lane): A[i][k] with i = lane % 16, k = lane / 16;
+// B[k][j] with j = lane % 16, k = lane / 16.  Result register v of a lane
+// holds D[4*(lane/16) + v][lane % 16] for the f32 instruction and
+// D[4*v + lane/16][lane % 16] for the f64 one (checked on gfx950).
+template <typename T, int BM, int BN, int BK>
+__global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
+                                                  int64_t lda, int64_t sA,
+                                                  const T* __restrict__ B, int64_t ldb,
+                                                  int64_t sB, T* __restrict__ C, int64_t ldc,
+                                                  int64_t sC) {
+  typedef typename Scal<T>::S S;
+  typedef typename Scal<T>::V4 V4;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  __shared__ S As_re[BK][BM + 1], As_im[BK][BM + 1];
+  __shared__ S Bs_re[BK][BN + 1], Bs_im[BK][BN + 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int64_t b = blockIdx.z;
+  A += b * sA;
+  B += b * sB;
+  C += b * sC;
+
+  V4 acc_re[FM][FN], acc_im[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc_re[i][j] = V4{0, 0, 0, 0};
+      acc_im[i][j] = V4{0, 0, 0, 0};
+    }
+
+  const int li = lane & 15, lk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // global -> LDS, A negated so the accumulators hold -A*B
+    for (int e = tid; e < BM * BK; e += 256) {
+      int m = e % BM, kk = e / BM;
+      int gm = m0 + m, gk = k0 + kk;
+      T v = (gm < M && gk < K) ? A[gm + (int64_t)gk * lda] : T{0, 0};
+      As_re[kk][m] = -v.re;
+      As_im[kk][m] = -v.im;
+    }
+    for (int e = tid; e < BN * BK; e += 256) {
+      int kk = e % BK, n = e / BK;
+      int gn = n0 + n, gk = k0 + kk;
+      T v = (gn < N && gk < K) ? B[gk + (int64_t)gn * ldb] : T{0, 0};
+      Bs_re[kk][n] = v.re;
+      Bs_im[kk][n] = v.im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k4 = 0; k4 < BK; k4 += 4) {
+      S ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        ar[i] = As_re[k4 + lk][wm0 + i * 16 + li];
+        ai[i] = As_im[k4 + lk][wm0 + i * 16 + li];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        br[j] = Bs_re[k4 + lk][wn0 + j * 16 + li];
+        bi[j] = Bs_im[k4 + lk][wn0 + j * 16 + li];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          acc_re[i][j] = mfma16(ar[i], br[j], acc_re[i][j]);
+          acc_re[i][j] = mfma16(nai[i], bi[j], acc_re[i][j]);
+          acc_im[i][j] = mfma16(ar[i], bi[j], acc_im[i][j]);
+          acc_im[i][j] = mfma16(ai[i], br[j], acc_im[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      int gn = n0 + wn0 + j * 16 + li;
+      if (gn >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        int gm = m0 + wm0 + i * 16 + (sizeof(S) == 8 ? 4 * v + lk : lk * 4 + v);
+        if (gm < M) {
+          T* p = C + gm + (int64_t)gn * ldc;
+          T c = *p;
+          c.re += acc_re[i][j][v];
+          c.im += acc_im[i][j][v];
+          *p = c;
+        }
+      }
+    }
+}
+
+template <typename T>
+int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA, const T* B,
+             int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
+  if (sizeof(T) == sizeof(cf)) {
+    constexpr int BM = 128, BN = 128, BK = 16;
+    dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+    hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK>), g, dim3(256), 0, ctx->stream, M, N, K, A,
+                       lda, sA, B, ldb, sB, C, ldc, sC);
+  } else {
+    constexpr int BM = 64, BN = 64, BK = 16;
+    dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+    hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK>), g, dim3(256), 0, ctx->stream, M, N, K, A,
+                       lda, sA, B, ldb, sB, C, ldc, sC);
+  }
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Leaf panel factorisation: columns [c0, c0+w) of rows [c0, n), w <= 32.
+// One workgroup per matrix.  Partial pivoting on max |re|+|im| (LAPACK's
+// izamax measure); a zero pivot leaves the column unscaled and counts in
+// info, the behaviour of zgetf2 behind lu!(check=false).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, int64_t sA,
+                                                int n, int c0, int w, int* __restrict__ ipiv,
+                                                int64_t sP, int* __restrict__ info) {
+  typedef typename Scal<T>::S S;
+  __shared__ S red_v[32];
+  __shared__ int red_i[32];
+  __shared__ T prow[32];
+  __shared__ int s_piv;
+  const int64_t b = blockIdx.x;
+  A += b * sA;
+  ipiv += b * sP;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  for (int c = 0; c < w; ++c) {
+    const int col = c0 + c;
+    T* Ac = A + (int64_t)col * lda;
+    // pivot search over rows col..n-1
+    S best = -1;
+    int bi = n;
+    for (int r = col + tid; r < n; r += nt) {
+      T v = Ac[r];
+      S a = fabs(v.re) + fabs(v.im);
+      if (a > best) {
+        best = a;
+        bi = r;
+      }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      S ob = __shfl_xor(best, off);
+      int oi = __shfl_xor(bi, off);
+      if (ob > best || (ob == best && oi < bi)) {
+        best = ob;
+        bi = oi;
+      }
+    }
+    if (lane == 0) {
+      red_v[wv] = best;
+      red_i[wv] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      S bv = red_v[0];
+      int bix = red_i[0];
+      for (int q = 1; q < (nt + 63) / 64; ++q)
+        if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bix)) {
+          bv = red_v[q];
+          bix = red_i[q];
+        }
+      if (bix >= n) bix = col;
+      s_piv = bix;
+      ipiv[col] = bix;
+    }
+    __syncthreads();
+    const int piv = s_piv;
+    // swap rows col and piv across the w panel columns, keep the pivot row
+    for (int cc = tid; cc < w; cc += nt) {
+      T* Acc = A + (int64_t)(c0 + cc) * lda;
+      T u = Acc[piv];
+      if (piv != col) {
+        T d = Acc[col];
+        Acc[piv] = d;
+        Acc[col] = u;
+      }
+      prow[cc] = u;
+    }
+    __syncthreads();
+    T p = prow[c];
+    bool nz = (p.re != 0 || p.im != 0);
+    if (!nz && tid == 0 && info) atomicCAS(info + b, 0, col + 1);
+    // scale the column below the pivot
+    T inv = T{1, 0};
+    if (nz) {
+      S den = p.re * p.re + p.im * p.im;
+      inv = T{p.re / den, -p.im / den};
+    }
+    for (int r = col + 1 + tid; r < n; r += nt) {
+      T v = Ac[r];
+      if (nz) {
+        T s = T{v.re * inv.re - v.im * inv.im, v.re * inv.im + v.im * inv.re};
+        Ac[r] = s;
+        v = s;
+      }
+      // rank-1 update of the remaining panel columns of this row
+      for (int cc = c + 1; cc < w; ++cc) {
+        T* q = A + (int64_t)(c0 + cc) * lda + r;
+        T u = prow[cc];
+        T x = *q;
+        x.re -= v.re * u.re - v.im * u.im;
+        x.im -= v.re * u.im + v.im * u.re;
+        *q = x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Row interchanges ipiv[k0..k1) applied in order to columns [cb, ce)
+// (LAPACK zlaswp).  One thread per column.
+template <typename T>
+__global__ void k_laswp(T* __restrict__ A, int64_t lda, int64_t sA, int cb, int ce, int k0, int k1,
+                        const int* __restrict__ ipiv, int64_t sP) {
+  const int64_t b = blockIdx.y;
+  int col = cb + blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ce) return;
+  A += b * sA + (int64_t)col * lda;
+  ipiv += b * sP;
+  for (int s = k0; s < k1; ++s) {
+    int p = ipiv[s];
+    if (p != s) {
+      T t = A[s];
+      A[s] = A[p];
+      A[p] = t;
+    }
+  }
+}
+
+// Unit-lower triangular solve X = L^{-1} B, n <= 32, one thread per column.
+template <typename T, bool UNIT_LOWER>
+__global__ void __launch_bounds__(256) k_trsm_leaf(int n, int nrhs, const T* __restrict__ L,
+                                                   int64_t ldl, int64_t sL, T* __restrict__ B,
+                                                   int64_t ldb, int64_t sB) {
+  __shared__ T Ls[32][33];
+  const int64_t b = blockIdx.y;
+  L += b * sL;
+  B += b * sB;
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    int i = e % n, j = e / n;
+    Ls[i][j] = L[i + (int64_t)j * ldl];
+  }
+  __syncthreads();
+  if (!UNIT_LOWER) {
+    // replace the diagonal by its reciprocal
+    if (threadIdx.x < n) {
+      T d = Ls[threadIdx.x][threadIdx.x];
+      auto den = d.re * d.re + d.im * d.im;
+      Ls[threadIdx.x][threadIdx.x] = T{d.re / den, -d.im / den};
+    }
+    __syncthreads();
+  }
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nrhs) return;
+  T* x = B + (int64_t)col * ldb;
+  T v[32];
+  for (int i = 0; i < 32; ++i) v[i] = (i < n) ? x[i] : T{0, 0};
+  if (UNIT_LOWER) {
+    for (int i = 0; i < n; ++i) {
+      T s = v[i];
+      for (int j = 0; j < i; ++j) {
+        T l = Ls[i][j];
+        s.re -= l.re * v[j].re - l.im * v[j].im;
+        s.im -= l.re * v[j].im + l.im * v[j].re;
+      }
+      v[i] = s;
+    }
+  } else {
+    for (int i = n - 1; i >= 0; --i) {
+      T s = v[i];
+      for (int j = i + 1; j < n; ++j) {
+        T u = Ls[i][j];
+        s.re -= u.re * v[j].re - u.im * v[j].im;
+        s.im -= u.re * v[j].im + u.im * v[j].re;
+      }
+      T d = Ls[i][i];
+      v[i] = T{s.re * d.re - s.im * d.im, s.re * d.im + s.im * d.re};
+    }
+  }
+  for (int i = 0; i < n; ++i) x[i] = v[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host orchestration
+// ---------------------------------------------------------------------------
+template <typename T>
+static int laswp(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int k0, int k1,
+                 const int* ipiv, int64_t sP, int batch) {
+  if (ce <= cb || k1 <= k0) return QB_OK;
+  dim3 g((ce - cb + 127) / 128, batch);
+  hipLaunchKernelGGL(k_laswp<T>, g, dim3(128), 0, ctx->stream, A, lda, sA, cb, ce, k0, k1, ipiv,
+                     sP);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+static int split32(int n) {
+  int h = (n / 2 + 31) / 32 * 32;
+  return h >= n ? n / 2 : h;
+}
+
+// X = L^{-1} B with L the unit-lower n x n block at L (ld ldl).
+template <typename T>
+static int trsm_lower(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int64_t sL, T* B,
+                      int64_t ldb, int64_t sB, int batch) {
+  if (n <= 0 || nrhs <= 0) return QB_OK;
+  if (n <= 32) {
+    dim3 g((nrhs + 255) / 256, batch);
+    hipLaunchKernelGGL((k_trsm_leaf<T, true>), g, dim3(256), 0, ctx->stream, n, nrhs, L, ldl, sL,
+                       B, ldb, sB);
+    QB_HIP(ctx, hipGetLastError());
+    return QB_OK;
+  }
+  int n1 = split32(n);
+  int rc = trsm_lower(ctx, n1, nrhs, L, ldl, sL, B, ldb, sB, batch);
+  if (rc) return rc;
+  rc = gemm_sub<T>(ctx, n - n1, nrhs, n1, L + n1, ldl, sL, B, ldb, sB, B + n1, ldb, sB, batch);
+  if (rc) return rc;
+  return trsm_lower(ctx, n - n1, nrhs, L + n1 + (int64_t)n1 * ldl, ldl, sL, B + n1, ldb, sB,
+                    batch);
+}
+
+// X = U^{-1} B with U the upper n x n block.
+template <typename T>
+static int trsm_upper(qb_ctx* ctx, int n, int nrhs, const T* U, int64_t ldu, int64_t sU, T* B,
+                      int64_t ldb, int64_t sB, int batch) {
+  if (n <= 0 || nrhs <= 0) return QB_OK;
+  if (n <= 32) {
+    dim3 g((nrhs + 255) / 256, batch);
+    hipLaunchKernelGGL((k_trsm_leaf<T, false>), g, dim3(256), 0, ctx->stream, n, nrhs, U, ldu,
+                       sU, B, ldb, sB);
+    QB_HIP(ctx, hipGetLastError());
+    return QB_OK;
+  }
+  int n1 = split32(n);
+  int rc = trsm_upper(ctx, n - n1, nrhs, U + n1 + (int64_t)n1 * ldu, ldu, sU, B + n1, ldb, sB,
+                      batch);
+  if (rc) return rc;
+  rc = gemm_sub<T>(ctx, n1, nrhs, n - n1, U + (int64_t)n1 * ldu, ldu, sU, B + n1, ldb, sB, B,
+                   ldb, sB, batch);
+  if (rc) return rc;
+  return trsm_upper(ctx, n1, nrhs, U, ldu, sU, B, ldb, sB, batch);
+}
+
+// Recursive panel: factor columns [c0, c0+w) (rows c0..n-1) of every matrix.
+template <typename T>
+static int panel_rec(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int n, int c0, int w, int* ipiv,
+                     int64_t sP, int* info, int batch) {
+  if (w <= 32) {
+    hipLaunchKernelGGL(k_getf2<T>, dim3(batch), dim3(1024), 0, ctx->stream, A, lda, sA, n, c0, w,
+                       ipiv, sP, info);
+    QB_HIP(ctx, hipGetLastError());
+    return QB_OK;
+  }
+  int w1 = split32(w);
+  int rc = panel_rec(ctx, A, lda, sA, n, c0, w1, ipiv, sP, info, batch);
+  if (rc) return rc;
+  rc = laswp(ctx, A, lda, sA, c0 + w1, c0 + w, c0, c0 + w1, ipiv, sP, batch);
+  if (rc) return rc;
+  T* A11 = A + c0 + (int64_t)c0 * lda;
+  rc = trsm_lower(ctx, w1, w - w1, A11, lda, sA, A + c0 + (int64_t)(c0 + w1) * lda, lda, sA,
+                  batch);
+  if (rc) return rc;
+  rc = gemm_sub<T>(ctx, n - c0 - w1, w - w1, w1, A + c0 + w1 + (int64_t)c0 * lda, lda, sA,
+                   A + c0 + (int64_t)(c0 + w1) * lda, lda, sA,
+                   A + c0 + w1 + (int64_t)(c0 + w1) * lda, lda, sA, batch);
+  if (rc) return rc;
+  rc = panel_rec(ctx, A, lda, sA, n, c0 + w1, w - w1, ipiv, sP, info, batch);
+  if (rc) return rc;
+  return laswp(ctx, A, lda, sA, c0, c0 + w1, c0 + w1, c0 + w, ipiv, sP, batch);
+}
+
+template <typename T>
+int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
+                     int* info, int batch) {
+  const int NB = 128;
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    int kb = n - k0 < NB ? n - k0 : NB;
+    int rc = panel_rec(ctx, A, lda, sA, n, k0, kb, ipiv, sP, info, batch);
+    if (rc) return rc;
+    rc = laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch);
+    if (rc) return rc;
+    rc = laswp(ctx, A, lda, sA, k0 + kb, n, k0, k0 + kb, ipiv, sP, batch);
+    if (rc) return rc;
+    if (k0 + kb < n) {
+      T* A11 = A + k0 + (int64_t)k0 * lda;
+      T* A12 = A + k0 + (int64_t)(k0 + kb) * lda;
+      rc = trsm_lower(ctx, kb, n - k0 - kb, A11, lda, sA, A12, lda, sA, batch);
+      if (rc) return rc;
+      rc = gemm_sub<T>(ctx, n - k0 - kb, n - k0 - kb, kb, A + k0 + kb + (int64_t)k0 * lda, lda,
+                       sA, A12, lda, sA, A + k0 + kb + (int64_t)(k0 + kb) * lda, lda, sA, batch);
+      if (rc) return rc;
+    }
+  }
+  return QB_OK;
+}
+
+// B <- U^{-1} L^{-1} B with the factors of qb_getrf_batched; the row
+// permutation of zgetrs is applied by the caller (fused into a gather).
+template <typename T>
+int qb_trsm_lu_batched(qb_ctx* ctx, const T* LU, int n, int64_t lda, int64_t sA, T* B, int nrhs,
+                       int64_t ldb, int64_t sB, int batch) {
+  int rc = trsm_lower(ctx, n, nrhs, LU, lda, sA, B, ldb, sB, batch);
+  if (rc) return rc;
+  return trsm_upper(ctx, n, nrhs, LU, lda, sA, B, ldb, sB, batch);
+}
+
+// perm[b][i]: the row of the original right-hand side that lands in row i
+// after the interchanges ipiv[b][0..n) are applied in order.
+__global__ void k_perm_from_ipiv(const int* __restrict__ ipiv, int* __restrict__ perm, int n,
+                                 int batch) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const int* ip = ipiv + (int64_t)b * n;
+  int* pm = perm + (int64_t)b * n;
+  for (int i = 0; i < n; ++i) pm[i] = i;
+  for (int s = 0; s < n; ++s) {
+    int p = ip[s];
+    if (p != s) {
+      int t = pm[s];
+      pm[s] = pm[p];
+      pm[p] = t;
+    }
+  }
+}
+
+int qb_perm_from_ipiv(qb_ctx* ctx, const int* ipiv, int* perm, int n, int batch) {
+  hipLaunchKernelGGL(k_perm_from_ipiv, dim3((batch + 63) / 64), dim3(64), 0, ctx->stream, ipiv,
+                     perm, n, batch);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+template <typename TO>
+__global__ void k_gather_rows(const cd* __restrict__ src, int64_t s_src,
+                              const int* __restrict__ perm, TO* __restrict__ dst, int n, int nrhs) {
+  const int64_t b = blockIdx.z;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.y;
+  if (i >= n) return;
+  int pi = perm[b * (int64_t)n + i];
+  cd v = src[b * s_src + pi + (int64_t)c * n];
+  TO* d = dst + b * (int64_t)n * nrhs + i + (int64_t)c * n;
+  if constexpr (sizeof(TO) == sizeof(cf))
+    *d = to_cf(v);
+  else
+    *d = v;
+}
+
+int qb_gather_rows_cd(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm, cd* dst, int n,
+                      int nrhs, int batch) {
+  dim3 g((n + 255) / 256, nrhs, batch);
+  hipLaunchKernelGGL(k_gather_rows<cd>, g, dim3(256), 0, ctx->stream, src, s_src, perm, dst, n,
+                     nrhs);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_gather_rows_cd2cf(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm, cf* dst,
+                         int n, int nrhs, int batch) {
+  dim3 g((n + 255) / 256, nrhs, batch);
+  hipLaunchKernelGGL(k_gather_rows<cf>, g, dim3(256), 0, ctx->stream, src, s_src, perm, dst, n,
+                     nrhs);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise helpers
+// ---------------------------------------------------------------------------
+__global__ void k_cd_to_cf(const cd* __restrict__ x, cf* __restrict__ y, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) y[i] = to_cf(x[i]);
+}
+__global__ void k_add_cf_to_cd(const cf* __restrict__ d, cd* __restrict__ x, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    cd v = x[i];
+    cf e = d[i];
+    v.re += (double)e.re;
+    v.im += (double)e.im;
+    x[i] = v;
+  }
+}
+// Copy V (one N x r block) into every node's slot of R.
+__global__ void k_bcast(const cd* __restrict__ V, cd* __restrict__ R, int64_t nv, int nb) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nv * nb; i += st) R[i] = V[i % nv];
+}
+// A0 = sum_j w_j X_j, A1 = sum_j w_j z_j X_j over nq nodes (beyn.jl:282-283).
+__global__ void k_moments(const cd* __restrict__ X, int64_t nv, int nq, const cd* __restrict__ w,
+                          const cd* __restrict__ wz, cd* __restrict__ A0, cd* __restrict__ A1) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  cd a0 = mk(0, 0), a1 = mk(0, 0);
+  for (int j = 0; j < nq; ++j) {
+    cd x = X[(int64_t)j * nv + i];
+    a0 = a0 + w[j] * x;
+    a1 = a1 + wz[j] * x;
+  }
+  A0[i] = a0;
+  A1[i] = a1;
+}
+// Per-node max |.|_inf of n_per entries (|re|+|im| bound not used: true modulus).
+__global__ void k_node_maxabs(const cd* __restrict__ X, int64_t n_per, double* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  X += b * n_per;
+  double m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_per;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, cabsd(X[i]));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax((unsigned long long*)(out + b), (unsigned long long)__double_as_longlong(m));
+}
+__global__ void k_node_maxabs32(const cf* __restrict__ X, int64_t n_per, double* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  X += b * n_per;
+  double m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_per;
+       i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, (double)hypotf(X[i].re, X[i].im));
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax((unsigned long long*)(out + b), (unsigned long long)__double_as_longlong(m));
+}
+// Per-node infinity norm of the matrix: max_i sum_j |A_ij|.  One thread per row.
+__global__ void k_node_rownorm(const cd* __restrict__ A, int64_t n, int64_t sA,
+                               double* __restrict__ out) {
+  const int64_t b = blockIdx.y;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0;
+  if (i < n) {
+    const cd* a = A + b * sA + i;
+    for (int64_t j = 0; j < n; ++j) s += cabsd(a[j * n]);
+  }
+  for (int off = 32; off >= 1; off >>= 1) s = fmax(s, __shfl_xor(s, off));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax((unsigned long long*)(out + b), (unsigned long long)__double_as_longlong(s));
+}
+
+static unsigned grid1(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return (unsigned)(g > 65536 ? 65536 : (g < 1 ? 1 : g));
+}
+
+int qb_cd_to_cf(qb_ctx* ctx, const cd* x, cf* y, int64_t n) {
+  hipLaunchKernelGGL(k_cd_to_cf, dim3(grid1(n)), dim3(256), 0, ctx->stream, x, y, n);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_add_cf_to_cd(qb_ctx* ctx, const cf* d, cd* x, int64_t n) {
+  hipLaunchKernelGGL(k_add_cf_to_cd, dim3(grid1(n)), dim3(256), 0, ctx->stream, d, x, n);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int nb) {
+  hipLaunchKernelGGL(k_bcast, dim3(grid1(nv * nb)), dim3(256), 0, ctx->stream, V, R, nv, nb);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_moments(qb_ctx* ctx, const cd* X, int64_t nv, int nq, const cd* w, const cd* wz, cd* A0,
+               cd* A1) {
+  hipLaunchKernelGGL(k_moments, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, ctx->stream, X,
+                     nv, nq, w, wz, A0, A1);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int nb, double* out_dev) {
+  QB_HIP(ctx, hipMemsetAsync(out_dev, 0, nb * sizeof(double), ctx->stream));
+  int64_t g = (n_per + 255) / 256;
+  dim3 grid((unsigned)(g > 1024 ? 1024 : g), nb);
+  hipLaunchKernelGGL(k_node_maxabs, grid, dim3(256), 0, ctx->stream, X, n_per, out_dev);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_node_maxabs32(qb_ctx* ctx, const cf* X, int64_t n_per, int nb, double* out_dev) {
+  QB_HIP(ctx, hipMemsetAsync(out_dev, 0, nb * sizeof(double), ctx->stream));
+  int64_t g = (n_per + 255) / 256;
+  dim3 grid((unsigned)(g > 1024 ? 1024 : g), nb);
+  hipLaunchKernelGGL(k_node_maxabs32, grid, dim3(256), 0, ctx->stream, X, n_per, out_dev);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev) {
+  QB_HIP(ctx, hipMemsetAsync(out_dev, 0, nb * sizeof(double), ctx->stream));
+  dim3 grid((unsigned)((n + 255) / 256), nb);
+  hipLaunchKernelGGL(k_node_rownorm, grid, dim3(256), 0, ctx->stream, A, n, n * n, out_dev);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+// explicit instantiations
+template int qb_getrf_batched<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int);
+template int qb_getrf_batched<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int);
+template int qb_trsm_lu_batched<cf>(qb_ctx*, const cf*, int, int64_t, int64_t, cf*, int, int64_t,
+                                    int64_t, int);
+template int qb_trsm_lu_batched<cd>(qb_ctx*, const cd*, int, int64_t, int64_t, cd*, int, int64_t,
+                                    int64_t, int);
+template int gemm_sub<cf>(qb_ctx*, int, int, int, const cf*, int64_t, int64_t, const cf*, int64_t,
+                          int64_t, cf*, int64_t, int64_t, int);
+template int gemm_sub<cd>(qb_ctx*, int, int, int, const cd*, int64_t, int64_t, const cd*, int64_t,
+                          int64_t, cd*, int64_t, int64_t, int);

@@ -1,0 +1,15 @@
+"""qbhip: host mirror of QuantumBilliards.jl's Kress/Beyn hot path on libqbhip.
+
+The compute runs in libqbhip.so (HIP, gfx950) behind the C ABI of
+include/qb.h; this package only arranges inputs and outputs the way the
+reference's Julia API does (see qbhip.solvers and qbhip.beyn).
+"""
+from ._lib import (QB_CFIE_KRESS, QB_DLP_KRESS, QB_FP64, QB_MIXED, QBError, LIB_PATH,
+                   EXPORTS, lib)
+from .context import Context, default_context
+from .geometry import (Billiard, BoundaryPointsCFIE, CircularHoleBilliard, evaluate_points,
+                       make_circle, make_full_ellipse, make_full_stadium,
+                       make_star_polar_boundary, n_kress)
+from .solvers import (CFIE_kress, DLP_kress, Geometry, assemble, bessel_eval,
+                      boundary_matrix_size, chebyshev_params, construct_boundary_matrices)
+from .beyn import Window, construct_B_matrix, footprint, probe_block, solve, solve_vect

@@ -1,0 +1,94 @@
+"""ctypes binding of libqbhip (include/qb.h).
+
+The library is loaded from the package directory (built in-tree by
+`make -C quantumbilliards.jl_amd/csrc`).  There is no fallback: if the shared
+library is missing the import of any entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libqbhip.so"))
+
+QB_OK, QB_E_ARG, QB_E_HIP, QB_E_NOMEM, QB_E_STATE, QB_E_NUMERIC = 0, -1, -2, -3, -4, -5
+QB_CFIE_KRESS, QB_DLP_KRESS = 1, 2
+QB_FP64, QB_MIXED = 0, 1
+QB_OUT_DEVICE = 1
+
+EXPORTS = (
+    "qb_ctx_create", "qb_ctx_destroy", "qb_last_error", "qb_ctx_device_bytes",
+    "qb_geom_create", "qb_geom_destroy", "qb_geom_size", "qb_assemble",
+    "qb_window_create", "qb_window_destroy", "qb_window_footprint", "qb_window_moments",
+    "qb_bessel_eval", "qb_cheb_tune",
+)
+
+
+class QBError(RuntimeError):
+    """A nonzero status from libqbhip, with the context's message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libqbhip error {code}: {msg}")
+        self.code = code
+
+
+class ChebParams(C.Structure):
+    _fields_ = [("n_panels_h", C.c_int32), ("M_h", C.c_int32),
+                ("n_panels_j", C.c_int32), ("M_j", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("N", C.c_int64), ("nq", C.c_int32), ("r", C.c_int32),
+                ("precision", C.c_int32), ("refine_iters_max", C.c_int32),
+                ("nodes_fp64_fallback", C.c_int32), ("refine_resid_max", C.c_double),
+                ("t_assemble_ms", C.c_double), ("t_factor_ms", C.c_double),
+                ("t_solve_ms", C.c_double), ("rmin", C.c_double), ("rmax", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libqbhip once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"libqbhip.so not built at {LIB_PATH}: run __graft_entry__.build() "
+                      "or make -C quantumbilliards.jl_amd/csrc")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    PP = C.POINTER(C.c_void_p)
+    sig = {
+        "qb_ctx_create": (C.c_int, [C.c_int, PP]),
+        "qb_ctx_destroy": (None, [P]),
+        "qb_last_error": (C.c_char_p, [P]),
+        "qb_ctx_device_bytes": (I64, [P]),
+        "qb_geom_create": (C.c_int, [P, I32, P, P, P, P, P, P, P, P, P, PP]),
+        "qb_geom_destroy": (None, [P]),
+        "qb_geom_size": (I64, [P]),
+        "qb_assemble": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P]),
+        "qb_window_create": (C.c_int, [P, P, I32, D, D, D, I32, C.POINTER(ChebParams), I32, PP,
+                                       C.POINTER(Stats)]),
+        "qb_window_destroy": (None, [P]),
+        "qb_window_footprint": (I64, [I64, I32, I32, I32]),
+        "qb_window_moments": (C.c_int, [P, I32, P, P, P, D, C.POINTER(Stats)]),
+        "qb_bessel_eval": (C.c_int, [P, P, I64, P, P, P, P]),
+        "qb_cheb_tune": (C.c_int, [P, P, I32, P, I64, D, I32, I32, C.POINTER(ChebParams), P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(ctx_ptr, rc: int):
+    if rc != QB_OK:
+        msg = lib().qb_last_error(ctx_ptr)
+        raise QBError(rc, msg.decode() if msg else "")

@@ -1,0 +1,174 @@
+"""Beyn contour method on libqbhip.
+
+Mirrors the reference entry points (src/solvers/acceleratedmethods/beyn.jl):
+  construct_B_matrix(solver, pts, N, k0, R; nq=64, r=48, svd_tol=1e-14, rng,
+                     use_chebyshev, n_panels_h, M_h, n_panels_j, M_j, info, multithreaded)
+      -> (B, Uk)                                                    beyn.jl:257-322
+  solve_vect(solver, basis, pts, k, dk; nq=32, r=48, ...) -> (lam, Uk, Y, k, dk, pts)
+                                                                    beyn.jl:364-375
+  solve(solver, basis, pts, k, dk; ...) -> lam                     beyn.jl:386-397
+The contour nodes, assembly, LU of every node and the moment solves run on
+the device (qb_window_*); the thin SVD with the absolute rank threshold,
+B = U_k^H A1 W_k Sigma_k^{-1} and eig(B) run on the host in FP64 (LAPACK
+through NumPy), as the reference runs them through LinearAlgebra.
+
+Extensions (keyword arguments the reference does not have):
+  precision="fp64"|"mixed": complex128 LU, or complex64 LU on MFMA with FP64
+      iterative refinement (refine_tol: relative correction at which it stops);
+  grow_r: keep doubling r while the rank saturates (the reference retries
+      once with 2r, beyn.jl:291-309, because of the unconditional break at
+      :307); with grow_r=False the single retry is reproduced;
+  V: explicit probe block instead of one drawn from rng.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .context import Context, default_context
+from .solvers import Geometry, boundary_matrix_size
+
+_PREC = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED}
+
+
+def probe_block(rng: np.random.Generator, N: int, r: int) -> np.ndarray:
+    """randn(rng, ComplexF64, N, r) (beyn.jl:193): real and imaginary parts
+    N(0, 1/2), column-major."""
+    V = (rng.standard_normal((N, r)) + 1j * rng.standard_normal((N, r))) * np.sqrt(0.5)
+    return np.asfortranarray(V)
+
+
+class Window:
+    """Contour nodes of one disk with T(z_j) assembled and factored on the
+    device (qb_window).  `moments(V)` may be called repeatedly."""
+
+    def __init__(self, solver, pts, k0: complex, R: float, nq: int, *, n_panels_h=15000, M_h=5,
+                 n_panels_j=3000, M_j=5, precision="fp64", geom: Optional[Geometry] = None,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or (geom.ctx if geom is not None else default_context())
+        self.geom = geom or Geometry(pts, self.ctx)
+        self.N = self.geom.N
+        self.nq = nq
+        self.k0, self.R = complex(k0), float(R)
+        self.precision = precision
+        cp = _lib.ChebParams(int(n_panels_h), int(M_h), int(n_panels_j), int(M_j))
+        st = _lib.Stats()
+        h = C.c_void_p()
+        rc = _lib.lib().qb_window_create(self.ctx.ptr, self.geom.ptr, solver.op, self.k0.real,
+                                         self.k0.imag, self.R, int(nq), C.byref(cp),
+                                         _PREC[precision], C.byref(h), C.byref(st))
+        _lib.check(self.ctx.ptr, rc)
+        self.ptr = h
+        self.stats = st.as_dict()
+
+    def moments(self, V: np.ndarray, refine_tol: float = 1e-13):
+        V = np.asfortranarray(V, dtype=np.complex128)
+        assert V.shape[0] == self.N
+        r = V.shape[1]
+        A0 = np.empty((self.N, r), dtype=np.complex128, order="F")
+        A1 = np.empty((self.N, r), dtype=np.complex128, order="F")
+        st = _lib.Stats()
+        rc = _lib.lib().qb_window_moments(self.ptr, int(r), V.ctypes.data, A0.ctypes.data,
+                                          A1.ctypes.data, float(refine_tol), C.byref(st))
+        _lib.check(self.ctx.ptr, rc)
+        self.stats = st.as_dict()
+        return A0, A1
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _lib.lib().qb_window_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(N: int, nq: int, precision: str = "fp64", r: int = 64) -> int:
+    """Device bytes of one window (the admission footprint of the serving loop)."""
+    return int(_lib.lib().qb_window_footprint(int(N), int(nq), _PREC[precision], int(r)))
+
+
+def _svd_B(A0, A1, svd_tol):
+    U, S, Wh = np.linalg.svd(A0, full_matrices=False)
+    rk = int(np.count_nonzero(S >= svd_tol))
+    return U, S, Wh, rk
+
+
+def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 64, r: int = 48,
+                       svd_tol: float = 1e-14, rng: Optional[np.random.Generator] = None,
+                       use_chebyshev: bool = True, n_panels_h: int = 15000, M_h: int = 5,
+                       n_panels_j: int = 3000, M_j: int = 5, info: bool = False,
+                       multithreaded: bool = True, precision: str = "fp64",
+                       refine_tol: float = 1e-13, grow_r: bool = False,
+                       V: Optional[np.ndarray] = None, window: Optional[Window] = None,
+                       return_info: bool = False):
+    """beyn.jl:257-322.  Returns (B, Uk) (and a diagnostics dict if
+    return_info)."""
+    if not use_chebyshev:
+        raise RuntimeError("only the Chebyshev path exists for complex contour nodes")
+    rng = rng if rng is not None else np.random.default_rng(0)
+    assert boundary_matrix_size(pts) == N, "N must equal the boundary matrix size"
+    t0 = time.perf_counter()
+    own = window is None
+    win = window or Window(solver, pts, k0, R, nq, n_panels_h=n_panels_h, M_h=M_h,
+                           n_panels_j=n_panels_j, M_j=M_j, precision=precision)
+    diag = {"window": dict(win.stats)}
+    try:
+        Vb = V if V is not None else probe_block(rng, N, r)
+        r = Vb.shape[1]
+        A0, A1 = win.moments(Vb, refine_tol)
+        U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
+        r_used = r
+        if rk == 0:
+            diag.update(rk=0, r=r_used, sigma=S, seconds=time.perf_counter() - t0)
+            e = np.zeros((N, 0), dtype=np.complex128)
+            return (e, e, diag) if return_info else (e, e)
+        if rk == r:
+            r_tmp = r + r
+            while r_tmp < N:
+                Vb = probe_block(rng, N, r_tmp)
+                A0, A1 = win.moments(Vb, refine_tol)
+                U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
+                r_used = r_tmp
+                if rk < r_tmp or not grow_r:
+                    break
+                r_tmp += r_tmp
+        Uk = U[:, :rk]
+        Wk = Wh.conj().T[:, :rk]
+        tmp = (A1 @ Wk) / S[:rk][None, :]
+        B = Uk.conj().T @ tmp
+        diag.update(rk=rk, r=r_used, sigma=S, moments=dict(win.stats),
+                    seconds=time.perf_counter() - t0)
+        return (B, Uk, diag) if return_info else (B, Uk)
+    finally:
+        if own:
+            win.close()
+
+
+def solve_vect(solver, basis, pts, k: complex, dk: float, *, nq: int = 32, r: int = 48,
+               svd_tol: float = 1e-14, res_tol: float = 1e-8, rng=None, **kw):
+    """beyn.jl:364-375 -> (lam, Uk, Y, k, dk, pts)."""
+    N = boundary_matrix_size(pts)
+    B, Uk = construct_B_matrix(solver, pts, N, k, dk, nq=nq, r=r, svd_tol=svd_tol, rng=rng, **kw)
+    if B.size == 0:
+        return np.zeros(0, np.complex128), Uk, np.zeros((0, 0), np.complex128), k, dk, pts
+    lam, Y = np.linalg.eig(B)
+    return lam, Uk, Y, k, dk, pts
+
+
+def solve(solver, basis, pts, k: complex, dk: float, *, nq: int = 32, r: int = 48,
+          svd_tol: float = 1e-14, res_tol: float = 1e-8, rng=None, **kw):
+    """beyn.jl:386-397 -> lam (no filtering, like the reference)."""
+    N = boundary_matrix_size(pts)
+    B, Uk = construct_B_matrix(solver, pts, N, k, dk, nq=nq, r=r, svd_tol=svd_tol, rng=rng, **kw)
+    if B.size == 0:
+        return np.zeros(0, np.complex128)
+    return np.linalg.eigvals(B)

@@ -1,0 +1,39 @@
+"""Device context (qb_ctx): one device, one stream."""
+from __future__ import annotations
+
+import ctypes as C
+import threading
+
+from . import _lib
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        L = _lib.lib()
+        h = C.c_void_p()
+        rc = L.qb_ctx_create(int(device), C.byref(h))
+        if rc != _lib.QB_OK:
+            raise _lib.QBError(rc, f"qb_ctx_create(device={device}) failed (no usable GPU?)")
+        self.ptr = h
+        self.device = device
+
+    def device_bytes(self) -> int:
+        return int(_lib.lib().qb_ctx_device_bytes(self.ptr))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _lib.lib().qb_ctx_destroy(self.ptr)
+            self.ptr = None
+
+
+_default = {}
+_lock = threading.Lock()
+
+
+def default_context(device: int = 0) -> Context:
+    with _lock:
+        c = _default.get(device)
+        if c is None:
+            c = Context(device)
+            _default[device] = c
+        return c

@@ -1,0 +1,73 @@
+// Test harness: host-array wrappers around libqbhip's internal batched GEMM,
+// LU and triangular solves, so GPU unit tests can compare each against NumPy.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../quantumbilliards.jl_amd/csrc/qb_linalg.h"
+
+template <typename T>
+static int run_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, const T* B, T* C, int batch) {
+  T *dA, *dB, *dC;
+  size_t a = (size_t)M * K * batch, b = (size_t)K * N * batch, c = (size_t)M * N * batch;
+  if (hipMalloc(&dA, a * sizeof(T)) || hipMalloc(&dB, b * sizeof(T)) || hipMalloc(&dC, c * sizeof(T)))
+    return -2;
+  hipMemcpy(dA, A, a * sizeof(T), hipMemcpyHostToDevice);
+  hipMemcpy(dB, B, b * sizeof(T), hipMemcpyHostToDevice);
+  hipMemcpy(dC, C, c * sizeof(T), hipMemcpyHostToDevice);
+  int rc = gemm_sub<T>(ctx, M, N, K, dA, M, (int64_t)M * K, dB, K, (int64_t)K * N, dC, M,
+                       (int64_t)M * N, batch);
+  hipStreamSynchronize(ctx->stream);
+  hipMemcpy(C, dC, c * sizeof(T), hipMemcpyDeviceToHost);
+  hipFree(dA); hipFree(dB); hipFree(dC);
+  return rc;
+}
+
+template <typename T>
+static int run_getrf(qb_ctx* ctx, int n, T* A, int* ipiv, int batch) {
+  T* dA;
+  int *dP, *dI;
+  size_t a = (size_t)n * n * batch;
+  if (hipMalloc(&dA, a * sizeof(T)) || hipMalloc(&dP, (size_t)n * batch * sizeof(int)) ||
+      hipMalloc(&dI, batch * sizeof(int)))
+    return -2;
+  hipMemcpy(dA, A, a * sizeof(T), hipMemcpyHostToDevice);
+  hipMemset(dI, 0, batch * sizeof(int));
+  int rc = qb_getrf_batched<T>(ctx, dA, n, n, (int64_t)n * n, dP, n, dI, batch);
+  hipStreamSynchronize(ctx->stream);
+  hipMemcpy(A, dA, a * sizeof(T), hipMemcpyDeviceToHost);
+  hipMemcpy(ipiv, dP, (size_t)n * batch * sizeof(int), hipMemcpyDeviceToHost);
+  hipFree(dA); hipFree(dP); hipFree(dI);
+  return rc;
+}
+
+template <typename T>
+static int run_trsm(qb_ctx* ctx, int n, const T* LU, T* B, int nrhs, int batch) {
+  T *dA, *dB;
+  size_t a = (size_t)n * n * batch, b = (size_t)n * nrhs * batch;
+  if (hipMalloc(&dA, a * sizeof(T)) || hipMalloc(&dB, b * sizeof(T))) return -2;
+  hipMemcpy(dA, LU, a * sizeof(T), hipMemcpyHostToDevice);
+  hipMemcpy(dB, B, b * sizeof(T), hipMemcpyHostToDevice);
+  int rc = qb_trsm_lu_batched<T>(ctx, dA, n, n, (int64_t)n * n, dB, nrhs, n, (int64_t)n * nrhs,
+                                 batch);
+  hipStreamSynchronize(ctx->stream);
+  hipMemcpy(B, dB, b * sizeof(T), hipMemcpyDeviceToHost);
+  hipFree(dA); hipFree(dB);
+  return rc;
+}
+
+extern "C" {
+int qbt_gemm_sub_c128(qb_ctx* c, int M, int N, int K, const cd* A, const cd* B, cd* C, int b) {
+  return run_gemm<cd>(c, M, N, K, A, B, C, b);
+}
+int qbt_gemm_sub_c64(qb_ctx* c, int M, int N, int K, const cf* A, const cf* B, cf* C, int b) {
+  return run_gemm<cf>(c, M, N, K, A, B, C, b);
+}
+int qbt_getrf_c128(qb_ctx* c, int n, cd* A, int* ipiv, int b) { return run_getrf<cd>(c, n, A, ipiv, b); }
+int qbt_getrf_c64(qb_ctx* c, int n, cf* A, int* ipiv, int b) { return run_getrf<cf>(c, n, A, ipiv, b); }
+int qbt_trsm_c128(qb_ctx* c, int n, const cd* LU, cd* B, int nrhs, int b) {
+  return run_trsm<cd>(c, n, LU, B, nrhs, b);
+}
+int qbt_trsm_c64(qb_ctx* c, int n, const cf* LU, cf* B, int nrhs, int b) {
+  return run_trsm<cf>(c, n, LU, B, nrhs, b);
+}
+}

@@ -1,0 +1,114 @@
+"""GPU parity of the hot path against the oracle (oracle/oracle.py)."""
+import numpy as np
+import pytest
+import scipy.special as sp
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def qb():
+    import qbhip
+    return qbhip
+
+
+def _pts(kind, Ns, **kw):
+    q = qb()
+    if kind == "circle":
+        b = q.make_circle(kw.get("R", 1.0))
+    elif kind == "ellipse":
+        b = q.make_full_ellipse(1.0, 0.6)
+    elif kind == "star":
+        b = q.make_star_polar_boundary(1.0, 0.2, 5)
+    elif kind == "circle_hole":
+        b = q.CircularHoleBilliard(1.0, holes=[(0.3, 0.35, 0.0)])
+    return b, q.evaluate_points(q.CFIE_kress(10.0, b), b, 1.0, N=Ns)
+
+
+def _relerr(A, B):
+    """max |A-B| / max |B| per matrix, and entrywise relative error over
+    entries at least 1e-3 of the matrix maximum."""
+    out = []
+    for a, b in zip(A, B):
+        s = np.max(np.abs(b))
+        m = np.abs(b) >= 1e-3 * s
+        out.append((np.max(np.abs(a - b)) / s, np.max(np.abs(a - b)[m] / np.abs(b)[m])))
+    return np.array(out)
+
+
+def test_bessel_device_matches_amos():
+    rng = np.random.default_rng(3)
+    rho = np.exp(rng.uniform(np.log(1e-3), np.log(2500), 20000))
+    im = rng.uniform(-1.2, 1.2, rho.size)
+    re = np.sqrt(np.maximum(rho ** 2 - im ** 2, 1e-6))
+    z = re + 1j * im
+    h0, h1, j0, j1 = qb().bessel_eval(z)
+    for a, b in ((h0, sp.hankel1(0, z)), (h1, sp.hankel1(1, z))):
+        assert np.max(np.abs(a - b) / np.abs(b)) < 2e-14
+    for a, b in ((j0, sp.jv(0, z)), (j1, sp.jv(1, z))):
+        scale = np.maximum(np.abs(b), np.minimum(1.0, np.exp(np.abs(z.imag)) / np.sqrt(np.abs(z))))
+        assert np.max(np.abs(a - b) / scale) < 1e-14
+
+
+@pytest.mark.parametrize("kind,Ns,k0", [("circle", [256], 20.0), ("star", [512], 60.0),
+                                         ("circle_hole", [200, 60], 25.0)])
+def test_cfie_assembly_matches_oracle(kind, Ns, k0):
+    q = qb()
+    b, pts = _pts(kind, Ns)
+    zj, _ = O.contour(k0, 0.5, 8)
+    cp = dict(n_panels_h=4000, M_h=5, n_panels_j=1500, M_j=5)
+    T = q.assemble(q.CFIE_kress(10.0, b), pts, zj, **cp)
+    comps = O.make_geometry(kind, Ns)
+    Tref = O.cfie_assemble(comps, zj, O.ChebParams(4000, 5, 1500, 5))
+    e = _relerr(T, Tref)
+    print(kind, e.max(axis=0))
+    assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
+
+
+def test_dlp_assembly_matches_oracle():
+    q = qb()
+    b = q.make_full_ellipse(1.0, 0.6)
+    pts = q.evaluate_points(q.DLP_kress(10.0, b), b, 1.0, N=[384])
+    zj, _ = O.contour(40.0, 0.5, 6)
+    T = q.assemble(q.DLP_kress(10.0, b), pts, zj, n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
+    Tref = O.dlp_assemble(O.make_geometry("ellipse", [384])[0], zj, O.ChebParams(3000, 5, 1000, 5))
+    e = _relerr(T, Tref)
+    print(e.max(axis=0))
+    assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_window_moments_match_host_solve(precision):
+    """LU + solves + moments on the device against LAPACK on the same T."""
+    q = qb()
+    b, pts = _pts("star", [320])
+    solver = q.CFIE_kress(10.0, b)
+    k0, R, nq = 30.0 + 0.0j, 0.4, 6
+    zj, wj = O.contour(k0, R, nq)
+    cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
+    T = q.assemble(solver, pts, zj, **cp)
+    V = q.probe_block(np.random.default_rng(5), 320, 16)
+    A0r, A1r = O.beyn_moments(np.array(T), zj, wj, V)
+    win = q.Window(solver, pts, k0, R, nq, precision=precision, **cp)
+    A0, A1 = win.moments(V, 1e-13)
+    print(win.stats)
+    for a, r in ((A0, A0r), (A1, A1r)):
+        assert np.max(np.abs(a - r)) / np.max(np.abs(r)) < (1e-11 if precision == "fp64" else 1e-10)
+    win.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_beyn_circle_gives_bessel_zeros(precision):
+    """cfg1: unit circle, N=256, k0=20, R=0.5, nq=32, 24 probes, svd_tol 1e-10."""
+    q = qb()
+    b = q.make_circle(1.0)
+    solver = q.CFIE_kress(10.0, b)
+    pts = q.evaluate_points(solver, b, 20.0, N=[256])
+    lam = q.solve(solver, None, pts, 20.0 + 0j, 0.5, nq=32, r=24, svd_tol=1e-10,
+                  rng=np.random.default_rng(1), precision=precision)
+    inside = np.sort(lam[np.abs(lam - 20.0) <= 0.5].real)
+    ex = O.jm_zeros_in_disk(20.0, 0.5)
+    print(precision, inside - ex if len(inside) == len(ex) else (inside, ex))
+    assert len(inside) == len(ex)
+    assert np.max(np.abs(inside - ex) / ex) < 1e-12
