@@ -69,6 +69,27 @@ const char* qb_last_error(const qb_ctx* ctx);
 /* Bytes currently held on the device by this context's objects. */
 int64_t qb_ctx_device_bytes(const qb_ctx* ctx);
 
+/* Kernel classes for the per-context launch accounting below. */
+enum {
+  QB_K_TABLES = 0,    /* Chebyshev table construction          */
+  QB_K_ASSEMBLE = 1,  /* CFIE/DLP-Kress assembly               */
+  QB_K_GEMM_C64 = 2,  /* complex64 C -= A*B (MFMA)             */
+  QB_K_GEMM_C128 = 3, /* complex128 C -= A*B (MFMA)            */
+  QB_K_PANEL = 4,     /* leaf panel factorisation              */
+  QB_K_TRSM = 5,      /* triangular-solve leaves               */
+  QB_K_OTHER = 6,     /* swaps, gathers, norms, moments        */
+  QB_K_COUNT = 7
+};
+/* Enable (1) / disable (0) recording a pair of HIP events on the context's
+ * stream around every launch, and clear the counters.  No counterpart in the
+ * reference (its @benchit timings are wall-clock, macros.jl:52-91). */
+int qb_ctx_set_profiling(qb_ctx* ctx, int32_t enable);
+/* Totals since profiling was enabled for one kernel class: device time (ms,
+ * from the events), launches, and algorithmic flops / bytes of those
+ * launches (complex GEMM: 8 M N K flops; assembly: 16 B per entry written). */
+int qb_ctx_kernel_stats(qb_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches,
+                        double* flops, double* bytes);
+
 /* Boundary discretisation, one or more periodic components, holes already
  * orientation-reversed by the caller.  Replaces the inputs the reference
  * builds in evaluate_points (cfie.jl:822-832, dlp_kress.jl:563-606) and the

@@ -316,6 +316,20 @@ def _rrange_blocks(comps):
     return rmin, rmax
 
 
+def _rrange_single_fast(c: Comp):
+    """Padded [rmin, rmax] of one component, computed in column blocks."""
+    rmin, rmax = np.inf, 0.0
+    n = len(c.x)
+    for j0 in range(0, n, 1024):
+        js = np.arange(j0, min(n, j0 + 1024))
+        R = np.hypot(c.x[:, None] - c.x[None, js], c.y[:, None] - c.y[None, js])
+        R[js, np.arange(len(js))] = 0.0
+        m = R > np.finfo(float).eps
+        rmin = min(rmin, float(np.min(R[m])))
+        rmax = max(rmax, float(np.max(R[m])))
+    return rmin * 0.95, rmax * 1.05
+
+
 def _hankel_direct(z):
     """Direct branch of h0_h1_multi_ks_at_r! (bessels:1190-1206): |z|<1e-3
     small series, else AMOS."""
@@ -327,11 +341,63 @@ def _hankel_direct(z):
 # --------------------------------------------------------------------------
 
 
-def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None):
+def cfie_assemble_cols(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int, rmin: float,
+                       rmax: float):
+    """Columns [j0, j1) of the single-component CFIE matrix at one z (the
+    same-block formulas of cheb-CFIE :579-615), for bounded CPU timing at
+    large N without N x N caches."""
+    n = len(comp.x)
+    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+    H0 = build_table("h", 0, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+    H1 = build_table("h", 1, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+    J0 = build_table("j", 0, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+    J1 = build_table("j", 1, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+    js = np.arange(j0, j1)
+    DX = comp.x[:, None] - comp.x[None, js]
+    DY = comp.y[:, None] - comp.y[None, js]
+    r = np.hypot(DX, DY)
+    diag = np.arange(n)[:, None] == js[None, :]
+    r[diag] = 1.0
+    invr = 1.0 / r
+    inner = comp.dy[None, js] * DX - comp.dx[None, js] * DY
+    d = (np.arange(n)[:, None] - js[None, :]) % n
+    Rk = kress_R_col(n)[d]
+    with np.errstate(divide="ignore"):
+        lt = np.log(4.0 * np.sin((comp.ts[:, None] - comp.ts[None, js]) / 2.0) ** 2)
+    lt[diag] = 0.0
+    pidx, th = ph.locate(r)
+    pjx, tj = pj.locate(r)
+    h0, h1 = clenshaw(H0[pidx], th), clenshaw(H1[pidx], th)
+    j0v, j1v = clenshaw(J0[pjx], tj), clenshaw(J1[pjx], tj)
+    sj = np.sqrt(comp.dx[js] ** 2 + comp.dy[js] ** 2)[None, :]
+    wj = comp.ws[js][None, :]
+    cD = inner * invr
+    aL1, aL2, ik = -z * INV_TWO_PI, 0.5j * z, 1j * z
+    D = (Rk * cD) * (aL1 * j1v) + (wj * cD) * (aL2 * h1) - (wj * lt * cD) * (aL1 * j1v)
+    S = (Rk * sj) * (-INV_TWO_PI * j0v) + (wj * sj) * (0.5j * h0) - (wj * sj * lt) * (-INV_TWO_PI * j0v)
+    A = -(D + ik * S)
+    # diagonal limits (cheb-CFIE :585-591) for the diagonal entries in the block
+    ii = js[(js >= 0) & (js < n)]
+    s_ = np.sqrt(comp.dx[ii] ** 2 + comp.dy[ii] ** 2)
+    kap = INV_TWO_PI * (-(comp.dx[ii] * comp.ddy[ii] - comp.dy[ii] * comp.ddx[ii]) / s_ ** 2)
+    w_ = comp.ws[ii]
+    m2 = ((0.5j - EULER_OVER_PI) - INV_TWO_PI * np.log((z * z / 4.0) * (s_ * s_))) * s_
+    sval = kress_R_col(n)[0] * (-INV_TWO_PI * s_) + w_ * m2
+    A[ii, ii - j0] = 1.0 - (w_ * kap + ik * sval)
+    return A
+
+
+def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None, cols=None):
     """construct_boundary_matrices! for CFIE_kress (cheb-CFIE:1431-1450 ->
     _all_k_nosymm_CFIE_chebyshev! :543-665).  Returns [nz, N, N] with
     A(z) = I - (D + i z S), row = target, column = source."""
     zs = np.asarray(zs, dtype=np.complex128)
+    if cols is not None:
+        assert len(comps) == 1
+        c = comps[0]
+        rmin, rmax = _rrange_single_fast(c)
+        return np.stack([cfie_assemble_cols(c, z, cp, cols[0], cols[1], rmin, rmax) for z in zs])
     rmin, rmax = _rrange_blocks(comps)
     ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
     pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)

@@ -40,6 +40,51 @@ void qb_dev_free(qb_ctx* ctx, void* p, size_t bytes) {
   ctx->bytes -= (int64_t)bytes;
 }
 
+// ---------------------------------------------------------------------------
+// launch accounting
+// ---------------------------------------------------------------------------
+static void prof_reduce(qb_ctx* ctx) {
+  if (ctx->prof_recs.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->prof_recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      ctx->prof_ms[r.kind] += ms;
+      ctx->prof_n[r.kind] += 1;
+      ctx->prof_flops[r.kind] += r.flops;
+      ctx->prof_bytes[r.kind] += r.bytes;
+    }
+    ctx->prof_pool.push_back(r.a);
+    ctx->prof_pool.push_back(r.b);
+  }
+  ctx->prof_recs.clear();
+}
+
+static hipEvent_t prof_event(qb_ctx* ctx) {
+  if (!ctx->prof_pool.empty()) {
+    hipEvent_t e = ctx->prof_pool.back();
+    ctx->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+int qb_prof_begin(qb_ctx* ctx, int kind, double flops, double bytes) {
+  if (!ctx->prof) return -1;
+  if (ctx->prof_recs.size() >= 8192) prof_reduce(ctx);
+  QbProfRec r{prof_event(ctx), prof_event(ctx), kind, flops, bytes};
+  (void)hipEventRecord(r.a, ctx->stream);
+  ctx->prof_recs.push_back(r);
+  return (int)ctx->prof_recs.size() - 1;
+}
+
+void qb_prof_end(qb_ctx* ctx, int slot) {
+  if (slot < 0) return;
+  (void)hipEventRecord(ctx->prof_recs[slot].b, ctx->stream);
+}
+
 static double now_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -68,6 +113,8 @@ int qb_ctx_create(int device, qb_ctx** out) {
 void qb_ctx_destroy(qb_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  prof_reduce(ctx);
+  for (hipEvent_t e : ctx->prof_pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -75,6 +122,33 @@ void qb_ctx_destroy(qb_ctx* ctx) {
 const char* qb_last_error(const qb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int64_t qb_ctx_device_bytes(const qb_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int qb_ctx_set_profiling(qb_ctx* ctx, int32_t enable) {
+  if (!ctx) return QB_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  prof_reduce(ctx);
+  for (int k = 0; k < QB_K_COUNT; ++k) {
+    ctx->prof_ms[k] = 0;
+    ctx->prof_n[k] = 0;
+    ctx->prof_flops[k] = 0;
+    ctx->prof_bytes[k] = 0;
+  }
+  ctx->prof = enable != 0;
+  return QB_OK;
+}
+
+int qb_ctx_kernel_stats(qb_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches,
+                        double* flops, double* bytes) {
+  if (!ctx) return QB_E_ARG;
+  if (kind < 0 || kind >= QB_K_COUNT) return qb_fail(ctx, QB_E_ARG, "unknown kernel class");
+  (void)hipSetDevice(ctx->device);
+  prof_reduce(ctx);
+  if (total_ms) *total_ms = ctx->prof_ms[kind];
+  if (launches) *launches = ctx->prof_n[kind];
+  if (flops) *flops = ctx->prof_flops[kind];
+  if (bytes) *bytes = ctx->prof_bytes[kind];
+  return QB_OK;
+}
 
 }  // extern "C"
 

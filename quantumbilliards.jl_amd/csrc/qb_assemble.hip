@@ -178,8 +178,12 @@ static void launch_asm(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, i
                        const DevTables& t, cd* o64, cf* o32) {
   int64_t N = g->d.N;
   dim3 grid((unsigned)((N + 63) / 64), (unsigned)((N + 15) / 16), (unsigned)nzc);
+  double ent = (double)N * N * nzc;
+  int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
+                           ent * ((o64 ? sizeof(cd) : 0) + (o32 ? sizeof(cf) : 0)));
   hipLaunchKernelGGL((k_assemble<OP, MH, MJ>), grid, dim3(64, 4), 0, ctx->stream, g->d, z, t.H,
                      t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0, o64, o32);
+  qb_prof_end(ctx, slot);
 }
 
 int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,

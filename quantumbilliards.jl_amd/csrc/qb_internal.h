@@ -30,12 +30,31 @@ struct DevGeom {
   const double* lt;     // log(4 sin^2((t_i - t_j)/2)) circulant columns (length N)
 };
 
+struct QbProfRec {
+  hipEvent_t a, b;
+  int kind;
+  double flops, bytes;
+};
+
 struct qb_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   int64_t bytes = 0;  // device bytes held by objects of this context
+  bool prof = false;
+  std::vector<QbProfRec> prof_recs;      // recorded, not yet reduced
+  std::vector<hipEvent_t> prof_pool;     // reusable events
+  double prof_ms[QB_K_COUNT] = {0};
+  int64_t prof_n[QB_K_COUNT] = {0};
+  double prof_flops[QB_K_COUNT] = {0};
+  double prof_bytes[QB_K_COUNT] = {0};
 };
+
+// Launch accounting: qb_prof_begin records an event before a launch and
+// returns its slot (or -1 when profiling is off); qb_prof_end records the
+// closing event.
+int qb_prof_begin(qb_ctx* ctx, int kind, double flops, double bytes);
+void qb_prof_end(qb_ctx* ctx, int slot);
 
 struct qb_geom {
   qb_ctx* ctx = nullptr;

@@ -152,6 +152,9 @@ template <typename T>
 int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA, const T* B,
              int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
+  int slot = qb_prof_begin(ctx, sizeof(T) == sizeof(cf) ? QB_K_GEMM_C64 : QB_K_GEMM_C128,
+                           8.0 * M * N * K * (double)batch,
+                           (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
   if (sizeof(T) == sizeof(cf)) {
     constexpr int BM = 128, BN = 128, BK = 16;
     dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
@@ -163,6 +166,7 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
     hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK>), g, dim3(256), 0, ctx->stream, M, N, K, A,
                        lda, sA, B, ldb, sB, C, ldc, sC);
   }
+  qb_prof_end(ctx, slot);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
@@ -369,8 +373,10 @@ static int trsm_lower(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int
   if (n <= 0 || nrhs <= 0) return QB_OK;
   if (n <= 32) {
     dim3 g((nrhs + 255) / 256, batch);
+    int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
     hipLaunchKernelGGL((k_trsm_leaf<T, true>), g, dim3(256), 0, ctx->stream, n, nrhs, L, ldl, sL,
                        B, ldb, sB);
+    qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
   }
@@ -390,8 +396,10 @@ static int trsm_upper(qb_ctx* ctx, int n, int nrhs, const T* U, int64_t ldu, int
   if (n <= 0 || nrhs <= 0) return QB_OK;
   if (n <= 32) {
     dim3 g((nrhs + 255) / 256, batch);
+    int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
     hipLaunchKernelGGL((k_trsm_leaf<T, false>), g, dim3(256), 0, ctx->stream, n, nrhs, U, ldu,
                        sU, B, ldb, sB);
+    qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
   }
@@ -410,8 +418,10 @@ template <typename T>
 static int panel_rec(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int n, int c0, int w, int* ipiv,
                      int64_t sP, int* info, int batch) {
   if (w <= 32) {
+    int slot = qb_prof_begin(ctx, QB_K_PANEL, 8.0 * (double)(n - c0) * w * w / 2 * batch, 0);
     hipLaunchKernelGGL(k_getf2<T>, dim3(batch), dim3(1024), 0, ctx->stream, A, lda, sA, n, c0, w,
                        ipiv, sP, info);
+    qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
   }

@@ -118,11 +118,13 @@ int qb_tables_build(qb_ctx* ctx, const double* z_dev, int64_t nz, double rmin_h,
   LobCoef lh = make_lob(cp.M_h), lj = make_lob(cp.M_j);
   const cd* zk = (const cd*)z_dev;
   int64_t nh = nz * (int64_t)cp.n_panels_h, nj = nz * (int64_t)cp.n_panels_j;
+  int slot = qb_prof_begin(ctx, QB_K_TABLES, 0.0, (double)(hb + jb));
   hipLaunchKernelGGL(k_build_tables<0>, dim3((unsigned)((nh + 63) / 64)), dim3(64), 0, ctx->stream,
                      zk, nz, rmin_h, rmax, cp.n_panels_h, lh, t->H);
   QB_HIP(ctx, hipGetLastError());
   hipLaunchKernelGGL(k_build_tables<1>, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, ctx->stream,
                      zk, nz, 0.0, rmax, cp.n_panels_j, lj, t->J);
+  qb_prof_end(ctx, slot);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }

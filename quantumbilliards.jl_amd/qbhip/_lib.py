@@ -21,8 +21,9 @@ EXPORTS = (
     "qb_ctx_create", "qb_ctx_destroy", "qb_last_error", "qb_ctx_device_bytes",
     "qb_geom_create", "qb_geom_destroy", "qb_geom_size", "qb_assemble",
     "qb_window_create", "qb_window_destroy", "qb_window_footprint", "qb_window_moments",
-    "qb_bessel_eval", "qb_cheb_tune",
+    "qb_bessel_eval", "qb_cheb_tune", "qb_ctx_set_profiling", "qb_ctx_kernel_stats",
 )
+KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
 
 class QBError(RuntimeError):
@@ -79,6 +80,9 @@ def lib():
         "qb_window_moments": (C.c_int, [P, I32, P, P, P, D, C.POINTER(Stats)]),
         "qb_bessel_eval": (C.c_int, [P, P, I64, P, P, P, P]),
         "qb_cheb_tune": (C.c_int, [P, P, I32, P, I64, D, I32, I32, C.POINTER(ChebParams), P]),
+        "qb_ctx_set_profiling": (C.c_int, [P, I32]),
+        "qb_ctx_kernel_stats": (C.c_int, [P, I32, C.POINTER(D), C.POINTER(I64), C.POINTER(D),
+                                          C.POINTER(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

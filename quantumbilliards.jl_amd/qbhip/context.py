@@ -20,6 +20,20 @@ class Context:
     def device_bytes(self) -> int:
         return int(_lib.lib().qb_ctx_device_bytes(self.ptr))
 
+    def set_profiling(self, enable: bool = True):
+        """Record HIP events around every launch (and clear the counters)."""
+        _lib.check(self.ptr, _lib.lib().qb_ctx_set_profiling(self.ptr, 1 if enable else 0))
+
+    def kernel_stats(self) -> dict:
+        """Per kernel class: device ms, launches, algorithmic flops and bytes."""
+        out = {}
+        for k, name in enumerate(_lib.KERNEL_CLASSES):
+            ms, n, fl, by = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
+            _lib.check(self.ptr, _lib.lib().qb_ctx_kernel_stats(self.ptr, k, C.byref(ms), C.byref(n),
+                                                                C.byref(fl), C.byref(by)))
+            out[name] = {"ms": ms.value, "launches": n.value, "flops": fl.value, "bytes": by.value}
+        return out
+
     def close(self):
         if getattr(self, "ptr", None):
             _lib.lib().qb_ctx_destroy(self.ptr)
