@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--r", type=int, default=0, help="fix the probe count (skip growth)")
     ap.add_argument("--svd-tol", type=float, default=1e-10)
     ap.add_argument("--cheb-tol", type=float, default=1e-12)
-    ap.add_argument("--refine-tol", type=float, default=1e-13)
+    ap.add_argument("--refine-tol", type=float, default=1e-12)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=512)
@@ -73,7 +73,9 @@ def window(q, solver, pts, geom, a, cheb, precision, r, rng_seed, grow):
                                        svd_tol=a.svd_tol, rng=np.random.default_rng(rng_seed),
                                        precision=precision, refine_tol=a.refine_tol,
                                        grow_r=grow, window=win, return_info=True)
+    te = time.perf_counter()
     lam = np.linalg.eigvals(B) if B.size else np.zeros(0, complex)
+    info["t_host_eig_s"] = time.perf_counter() - te
     dt = time.perf_counter() - t0
     st = dict(win.stats)
     win.close()
@@ -134,12 +136,14 @@ def main():
     cheb = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
     print(f"# cheb tables {cheb} errs {errs}", file=sys.stderr, flush=True)
 
-    # warmup: grow r (mixed path), keep the r that stops saturating
-    r_used = a.r or a.r0
+    # probes from Weyl's law for the captured interval; warmup doubles r if
+    # the rank still saturates and keeps the r that does not
+    r_used = a.r or q.probe_count(b, a.k0, a.R, a.nq, a.svd_tol, a.r0)
     for w in range(max(1, a.warmup)):
         lam_m, dt, info = window(q, solver, pts, geom, a, cheb, "mixed", r_used, 1, a.r == 0)
         r_used = info["r"]
-        print(f"# warmup {w}: {dt:.2f}s rk={info['rk']} r={r_used} stats={info['window_stats']}",
+        print(f"# warmup {w}: {dt:.2f}s rk={info['rk']} r={r_used} svd={info['t_host_svd_s']:.2f}s "
+              f"eig={info['t_host_eig_s']:.2f}s stats={info['window_stats']}",
               file=sys.stderr, flush=True)
     ref = None
     fp64_s = None
@@ -162,6 +166,7 @@ def main():
         counts.append(n)
         worst = max(worst, wv) if not math.isnan(wv) else worst
         print(f"# step {s}: {dt:.3f}s eig {n}/{nin} worst {wv:.2e} rk={info['rk']} "
+              f"svd={info['t_host_svd_s']:.2f}s eig={info['t_host_eig_s']:.2f}s "
               f"stats={info['window_stats']}", file=sys.stderr, flush=True)
     ks = ctx.kernel_stats()
     ctx.set_profiling(False)

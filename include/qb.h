@@ -56,7 +56,7 @@ typedef struct {
   int32_t precision;
   int32_t refine_iters_max;     /* max refinement steps over nodes (mixed)   */
   int32_t nodes_fp64_fallback;  /* nodes refactored in FP64 (mixed)          */
-  double refine_resid_max;      /* max final normwise backward error (mixed) */
+  double refine_resid_max;      /* max final backward error of refined nodes */
   double t_assemble_ms, t_factor_ms, t_solve_ms;
   double rmin, rmax;            /* padded distance range of the tables       */
 } qb_stats;
@@ -130,8 +130,10 @@ int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r)
 
 /* The moment loop of construct_B_matrix (beyn.jl:278-285): X_j = T(z_j)^{-1} V,
  * A0 = sum_j w_j X_j, A1 = sum_j w_j z_j X_j, w_j = (R/nq) e^{i 2pi j/nq}.
- * V, A0, A1: host N x r complex128.  refine_tol: relative correction at
- * which mixed-precision refinement stops (ignored for QB_FP64). */
+ * V, A0, A1: host N x r complex128.  refine_tol: normwise backward error
+ * ||V - T X||_max / (||T||_inf ||X||_max + ||V||_max) at which the
+ * mixed-precision refinement of a node stops; a node whose error stops
+ * halving is refactored and solved in FP64 (ignored for QB_FP64). */
 int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0,
                       double* A1, double refine_tol, qb_stats* stats);
 

@@ -712,18 +712,26 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
       goto fail;
     }
     QB_TRY(qb_bcast(ctx, Vd, Rr, nv, nq));
-    std::vector<char> done(nq, 0);
-    std::vector<double> prev_corr(nq, 1e300);
+    // host copies of ||T_j||_inf and ||V||_max for the backward error
+    std::vector<double> tn(nq);
+    double vmax = 0.0;
+    for (int64_t i = 0; i < nv; ++i) vmax = std::max(vmax, hypot(V[2 * i], V[2 * i + 1]));
+    if (hipMemcpyAsync(tn.data(), w->tnorm, nq * sizeof(double), hipMemcpyDeviceToHost,
+                       ctx->stream) != hipSuccess) {
+      rc = qb_fail(ctx, QB_E_HIP, "norm download failed");
+      goto fail;
+    }
+    std::vector<double> be(nq, 1e300), prev(nq, 1e300);
+    std::vector<char> stag(nq, 0);
     std::vector<int> stagnant;
     const int maxit = 8;
     int it = 0;
     for (; it < maxit; ++it) {
+      // D = LU32^{-1} P R (complex64), X += D, R = V - T X (FP64)
       QB_TRY(qb_gather_rows_cd2cf(ctx, Rr, nv, w->perm, D32, (int)N, r, nq));
       QB_TRY(qb_trsm_lu_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, D32, r, N, nv, nq));
       QB_TRY(qb_add_cf_to_cd(ctx, D32, X, (int64_t)nq * nv));
-      QB_TRY(qb_node_maxabs32(ctx, D32, nv, nq, w->scratch_d));
       QB_TRY(qb_node_maxabs(ctx, X, nv, nq, w->scratch_d + nq));
-      // R = V - T X  (FP64)
       QB_TRY(qb_bcast(ctx, Vd, Rr, nv, nq));
       QB_TRY(gemm_sub<cd>(ctx, (int)N, r, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N, nv,
                           nq));
@@ -734,29 +742,21 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
         rc = qb_fail(ctx, QB_E_HIP, "refinement norms failed");
         goto fail;
       }
-      bool all = true;
+      bool active = false;
       for (int j = 0; j < nq; ++j) {
-        double corr = h3[j] / (h3[nq + j] > 0 ? h3[nq + j] : 1.0);
-        if (!(corr <= refine_tol)) {
-          all = false;
-          if (it >= 2 && !(corr < 0.5 * prev_corr[j])) done[j] = 2;  // stagnating
-        } else if (done[j] == 0) {
-          done[j] = 1;
-        }
-        prev_corr[j] = corr;
+        // normwise backward error ||V - T X|| / (||T|| ||X|| + ||V||)
+        be[j] = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
+        if (be[j] <= refine_tol) continue;
+        if (it >= 1 && !(be[j] < 0.5 * prev[j])) stag[j] = 1;
+        if (!stag[j]) active = true;
+        prev[j] = be[j];
       }
-      if (all) break;
-      bool any_active = false;
-      for (int j = 0; j < nq; ++j)
-        if (done[j] == 0) any_active = true;
-      if (!any_active) break;
+      if (!active) break;
     }
     iters_max = it + 1 > maxit ? maxit : it + 1;
-    // nodes that did not converge: FP64 refactorisation and solve
-    for (int j = 0; j < nq; ++j) {
-      double corr = h3[j] / (h3[nq + j] > 0 ? h3[nq + j] : 1.0);
-      if (!(corr <= refine_tol)) stagnant.push_back(j);
-    }
+    // nodes that did not reach the tolerance: FP64 refactorisation and solve
+    for (int j = 0; j < nq; ++j)
+      if (!(be[j] <= refine_tol)) stagnant.push_back(j);
     if (!stagnant.empty()) {
       std::vector<int> fresh;
       for (int j : stagnant)
@@ -771,20 +771,8 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
                                       N, nv, 1));
       }
     }
-    {
-      std::vector<double> tn(nq);
-      if (hipMemcpyAsync(tn.data(), w->tnorm, nq * sizeof(double), hipMemcpyDeviceToHost,
-                         ctx->stream) != hipSuccess ||
-          hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        rc = qb_fail(ctx, QB_E_HIP, "norm download failed");
-        goto fail;
-      }
-      for (int j = 0; j < nq; ++j) {
-        if (std::find(stagnant.begin(), stagnant.end(), j) != stagnant.end()) continue;
-        double be = h3[2 * nq + j] / (tn[j] * h3[nq + j] + 1e-300);
-        resid_max = std::max(resid_max, be);
-      }
-    }
+    for (int j = 0; j < nq; ++j)
+      if (be[j] <= refine_tol) resid_max = std::max(resid_max, be[j]);
     w->st.nodes_fp64_fallback = (int)w->fb_nodes.size();
   }
   QB_TRY(qb_moments(ctx, X, nv, nq, w->w_dev, w->w_dev + nq, A0d, A1d));

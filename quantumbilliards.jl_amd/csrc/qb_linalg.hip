@@ -9,7 +9,7 @@
 // are column-major, complex interleaved.  T = cf (complex64) or cd
 // (complex128).
 //
-// Structure (right-looking, NB = 128):
+// Structure (right-looking, NB = 256):
 //   panel (recursive, leaf width 32: single-workgroup column elimination)
 //   -> row swaps on the other columns -> unit-lower TRSM of the block row
 //   -> trailing update C -= A*B on MFMA.
@@ -47,13 +47,24 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
 // ---------------------------------------------------------------------------
 // GEMM: C[b] -= A[b] * B[b]  (A: M x K, B: K x N, C: M x N)
 // ---------------------------------------------------------------------------
-// Workgroup tile BM x BN, k-tile BK, 4 waves as 2 x 2, wave tile
-// (BM/2) x (BN/2) built from 16 x 16 MFMA blocks.  MFMA 16x16x4 operand
-// layout (one value per lane): A[i][k] with i = lane % 16, k = lane / 16;
-// B[k][j] with j = lane % 16, k = lane / 16.  Result register v of a lane
-// holds D[4*(lane/16) + v][lane % 16] for the f32 instruction and
-// D[4*v + lane/16][lane % 16] for the f64 one (checked on gfx950).
-template <typename T, int BM, int BN, int BK>
+// Workgroup tile BM x BN, k-tile BK, 4 waves arranged WGM x WGN; each wave
+// computes a (BM/WGM) x (BN/WGN) tile of 16 x 16 MFMA blocks, one
+// accumulator block for the real and one for the imaginary part.  MFMA
+// 16x16x4 operand layout (one value per lane): A[i][k] with i = lane % 16,
+// k = lane / 16; B[k][j] with j = lane % 16, k = lane / 16.  Result
+// register v of a lane holds D[4*(lane/16) + v][lane % 16] for the f32
+// instruction and D[4*v + lane/16][lane % 16] for the f64 one (checked on
+// gfx950).
+//
+// Pipeline: the next k-tile is loaded from global memory into registers
+// while the current one is multiplied out of LDS; the two LDS buffers
+// alternate, so each k-tile costs one barrier.  A is stored k-major
+// ([k][m], row pad of 16 words puts the four k rows a wave reads in
+// different bank halves), B n-major ([n][k], pad 1) so that both the
+// coalesced global loads and the fragment reads are conflict-free.  A is
+// negated when it is written to LDS, so the accumulators hold -A*B and the
+// epilogue adds them to C.
+template <typename T, int BM, int BN, int BK, int WGM, int WGN>
 __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
                                                   int64_t lda, int64_t sA,
                                                   const T* __restrict__ B, int64_t ldb,
@@ -61,19 +72,27 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
                                                   int64_t sC) {
   typedef typename Scal<T>::S S;
   typedef typename Scal<T>::V4 V4;
-  constexpr int WM = BM / 2, WN = BN / 2;
+  static_assert(WGM * WGN == 4, "4 waves");
+  constexpr int WM = BM / WGM, WN = BN / WGN;
   constexpr int FM = WM / 16, FN = WN / 16;
-  __shared__ S As_re[BK][BM + 1], As_im[BK][BM + 1];
-  __shared__ S Bs_re[BK][BN + 1], Bs_im[BK][BN + 1];
+  constexpr int APAD = (sizeof(S) == 4) ? 16 : 8;
+  constexpr int AST = BM + APAD;       // A row stride (elements)
+  constexpr int BST = BK + 1;          // B row stride
+  constexpr int NA = BM * BK / 256;    // complex elements of A per thread per tile
+  constexpr int NB = BN * BK / 256;
+  static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "tile/thread mismatch");
+  __shared__ S As[2][2][BK * AST];     // [buf][re/im]
+  __shared__ S Bs[2][2][BN * BST];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int wm0 = (wave % WGM) * WM, wn0 = (wave / WGM) * WN;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int64_t b = blockIdx.z;
   A += b * sA;
   B += b * sB;
   C += b * sC;
+  const int li = lane & 15, lk = lane >> 4;
 
   V4 acc_re[FM][FN], acc_im[FM][FN];
 #pragma unroll
@@ -84,37 +103,66 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
       acc_im[i][j] = V4{0, 0, 0, 0};
     }
 
-  const int li = lane & 15, lk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    // global -> LDS, A negated so the accumulators hold -A*B
-    for (int e = tid; e < BM * BK; e += 256) {
+  T ra[NA], rb[NB];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+      int e = tid + 256 * p;
       int m = e % BM, kk = e / BM;
       int gm = m0 + m, gk = k0 + kk;
-      T v = (gm < M && gk < K) ? A[gm + (int64_t)gk * lda] : T{0, 0};
-      As_re[kk][m] = -v.re;
-      As_im[kk][m] = -v.im;
+      ra[p] = (gm < M && gk < K) ? A[gm + (int64_t)gk * lda] : T{0, 0};
     }
-    for (int e = tid; e < BN * BK; e += 256) {
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      int e = tid + 256 * p;
       int kk = e % BK, n = e / BK;
       int gn = n0 + n, gk = k0 + kk;
-      T v = (gn < N && gk < K) ? B[gk + (int64_t)gn * ldb] : T{0, 0};
-      Bs_re[kk][n] = v.re;
-      Bs_im[kk][n] = v.im;
+      rb[p] = (gn < N && gk < K) ? B[gk + (int64_t)gn * ldb] : T{0, 0};
     }
-    __syncthreads();
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+      int e = tid + 256 * p;
+      int m = e % BM, kk = e / BM;
+      As[buf][0][kk * AST + m] = -ra[p].re;
+      As[buf][1][kk * AST + m] = -ra[p].im;
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      int e = tid + 256 * p;
+      int kk = e % BK, n = e / BK;
+      Bs[buf][0][n * BST + kk] = rb[p].re;
+      Bs[buf][1][n * BST + kk] = rb[p].im;
+    }
+  };
+
+  const int nt = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int t = 0; t < nt; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < nt) gload((t + 1) * BK);
+    const S* Ar = As[buf][0];
+    const S* Ai = As[buf][1];
+    const S* Br = Bs[buf][0];
+    const S* Bi = Bs[buf][1];
 #pragma unroll
     for (int k4 = 0; k4 < BK; k4 += 4) {
       S ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        ar[i] = As_re[k4 + lk][wm0 + i * 16 + li];
-        ai[i] = As_im[k4 + lk][wm0 + i * 16 + li];
+        int off = (k4 + lk) * AST + wm0 + i * 16 + li;
+        ar[i] = Ar[off];
+        ai[i] = Ai[off];
         nai[i] = -ai[i];
       }
 #pragma unroll
       for (int j = 0; j < FN; ++j) {
-        br[j] = Bs_re[k4 + lk][wn0 + j * 16 + li];
-        bi[j] = Bs_im[k4 + lk][wn0 + j * 16 + li];
+        int off = (wn0 + j * 16 + li) * BST + k4 + lk;
+        br[j] = Br[off];
+        bi[j] = Bi[off];
       }
 #pragma unroll
       for (int i = 0; i < FM; ++i)
@@ -126,6 +174,7 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
           acc_im[i][j] = mfma16(ai[i], br[j], acc_im[i][j]);
         }
     }
+    if (t + 1 < nt) sstore(buf ^ 1);
     __syncthreads();
   }
 #pragma unroll
@@ -148,6 +197,15 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
     }
 }
 
+template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
+                        const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
+                        int batch) {
+  dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN>), g, dim3(256), 0, ctx->stream, M, N, K,
+                     A, lda, sA, B, ldb, sB, C, ldc, sC);
+}
+
 template <typename T>
 int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA, const T* B,
              int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
@@ -155,16 +213,16 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
   int slot = qb_prof_begin(ctx, sizeof(T) == sizeof(cf) ? QB_K_GEMM_C64 : QB_K_GEMM_C128,
                            8.0 * M * N * K * (double)batch,
                            (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
-  if (sizeof(T) == sizeof(cf)) {
-    constexpr int BM = 128, BN = 128, BK = 16;
-    dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-    hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK>), g, dim3(256), 0, ctx->stream, M, N, K, A,
-                       lda, sA, B, ldb, sB, C, ldc, sC);
+  if constexpr (sizeof(T) == sizeof(cf)) {
+    if (N <= 64)
+      launch_gemm<T, 256, 64, 16, 4, 1>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
+    else
+      launch_gemm<T, 128, 128, 16, 2, 2>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
   } else {
-    constexpr int BM = 64, BN = 64, BK = 16;
-    dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-    hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK>), g, dim3(256), 0, ctx->stream, M, N, K, A,
-                       lda, sA, B, ldb, sB, C, ldc, sC);
+    if (N <= 32)
+      launch_gemm<T, 128, 32, 16, 4, 1>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
+    else
+      launch_gemm<T, 128, 64, 16, 2, 2>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
   }
   qb_prof_end(ctx, slot);
   QB_HIP(ctx, hipGetLastError());
@@ -294,57 +352,106 @@ __global__ void k_laswp(T* __restrict__ A, int64_t lda, int64_t sA, int cb, int 
   }
 }
 
-// Unit-lower triangular solve X = L^{-1} B, n <= 32, one thread per column.
+// Triangular-solve leaf, n <= 32 rows: X = L^{-1} B (unit lower) or
+// X = U^{-1} B (upper).  One wave per 64 right-hand-side columns: the 32 x 64
+// tile of B is staged through LDS with coalesced loads/stores (each column's
+// 32 rows are contiguous), every lane then substitutes down its own column
+// with the triangle read from LDS.
 template <typename T, bool UNIT_LOWER>
-__global__ void __launch_bounds__(256) k_trsm_leaf(int n, int nrhs, const T* __restrict__ L,
-                                                   int64_t ldl, int64_t sL, T* __restrict__ B,
-                                                   int64_t ldb, int64_t sB) {
+__global__ void __launch_bounds__(64) k_trsm_leaf(int n, int nrhs, const T* __restrict__ L,
+                                                  int64_t ldl, int64_t sL, T* __restrict__ B,
+                                                  int64_t ldb, int64_t sB) {
   __shared__ T Ls[32][33];
+  __shared__ T Bt[64][33];
   const int64_t b = blockIdx.y;
   L += b * sL;
   B += b * sB;
-  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
-    int i = e % n, j = e / n;
-    Ls[i][j] = L[i + (int64_t)j * ldl];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * 64;
+  for (int e = tid; e < 32 * 32; e += 64) {
+    int i = e % 32, j = e / 32;
+    Ls[i][j] = (i < n && j < n) ? L[i + (int64_t)j * ldl] : T{0, 0};
+  }
+  // coalesced load: lanes 0..31 read rows of one column, 32..63 the next
+  for (int cc = tid >> 5; cc < 64; cc += 2) {
+    int i = tid & 31, col = c0 + cc;
+    Bt[cc][i] = (i < n && col < nrhs) ? B[i + (int64_t)col * ldb] : T{0, 0};
   }
   __syncthreads();
-  if (!UNIT_LOWER) {
-    // replace the diagonal by its reciprocal
-    if (threadIdx.x < n) {
-      T d = Ls[threadIdx.x][threadIdx.x];
-      auto den = d.re * d.re + d.im * d.im;
-      Ls[threadIdx.x][threadIdx.x] = T{d.re / den, -d.im / den};
-    }
-    __syncthreads();
+  if (!UNIT_LOWER && tid < 32) {
+    T d = tid < n ? Ls[tid][tid] : T{1, 0};
+    auto den = d.re * d.re + d.im * d.im;
+    Ls[tid][tid] = T{d.re / den, -d.im / den};
   }
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= nrhs) return;
-  T* x = B + (int64_t)col * ldb;
-  T v[32];
-  for (int i = 0; i < 32; ++i) v[i] = (i < n) ? x[i] : T{0, 0};
-  if (UNIT_LOWER) {
-    for (int i = 0; i < n; ++i) {
-      T s = v[i];
-      for (int j = 0; j < i; ++j) {
-        T l = Ls[i][j];
-        s.re -= l.re * v[j].re - l.im * v[j].im;
-        s.im -= l.re * v[j].im + l.im * v[j].re;
+  __syncthreads();
+  if (c0 + tid < nrhs) {
+    // blocks of 8 rows in registers; rows already final are read back from
+    // the lane's own LDS column (conflict-free: stride 33)
+    T* x = Bt[tid];
+    if (UNIT_LOWER) {
+#pragma unroll 1
+      for (int ib = 0; ib < 32; ib += 8) {
+        T v[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v[a] = x[ib + a];
+#pragma unroll 1
+        for (int j = 0; j < ib; ++j) {
+          T xj = x[j];
+#pragma unroll
+          for (int a = 0; a < 8; ++a) {
+            T l = Ls[ib + a][j];
+            v[a].re -= l.re * xj.re - l.im * xj.im;
+            v[a].im -= l.re * xj.im + l.im * xj.re;
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+          for (int c = 0; c < a; ++c) {
+            T l = Ls[ib + a][ib + c];
+            v[a].re -= l.re * v[c].re - l.im * v[c].im;
+            v[a].im -= l.re * v[c].im + l.im * v[c].re;
+          }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) x[ib + a] = v[a];
       }
-      v[i] = s;
-    }
-  } else {
-    for (int i = n - 1; i >= 0; --i) {
-      T s = v[i];
-      for (int j = i + 1; j < n; ++j) {
-        T u = Ls[i][j];
-        s.re -= u.re * v[j].re - u.im * v[j].im;
-        s.im -= u.re * v[j].im + u.im * v[j].re;
+    } else {
+#pragma unroll 1
+      for (int ib = 24; ib >= 0; ib -= 8) {
+        T v[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v[a] = x[ib + a];
+#pragma unroll 1
+        for (int j = ib + 8; j < 32; ++j) {
+          T xj = x[j];
+#pragma unroll
+          for (int a = 0; a < 8; ++a) {
+            T u = Ls[ib + a][j];
+            v[a].re -= u.re * xj.re - u.im * xj.im;
+            v[a].im -= u.re * xj.im + u.im * xj.re;
+          }
+        }
+#pragma unroll
+        for (int a = 7; a >= 0; --a) {
+#pragma unroll
+          for (int c = a + 1; c < 8; ++c) {
+            T u = Ls[ib + a][ib + c];
+            v[a].re -= u.re * v[c].re - u.im * v[c].im;
+            v[a].im -= u.re * v[c].im + u.im * v[c].re;
+          }
+          T d = Ls[ib + a][ib + a];
+          v[a] = T{v[a].re * d.re - v[a].im * d.im, v[a].re * d.im + v[a].im * d.re};
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) x[ib + a] = v[a];
       }
-      T d = Ls[i][i];
-      v[i] = T{s.re * d.re - s.im * d.im, s.re * d.im + s.im * d.re};
     }
   }
-  for (int i = 0; i < n; ++i) x[i] = v[i];
+  __syncthreads();
+  for (int cc = tid >> 5; cc < 64; cc += 2) {
+    int i = tid & 31, col = c0 + cc;
+    if (i < n && col < nrhs) B[i + (int64_t)col * ldb] = Bt[cc][i];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -372,9 +479,9 @@ static int trsm_lower(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int
                       int64_t ldb, int64_t sB, int batch) {
   if (n <= 0 || nrhs <= 0) return QB_OK;
   if (n <= 32) {
-    dim3 g((nrhs + 255) / 256, batch);
+    dim3 g((nrhs + 63) / 64, batch);
     int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
-    hipLaunchKernelGGL((k_trsm_leaf<T, true>), g, dim3(256), 0, ctx->stream, n, nrhs, L, ldl, sL,
+    hipLaunchKernelGGL((k_trsm_leaf<T, true>), g, dim3(64), 0, ctx->stream, n, nrhs, L, ldl, sL,
                        B, ldb, sB);
     qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
@@ -395,9 +502,9 @@ static int trsm_upper(qb_ctx* ctx, int n, int nrhs, const T* U, int64_t ldu, int
                       int64_t ldb, int64_t sB, int batch) {
   if (n <= 0 || nrhs <= 0) return QB_OK;
   if (n <= 32) {
-    dim3 g((nrhs + 255) / 256, batch);
+    dim3 g((nrhs + 63) / 64, batch);
     int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
-    hipLaunchKernelGGL((k_trsm_leaf<T, false>), g, dim3(256), 0, ctx->stream, n, nrhs, U, ldu,
+    hipLaunchKernelGGL((k_trsm_leaf<T, false>), g, dim3(64), 0, ctx->stream, n, nrhs, U, ldu,
                        sU, B, ldb, sB);
     qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
@@ -446,7 +553,7 @@ static int panel_rec(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int n, int c0, 
 template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
                      int* info, int batch) {
-  const int NB = 128;
+  const int NB = 256;
   for (int k0 = 0; k0 < n; k0 += NB) {
     int kb = n - k0 < NB ? n - k0 : NB;
     int rc = panel_rec(ctx, A, lda, sA, n, k0, kb, ipiv, sP, info, batch);

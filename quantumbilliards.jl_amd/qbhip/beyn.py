@@ -14,7 +14,8 @@ through NumPy), as the reference runs them through LinearAlgebra.
 
 Extensions (keyword arguments the reference does not have):
   precision="fp64"|"mixed": complex128 LU, or complex64 LU on MFMA with FP64
-      iterative refinement (refine_tol: relative correction at which it stops);
+      iterative refinement (refine_tol: normwise backward error at which a node
+      stops; nodes that stagnate are refactored in FP64);
   grow_r: keep doubling r while the rank saturates (the reference retries
       once with 2r, beyn.jl:291-309, because of the unconditional break at
       :307); with grow_r=False the single retry is reproduced;
@@ -102,6 +103,23 @@ def _svd_B(A0, A1, svd_tol):
     return U, S, Wh, rk
 
 
+def weyl_count(area: float, length: float, k1: float, k2: float) -> float:
+    """Dirichlet Weyl estimate N(k2) - N(k1), N(k) = A k^2/(4 pi) - L k/(4 pi)
+    (the estimate behind the reference's window planning, beyn.jl:141-160)."""
+    Nw = lambda k: area * k * k / (4 * np.pi) - length * k / (4 * np.pi)
+    return max(0.0, Nw(k2) - Nw(k1))
+
+
+def probe_count(billiard, k0: float, R: float, nq: int, svd_tol: float, r0: int = 64) -> int:
+    """Probes for a disk: the levels in [k0 - R - d, k0 + R + d] with the
+    leak margin d = R (svd_tol^{-1/nq} - 1) of the trapezoid filter, +10 %,
+    rounded up to a multiple of r0 (at least r0)."""
+    d = R * (svd_tol ** (-1.0 / nq) - 1.0)
+    c = weyl_count(billiard.area, billiard.length, k0 - R - d, k0 + R + d)
+    r = int(np.ceil((1.1 * c + 16) / r0)) * r0
+    return max(r0, r)
+
+
 def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 64, r: int = 48,
                        svd_tol: float = 1e-14, rng: Optional[np.random.Generator] = None,
                        use_chebyshev: bool = True, n_panels_h: int = 15000, M_h: int = 5,
@@ -125,7 +143,9 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
         Vb = V if V is not None else probe_block(rng, N, r)
         r = Vb.shape[1]
         A0, A1 = win.moments(Vb, refine_tol)
+        th = time.perf_counter()
         U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
+        t_svd = time.perf_counter() - th
         r_used = r
         if rk == 0:
             diag.update(rk=0, r=r_used, sigma=S, seconds=time.perf_counter() - t0)
@@ -136,16 +156,20 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
             while r_tmp < N:
                 Vb = probe_block(rng, N, r_tmp)
                 A0, A1 = win.moments(Vb, refine_tol)
+                th = time.perf_counter()
                 U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
+                t_svd += time.perf_counter() - th
                 r_used = r_tmp
                 if rk < r_tmp or not grow_r:
                     break
                 r_tmp += r_tmp
+        th = time.perf_counter()
         Uk = U[:, :rk]
         Wk = Wh.conj().T[:, :rk]
         tmp = (A1 @ Wk) / S[:rk][None, :]
         B = Uk.conj().T @ tmp
-        diag.update(rk=rk, r=r_used, sigma=S, moments=dict(win.stats),
+        t_svd += time.perf_counter() - th
+        diag.update(rk=rk, r=r_used, sigma=S, moments=dict(win.stats), t_host_svd_s=t_svd,
                     seconds=time.perf_counter() - t0)
         return (B, Uk, diag) if return_info else (B, Uk)
     finally:

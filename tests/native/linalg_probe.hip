@@ -71,3 +71,43 @@ int qbt_trsm_c64(qb_ctx* c, int n, const cf* LU, cf* B, int nrhs, int b) {
   return run_trsm<cf>(c, n, LU, B, nrhs, b);
 }
 }
+
+// Timing of gemm_sub at one shape: buffers filled on the device, `iters`
+// launches timed with events on the context stream; returns TFLOP/s.
+__global__ void k_fill(double* p, int64_t n, double s) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = s * (double)((i * 2654435761ull) % 1000) / 1000.0;
+}
+template <typename T>
+static double bench_gemm(qb_ctx* ctx, int M, int N, int K, int batch, int iters) {
+  T *dA, *dB, *dC;
+  size_t a = (size_t)M * K * batch, b = (size_t)K * N * batch, c = (size_t)M * N * batch;
+  if (hipMalloc(&dA, a * sizeof(T)) || hipMalloc(&dB, b * sizeof(T)) || hipMalloc(&dC, c * sizeof(T)))
+    return -1;
+  int64_t na = a * sizeof(T) / 8, nb = b * sizeof(T) / 8, nc = c * sizeof(T) / 8;
+  if (sizeof(T) == 16) {
+    hipLaunchKernelGGL(k_fill, dim3((na + 255) / 256), dim3(256), 0, ctx->stream, (double*)dA, na, 1e-3);
+    hipLaunchKernelGGL(k_fill, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, (double*)dB, nb, 1e-3);
+    hipLaunchKernelGGL(k_fill, dim3((nc + 255) / 256), dim3(256), 0, ctx->stream, (double*)dC, nc, 1e-3);
+  } else {
+    hipMemsetAsync(dA, 0, a * sizeof(T), ctx->stream);
+    hipMemsetAsync(dB, 0, b * sizeof(T), ctx->stream);
+    hipMemsetAsync(dC, 0, c * sizeof(T), ctx->stream);
+  }
+  gemm_sub<T>(ctx, M, N, K, dA, M, (int64_t)M * K, dB, K, (int64_t)K * N, dC, M, (int64_t)M * N, batch);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, ctx->stream);
+  for (int i = 0; i < iters; ++i)
+    gemm_sub<T>(ctx, M, N, K, dA, M, (int64_t)M * K, dB, K, (int64_t)K * N, dC, M, (int64_t)M * N, batch);
+  hipEventRecord(e1, ctx->stream);
+  hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipFree(dA); hipFree(dB); hipFree(dC);
+  return 8.0 * M * N * (double)K * batch * iters / (ms * 1e-3) / 1e12;
+}
+extern "C" double qbt_gemm_bench(qb_ctx* c, int M, int N, int K, int batch, int iters, int dbl) {
+  return dbl ? bench_gemm<cd>(c, M, N, K, batch, iters) : bench_gemm<cf>(c, M, N, K, batch, iters);
+}
