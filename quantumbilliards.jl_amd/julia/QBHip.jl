@@ -1,0 +1,176 @@
+# QBHip: binds QuantumBilliards.jl's Kress/Beyn hot path to libqbhip (HIP, gfx950)
+# through its C ABI (include/qb.h).  Include it after QuantumBilliards:
+#
+#     using QuantumBilliards
+#     include("QBHip.jl"); using .QBHip
+#     QBHip.enable!()          # route construct_boundary_matrices!/construct_B_matrix here
+#
+# Not executed in this repository (there is no Julia toolchain here); the C ABI
+# it calls is exercised by the Python tests.
+module QBHip
+
+using LinearAlgebra
+import QuantumBilliards
+const QB = QuantumBilliards
+
+const LIB = get(ENV, "QBHIP_LIB", joinpath(@__DIR__, "..", "libqbhip.so"))
+
+const QB_CFIE_KRESS = Int32(1)
+const QB_DLP_KRESS = Int32(2)
+const QB_FP64 = Int32(0)
+const QB_MIXED = Int32(1)
+
+struct ChebParams
+    n_panels_h::Int32
+    M_h::Int32
+    n_panels_j::Int32
+    M_j::Int32
+end
+
+struct Stats
+    N::Int64
+    nq::Int32
+    r::Int32
+    precision::Int32
+    refine_iters_max::Int32
+    nodes_fp64_fallback::Int32
+    refine_resid_max::Float64
+    t_assemble_ms::Float64
+    t_factor_ms::Float64
+    t_solve_ms::Float64
+    rmin::Float64
+    rmax::Float64
+end
+Stats() = Stats(0, 0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+
+function ctx()
+    if CTX[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        rc = ccall((:qb_ctx_create, LIB), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, h)
+        rc == 0 || error("qb_ctx_create failed ($rc)")
+        CTX[] = h[]
+    end
+    return CTX[]
+end
+
+# nonzero status -> the exception type the reference raises in the same case
+function check(rc::Integer)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:qb_last_error, LIB), Cstring, (Ptr{Cvoid},), ctx()))
+    rc == -1 ? throw(AssertionError(msg)) : error("libqbhip ($rc): $msg")
+end
+
+opcode(::Union{QB.CFIE_kress,QB.CFIE_kress_corners,QB.CFIE_kress_global_corners}) = QB_CFIE_KRESS
+opcode(::Union{QB.DLP_kress,QB.DLP_kress_global_corners}) = QB_DLP_KRESS
+
+# Upload a Vector{BoundaryPointsCFIE} (holes already reversed by evaluate_points).
+function geom(pts::Vector{QB.BoundaryPointsCFIE{Float64}})
+    n = Int64[length(p.xy) for p in pts]
+    cat(f) = reduce(vcat, [f(p) for p in pts])
+    x = cat(p -> getindex.(p.xy, 1));         y = cat(p -> getindex.(p.xy, 2))
+    dx = cat(p -> getindex.(p.tangent, 1));   dy = cat(p -> getindex.(p.tangent, 2))
+    ddx = cat(p -> getindex.(p.tangent_2, 1)); ddy = cat(p -> getindex.(p.tangent_2, 2))
+    ts = cat(p -> p.ts);                      ws = cat(p -> p.ws)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:qb_geom_create, LIB), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ref{Ptr{Cvoid}}),
+                ctx(), Int32(length(pts)), n, x, y, dx, dy, ddx, ddy, ts, ws, h))
+    return h[]
+end
+geom(p::QB.BoundaryPointsCFIE{Float64}) = geom([p])
+geom_destroy(g) = ccall((:qb_geom_destroy, LIB), Cvoid, (Ptr{Cvoid},), g)
+
+# construct_boundary_matrices! (cheb-CFIE:1431-1450, cheb-DLP:1110-1124)
+function construct_boundary_matrices!(Tbufs::Vector{Matrix{ComplexF64}}, solver, pts,
+                                      zj::AbstractVector{ComplexF64}; multithreaded::Bool=true,
+                                      use_chebyshev::Bool=true, n_panels_h::Int=15000,
+                                      M_h::Int=5, n_panels_j::Int=3000, M_j::Int=5,
+                                      timeit::Bool=false)
+    use_chebyshev || error("Direct complex-k construction is not implemented.")
+    Mk = length(zj)
+    @assert length(Tbufs) == Mk
+    g = geom(pts)
+    try
+        N = Int(ccall((:qb_geom_size, LIB), Int64, (Ptr{Cvoid},), g))
+        for T in Tbufs
+            @assert size(T) == (N, N)
+        end
+        out = Array{ComplexF64}(undef, N, N, Mk)      # one contiguous block
+        cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
+        check(ccall((:qb_assemble, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Int64, Ref{ChebParams},
+                     Ptr{ComplexF64}, Int32, Ptr{Float64}),
+                    ctx(), g, opcode(solver), collect(zj), Int64(Mk), cp, out, Int32(0), C_NULL))
+        for m in 1:Mk
+            copyto!(Tbufs[m], view(out, :, :, m))
+        end
+    finally
+        geom_destroy(g)
+    end
+    return nothing
+end
+
+# construct_B_matrix (beyn.jl:257-322): window on the device, SVD/eig on the host
+function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float64; nq::Int=64,
+                            r::Int=48, svd_tol=1e-14, rng=QB.MersenneTwister(0),
+                            use_chebyshev=true, n_panels_h::Int=15000, M_h::Int=5,
+                            n_panels_j::Int=3000, M_j::Int=5, info::Bool=false,
+                            multithreaded::Bool=true, precision::Symbol=:fp64,
+                            refine_tol::Float64=1e-12)
+    g = geom(pts)
+    w = Ref{Ptr{Cvoid}}(C_NULL)
+    st = Ref(Stats())
+    cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
+    prec = precision === :mixed ? QB_MIXED : QB_FP64
+    try
+        check(ccall((:qb_window_create, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64, Float64, Int32,
+                     Ref{ChebParams}, Int32, Ref{Ptr{Cvoid}}, Ref{Stats}),
+                    ctx(), g, opcode(solver), real(k0), imag(k0), R, Int32(nq), cp, prec, w, st))
+        function moments(V)
+            A0 = similar(V); A1 = similar(V)
+            check(ccall((:qb_window_moments, LIB), Cint,
+                        (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64},
+                         Float64, Ref{Stats}),
+                        w[], Int32(size(V, 2)), V, A0, A1, refine_tol, st))
+            return A0, A1
+        end
+        V = randn(rng, ComplexF64, N, r)
+        A0, A1 = moments(V)
+        U, Σ, W = svd!(copy(A0); full=false)
+        rk = count(>=(svd_tol), Σ)
+        rk == 0 && return Matrix{ComplexF64}(undef, N, 0), Matrix{ComplexF64}(undef, N, 0)
+        if rk == r && 2r < N                       # the single retry of beyn.jl:291-309
+            V = randn(rng, ComplexF64, N, 2r)
+            A0, A1 = moments(V)
+            U, Σ, W = svd!(copy(A0); full=false)
+            rk = count(>=(svd_tol), Σ)
+        end
+        Uk = U[:, 1:rk]; Wk = W[:, 1:rk]
+        B = adjoint(Uk) * ((A1 * Wk) ./ transpose(Σ[1:rk]))
+        return B, Uk
+    finally
+        w[] != C_NULL && ccall((:qb_window_destroy, LIB), Cvoid, (Ptr{Cvoid},), w[])
+        geom_destroy(g)
+    end
+end
+
+# Route the reference's entry points here for the Kress solvers.
+function enable!()
+    @eval QB function construct_boundary_matrices!(Tbufs::Vector{Matrix{ComplexF64}},
+                                                   solver::Union{CFIE_kress,DLP_kress}, pts,
+                                                   zj::AbstractVector{ComplexF64}; kw...)
+        return $(QBHip).construct_boundary_matrices!(Tbufs, solver, pts, zj; kw...)
+    end
+    @eval QB function construct_B_matrix(solver::Union{CFIE_kress,DLP_kress}, pts, N::Int,
+                                         k0::Complex{Float64}, R::Float64; kw...)
+        return $(QBHip).construct_B_matrix(solver, pts, N, k0, R; kw...)
+    end
+    return nothing
+end
+
+end # module
