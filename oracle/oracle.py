@@ -388,6 +388,82 @@ def cfie_assemble_cols(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int,
     return A
 
 
+def dlp_assemble_cols(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int, rmin: float,
+                      rmax: float):
+    """Columns [j0, j1) of F(z) = I - D(z) for DLP-Kress at one z
+    (cheb-DLP :702-769, :1086-1095), without N x N caches."""
+    n = len(comp.x)
+    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+    H1 = build_table("h", 1, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+    J1 = build_table("j", 1, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+    js = np.arange(j0, j1)
+    DX = comp.x[:, None] - comp.x[None, js]
+    DY = comp.y[:, None] - comp.y[None, js]
+    r = np.hypot(DX, DY)
+    diag = np.arange(n)[:, None] == js[None, :]
+    r[diag] = 1.0
+    inner = comp.dy[None, js] * DX - comp.dx[None, js] * DY
+    d = (np.arange(n)[:, None] - js[None, :]) % n
+    Rk = kress_R_col(n)[d]
+    with np.errstate(divide="ignore"):
+        lt = np.log(4.0 * np.sin((comp.ts[:, None] - comp.ts[None, js]) / 2.0) ** 2)
+    lt[diag] = 0.0
+    pidx, th = ph.locate(r)
+    pjx, tj = pj.locate(r)
+    h1, j1v = clenshaw(H1[pidx], th), clenshaw(J1[pjx], tj)
+    cD = inner / r
+    wj = comp.ws[js][None, :]
+    l1, ht = (-z * INV_TWO_PI) * j1v, (0.5j * z) * h1
+    D = (Rk * cD) * l1 + (wj * cD) * ht - (wj * lt * cD) * l1
+    ii = js
+    kap = INV_TWO_PI * (-(comp.dx[ii] * comp.ddy[ii] - comp.dy[ii] * comp.ddx[ii])
+                        / (comp.dx[ii] ** 2 + comp.dy[ii] ** 2))
+    D[ii, ii - j0] = comp.ws[ii] * kap
+    F = -D
+    F[ii, ii - j0] += 1.0
+    return F
+
+
+def _liboracle():
+    import ctypes
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "liboracle.so")
+    L = ctypes.CDLL(path)
+    L.qbo_cfie_cols.restype = None
+    return L
+
+
+def cfie_cols_c(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int, rmin: float,
+                rmax: float, tables=None):
+    """Same as cfie_assemble_cols, through the C/OpenMP restatement in
+    oracle/qb_oracle.c (tables built here with AMOS).  Returns (N, j1-j0)."""
+    import ctypes as C
+    n = len(comp.x)
+    if tables is None:
+        H0 = build_table("h", 0, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+        H1 = build_table("h", 1, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+        J0 = build_table("j", 0, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+        J1 = build_table("j", 1, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+        tables = (np.ascontiguousarray(np.stack([H0, H1], axis=-1)),
+                  np.ascontiguousarray(np.stack([J0, J1], axis=-1)))
+    H, J = tables
+    out = np.empty((j1 - j0, n), np.complex128)
+    rk = np.ascontiguousarray(kress_R_col(n))
+    lt = np.zeros(n)
+    d = np.arange(1, n)
+    lt[1:] = np.log(4.0 * np.sin(np.pi * d / n) ** 2)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in
+            (comp.x, comp.y, comp.dx, comp.dy, comp.ddx, comp.ddy, comp.ws)]
+    P = lambda a: C.c_void_p(a.ctypes.data)
+    _liboracle().qbo_cfie_cols(C.c_int64(n), *[P(a) for a in arrs], P(rk), P(lt),
+                               C.c_int64(j0), C.c_int64(j1), C.c_double(z.real),
+                               C.c_double(z.imag), P(H), C.c_int(cp.n_panels_h), C.c_int(cp.M_h),
+                               C.c_double(rmin), C.c_double(rmax), P(J),
+                               C.c_int(cp.n_panels_j), C.c_int(cp.M_j), P(out))
+    return out.T
+
+
 def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None, cols=None):
     """construct_boundary_matrices! for CFIE_kress (cheb-CFIE:1431-1450 ->
     _all_k_nosymm_CFIE_chebyshev! :543-665).  Returns [nz, N, N] with

@@ -6,12 +6,10 @@
 // (chebyshev_euclidian_helmholtz_dlp_kress.jl:702-769, :1086-1095).
 //
 // Layout: out[q*N*N + i + j*N] (column-major per wavenumber, Julia's layout).
-// A 64x4 thread block covers 64 consecutive rows (one wave per column, so
-// every store of a wave is one contiguous 1 KiB run) and 16 columns.  The
-// geometry of a pair (distance, 1/r, the DLP numerator, the circulant Kress
-// R and log term) is computed in registers instead of being read from N x N
-// caches as the reference does; grid.z walks the wavenumbers so that the
-// blocks resident at one time read the tables of one wavenumber from L2.
+// The geometry of a pair (distance, 1/r, the DLP numerator, the circulant
+// Kress R and log term) is computed in registers instead of being read from
+// N x N caches as the reference does.  As in the reference, the specials of a
+// distance are evaluated once for the pair (i, j), (j, i).
 #include "qb_bessel.h"
 #include "qb_cheb.h"
 #include "qb_internal.h"
@@ -58,97 +56,105 @@ __device__ __forceinline__ void bessj(const TabView& tv, double r, cd* j0, cd* j
     qb_clenshaw2_dyn(c, tv.M_j, t, j0, j1);
 }
 
-// CFIE entry A(z)[i,j] (target i, source j).
-template <int MH, int MJ>
-__device__ cd cfie_entry(const DevGeom& g, const TabView& tv, int64_t i, int64_t j, cd k) {
+// Special functions of one distance r at one wavenumber: H0, H1 (Hankel
+// table, or the direct evaluation below |kr| = 0.2 for CFIE), J0, J1 (J table,
+// only needed inside one component).
+struct Spec {
+  cd h0, h1, j0, j1;
+};
+
+template <int OP, int MH, int MJ>
+__device__ __forceinline__ Spec specials(const TabView& tv, double r, cd k, bool same) {
+  Spec s;
+  hank<MH, MJ>(tv, r, k, OP == QB_CFIE_KRESS, &s.h0, &s.h1);
+  if (same) {
+    bessj<MH, MJ>(tv, r, &s.j0, &s.j1);
+  } else {
+    s.j0 = mk(0, 0);
+    s.j1 = mk(0, 0);
+  }
+  return s;
+}
+
+// Off-diagonal entry A(z)[i,j] (target i, source j) from the shared
+// specials of r = |x_i - x_j| (cheb-CFIE :594-612 and :636-646;
+// cheb-DLP :737-765 with F = I - D).
+template <int OP>
+__device__ __forceinline__ cd offdiag_entry(const DevGeom& g, int64_t i, int64_t j, double dx,
+                                            double dy, double invr, bool same, double Rij,
+                                            double lt, cd k, const Spec& sp) {
   const cd ik = mk(-k.im, k.re);
-  int ci = g.comp[i], cj = g.comp[j];
-  double sj = g.speed[j], wj = g.ws[j];
-  if (i == j) {
-    // diagonal limits (cheb-CFIE :585-591)
-    double kap = g.kappa[j];
-    double rjj = g.rk[g.off[cj]];
-    cd dval = mk(wj * kap, 0.0);
-    double m1 = -QB_INV_TWO_PI * sj;
-    cd k2 = k * k;
-    cd lg = clog(mk(0.25 * k2.re * (sj * sj), 0.25 * k2.im * (sj * sj)));
-    cd m2 = mk(-QB_EULER_OVER_PI - QB_INV_TWO_PI * lg.re, 0.5 - QB_INV_TWO_PI * lg.im) * sj;
-    cd sval = mk(rjj * m1, 0.0) + wj * m2;
-    cd v = dval + ik * sval;
-    return mk(1.0 - v.re, -v.im);
-  }
-  double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j];
-  double r = hypot(dx, dy);
-  double invr = 1.0 / r;
-  double inn = g.dy[j] * dx - g.dx[j] * dy;
-  cd h0, h1;
-  hank<MH, MJ>(tv, r, k, true, &h0, &h1);
-  const cd aL2 = mk(-0.5 * k.im, 0.5 * k.re);  // 0.5 i k
+  const cd aL2 = mk(-0.5 * k.im, 0.5 * k.re);  // i k / 2
   const cd aM2 = mk(0.0, 0.5);
-  if (ci != cj) {
-    // smooth off-component block (cheb-CFIE :632-648)
-    double cD = wj * inn * invr;
-    double cS = wj * sj;
-    cd dval = cD * (aL2 * h1);
-    cd sval = cS * (aM2 * h0);
-    cd v = dval + ik * sval;
-    return -v;
-  }
-  cd j0, j1;
-  bessj<MH, MJ>(tv, r, &j0, &j1);
-  int64_t o = g.off[ci];
-  int64_t nc = g.off[ci + 1] - o;
-  int64_t d = (i - j) % nc;
-  if (d < 0) d += nc;
-  double Rij = g.rk[o + d], lt = g.lt[o + d];
   const cd aL1 = mk(-k.re * QB_INV_TWO_PI, -k.im * QB_INV_TWO_PI);
   const double aM1 = -QB_INV_TWO_PI;
-  cd L1 = aL1 * j1, L2 = aL2 * h1;
-  cd M1 = aM1 * j0, M2 = aM2 * h0;
+  double wj = g.ws[j];
+  double inn = g.dy[j] * dx - g.dx[j] * dy;  // t_j x (x_i - x_j)
+  if (OP == QB_DLP_KRESS) {
+    double cD = inn * invr;
+    double c1 = Rij * cD, c2 = wj * cD, c3 = wj * lt * cD;
+    cd l1 = aL1 * sp.j1, ht = aL2 * sp.h1;
+    return -(c1 * l1 + c2 * ht - c3 * l1);
+  }
+  double sj = g.speed[j];
+  if (!same) {
+    double cD = wj * inn * invr;
+    double cS = wj * sj;
+    cd v = cD * (aL2 * sp.h1) + ik * (cS * (aM2 * sp.h0));
+    return -v;
+  }
+  cd L1 = aL1 * sp.j1, L2 = aL2 * sp.h1;
+  cd M1 = aM1 * sp.j0, M2 = aM2 * sp.h0;
   double cD1 = Rij * inn * invr, cD2 = wj * inn * invr, cD3 = wj * lt * inn * invr;
   double cS1 = Rij * sj, cS2 = wj * sj, cS3 = wj * sj * lt;
   cd dval = cD1 * L1 + cD2 * L2 - cD3 * L1;
   cd sval = cS1 * M1 + cS2 * M2 - cS3 * M1;
-  cd v = dval + ik * sval;
-  return -v;
+  return -(dval + ik * sval);
 }
 
-// DLP entry F(z)[i,j] = delta_ij - D_ij (single component).
-template <int MH, int MJ>
-__device__ cd dlp_entry(const DevGeom& g, const TabView& tv, int64_t i, int64_t j, cd k) {
+// Diagonal entry (cheb-CFIE :585-591; cheb-DLP :729-733 with F = I - D).
+template <int OP>
+__device__ __forceinline__ cd diag_entry(const DevGeom& g, int64_t j, cd k) {
   double wj = g.ws[j];
-  if (i == j) return mk(1.0 - wj * g.kappa[j], 0.0);
-  double dx = g.x[i] - g.x[j], dy = g.y[i] - g.y[j];
-  double r = hypot(dx, dy);
-  double invr = 1.0 / r;
-  double inn = g.dy[j] * dx - g.dx[j] * dy;
-  cd h0, h1, j0, j1;
-  // eval_h_multi_ks! (bessels:1051-1076) has no |kr| < 0.2 branch
-  hank<MH, MJ>(tv, r, k, false, &h0, &h1);
-  bessj<MH, MJ>(tv, r, &j0, &j1);
-  int64_t nc = g.N;
-  int64_t d = (i - j) % nc;
-  if (d < 0) d += nc;
-  double Rij = g.rk[d], lt = g.lt[d];
-  const cd aL1 = mk(-k.re * QB_INV_TWO_PI, -k.im * QB_INV_TWO_PI);
-  const cd aL2 = mk(-0.5 * k.im, 0.5 * k.re);
-  double cD = inn * invr;
-  double c1 = Rij * cD, c2 = wj * cD, c3 = wj * lt * cD;
-  cd l1 = aL1 * j1, ht = aL2 * h1;
-  cd D = c1 * l1 + c2 * ht - c3 * l1;
-  return -D;
+  if (OP == QB_DLP_KRESS) return mk(1.0 - wj * g.kappa[j], 0.0);
+  const cd ik = mk(-k.im, k.re);
+  double sj = g.speed[j];
+  double rjj = g.rk[g.off[g.comp[j]]];
+  cd dval = mk(wj * g.kappa[j], 0.0);
+  double m1 = -QB_INV_TWO_PI * sj;
+  cd k2 = k * k;
+  cd lg = clog(mk(0.25 * k2.re * (sj * sj), 0.25 * k2.im * (sj * sj)));
+  cd m2 = mk(-QB_EULER_OVER_PI - QB_INV_TWO_PI * lg.re, 0.5 - QB_INV_TWO_PI * lg.im) * sj;
+  cd sval = mk(rjj * m1, 0.0) + wj * m2;
+  cd v = dval + ik * sval;
+  return mk(1.0 - v.re, -v.im);
 }
 
+// Tile-pair assembly.  Block b of grid.x is the pair of 64 x 64 tiles (I, J)
+// and (J, I), I >= J, enumerated diagonal by diagonal (d = I - J), so the
+// workgroups resident together work on similar distances and read nearby
+// table panels.  Each thread computes the specials of 16 distances once and
+// produces both A[i,j] (stored directly: a wave writes 64 consecutive rows)
+// and A[j,i] (staged in LDS, then written column by column of the mirror
+// tile).  grid.z walks the wavenumbers.
 template <int OP, int MH, int MJ>
 __global__ void __launch_bounds__(256) k_assemble(DevGeom g, const cd* __restrict__ zk,
                                                   const cd* __restrict__ H,
                                                   const cd* __restrict__ J, int np_h, int M_h,
                                                   int np_j, int M_j, double rmin_h, double rmax,
-                                                  int64_t q0, cd* __restrict__ out64,
+                                                  int64_t q0, int T, cd* __restrict__ out64,
                                                   cf* __restrict__ out32) {
+  __shared__ cd mir[64][65];  // mir[i_local][j_local] = A[j, i]
   const int64_t N = g.N;
-  const int64_t i = (int64_t)blockIdx.x * 64 + threadIdx.x;
   const int64_t q = q0 + blockIdx.z;
+  // pair index -> (d, J): diagonal d has T - d pairs
+  int64_t p = blockIdx.x;
+  int d = 0;
+  while (p >= T - d) {
+    p -= T - d;
+    ++d;
+  }
+  const int Jt = (int)p, It = Jt + d;
   TabView tv;
   tv.H = H + q * (int64_t)np_h * (2 * (M_h + 1));
   tv.J = J + q * (int64_t)np_j * (2 * (M_j + 1));
@@ -159,15 +165,61 @@ __global__ void __launch_bounds__(256) k_assemble(DevGeom g, const cd* __restric
   tv.rmin_h = rmin_h;
   tv.rmax = rmax;
   const cd k = zk[q];
-  if (i >= N) return;
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t i = (int64_t)It * 64 + il;
   const int64_t base = (q - q0) * N * N;
+  const bool diag_tile = (It == Jt);
+  const bool iok = i < N;
+  double xi = 0, yi = 0;
+  int ci = 0;
+  if (iok) {
+    xi = g.x[i];
+    yi = g.y[i];
+    ci = g.comp[i];
+  }
 #pragma unroll 1
-  for (int c = 0; c < 4; ++c) {
-    int64_t j = (int64_t)blockIdx.y * 16 + threadIdx.y + 4 * c;
-    if (j >= N) break;
-    cd v = (OP == QB_CFIE_KRESS) ? cfie_entry<MH, MJ>(g, tv, i, j, k)
-                                 : dlp_entry<MH, MJ>(g, tv, i, j, k);
+  for (int c = 0; c < 16; ++c) {
+    const int jl = jl0 + 4 * c;
+    const int64_t j = (int64_t)Jt * 64 + jl;
+    if (!iok || j >= N) continue;
+    cd vij, vji = mk(0, 0);
+    if (i == j) {
+      vij = diag_entry<OP>(g, j, k);
+    } else {
+      double dx = xi - g.x[j], dy = yi - g.y[j];
+      double r = hypot(dx, dy);
+      double invr = 1.0 / r;
+      int cj = g.comp[j];
+      bool same = (ci == cj);
+      double Rij = 0, lt = 0;
+      if (same) {
+        int64_t o = g.off[ci];
+        int64_t nc = g.off[ci + 1] - o;
+        int64_t dd = (i - j) % nc;
+        if (dd < 0) dd += nc;
+        Rij = g.rk[o + dd];
+        lt = g.lt[o + dd];  // both circulant columns are even in (i - j)
+      }
+      Spec sp = specials<OP, MH, MJ>(tv, r, k, same || OP == QB_DLP_KRESS);
+      vij = offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, k, sp);
+      if (!diag_tile) vji = offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, k, sp);
+    }
     int64_t idx = base + i + j * N;
+    if (out64) out64[idx] = vij;
+    if (out32) out32[idx] = to_cf(vij);
+    if (!diag_tile) mir[il][jl] = vji;
+  }
+  if (diag_tile) return;
+  __syncthreads();
+  // mirror tile: column i (64 of them), rows j = Jt*64 + 0..63 contiguous
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int il2 = jl0 + 4 * c;             // which column of the mirror tile
+    const int jl2 = threadIdx.x & 63;        // row inside it
+    const int64_t ii = (int64_t)It * 64 + il2, jj = (int64_t)Jt * 64 + jl2;
+    if (ii >= N || jj >= N) continue;
+    cd v = mir[il2][jl2];
+    int64_t idx = base + jj + ii * N;
     if (out64) out64[idx] = v;
     if (out32) out32[idx] = to_cf(v);
   }
@@ -177,12 +229,14 @@ template <int OP, int MH, int MJ>
 static void launch_asm(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, int64_t nzc,
                        const DevTables& t, cd* o64, cf* o32) {
   int64_t N = g->d.N;
-  dim3 grid((unsigned)((N + 63) / 64), (unsigned)((N + 15) / 16), (unsigned)nzc);
+  int T = (int)((N + 63) / 64);
+  int64_t pairs = (int64_t)T * (T + 1) / 2;
+  dim3 grid((unsigned)pairs, 1, (unsigned)nzc);
   double ent = (double)N * N * nzc;
   int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
                            ent * ((o64 ? sizeof(cd) : 0) + (o32 ? sizeof(cf) : 0)));
-  hipLaunchKernelGGL((k_assemble<OP, MH, MJ>), grid, dim3(64, 4), 0, ctx->stream, g->d, z, t.H,
-                     t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0, o64, o32);
+  hipLaunchKernelGGL((k_assemble<OP, MH, MJ>), grid, dim3(256), 0, ctx->stream, g->d, z, t.H,
+                     t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0, T, o64, o32);
   qb_prof_end(ctx, slot);
 }
 
@@ -193,23 +247,26 @@ int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int6
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   int64_t N = g->d.N;
-  if ((N + 15) / 16 > 65535)
-    return qb_fail(ctx, QB_E_ARG, "N too large for the assembly grid");
+  // grid.x = number of 64 x 64 tile pairs (2^31 - 1 limit: N < 2.9e6)
   // grid.z is limited to 65535; chunk the wavenumbers
   for (int64_t q0 = 0; q0 < nz; q0 += 1024) {
     int64_t nzc = nz - q0 < 1024 ? nz - q0 : 1024;
     cd* o64 = out64 ? out64 + q0 * N * N : nullptr;
     cf* o32 = out32 ? out32 + q0 * N * N : nullptr;
     // the kernel indexes its output from the chunk start; pass q0 for tables
-    bool fast = (t.M_h == 5 && t.M_j == 5);
+    const int mh = t.M_h, mj = t.M_j;
     if (op == QB_CFIE_KRESS) {
-      if (fast)
+      if (mh == 5 && mj == 5)
         launch_asm<QB_CFIE_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+      else if (mh == 7 && mj == 7)
+        launch_asm<QB_CFIE_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, o64, o32);
       else
         launch_asm<QB_CFIE_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, o64, o32);
     } else {
-      if (fast)
+      if (mh == 5 && mj == 5)
         launch_asm<QB_DLP_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+      else if (mh == 7 && mj == 7)
+        launch_asm<QB_DLP_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, o64, o32);
       else
         launch_asm<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, o64, o32);
     }

@@ -38,7 +38,9 @@ struct QbProfRec {
 
 struct qb_ctx {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // stream every launch of this context uses
+  hipStream_t stream_aux = nullptr;  // second stream for overlapped work (LU lookahead)
+  hipEvent_t ev_main = nullptr, ev_aux = nullptr;  // cross-stream ordering
   std::string err;
   int64_t bytes = 0;  // device bytes held by objects of this context
   bool prof = false;

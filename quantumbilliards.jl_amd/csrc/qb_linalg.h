@@ -23,9 +23,12 @@ int qb_gather_rows_cd2cf(qb_ctx* ctx, const cd* src, int64_t s_src, const int* p
                          int n, int nrhs, int batch);
 int qb_add_cf_to_cd(qb_ctx* ctx, const cf* d, cd* x, int64_t n);
 int qb_cd_to_cf(qb_ctx* ctx, const cd* x, cf* y, int64_t n);
-int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int nb);
+// R[b*ns + e] = V[e] for e < nv, b < nb
+int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int64_t ns, int nb);
 int qb_moments(qb_ctx* ctx, const cd* X, int64_t nv, int nq, const cd* w, const cd* wz, cd* A0,
                cd* A1);
-int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int nb, double* out_dev);
+// out[b] = max_{e < n_per} |X[b*stride + e]|
+int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int64_t stride, int nb,
+                   double* out_dev);
 int qb_node_maxabs32(qb_ctx* ctx, const cf* X, int64_t n_per, int nb, double* out_dev);
 int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev);

@@ -553,24 +553,50 @@ static int panel_rec(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int n, int c0, 
 template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
                      int* info, int batch) {
+  // Lookahead: after the block row of step s is solved, the columns of panel
+  // s+1 are updated first; panel s+1 is then factored on the auxiliary
+  // stream while the rest of the trailing update runs on the main stream.
+  // The two touch disjoint columns; the main stream waits for the panel
+  // before the row swaps of step s+1.
   const int NB = 256;
+  hipStream_t main = ctx->stream, aux = ctx->stream_aux;
+  auto on_aux = [&](auto&& f) -> int {
+    ctx->stream = aux;
+    int rc = f();
+    ctx->stream = main;
+    return rc;
+  };
+  int rc = panel_rec(ctx, A, lda, sA, n, 0, n < NB ? n : NB, ipiv, sP, info, batch);
+  if (rc) return rc;
   for (int k0 = 0; k0 < n; k0 += NB) {
     int kb = n - k0 < NB ? n - k0 : NB;
-    int rc = panel_rec(ctx, A, lda, sA, n, k0, kb, ipiv, sP, info, batch);
-    if (rc) return rc;
     rc = laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch);
     if (rc) return rc;
     rc = laswp(ctx, A, lda, sA, k0 + kb, n, k0, k0 + kb, ipiv, sP, batch);
     if (rc) return rc;
-    if (k0 + kb < n) {
-      T* A11 = A + k0 + (int64_t)k0 * lda;
-      T* A12 = A + k0 + (int64_t)(k0 + kb) * lda;
-      rc = trsm_lower(ctx, kb, n - k0 - kb, A11, lda, sA, A12, lda, sA, batch);
-      if (rc) return rc;
-      rc = gemm_sub<T>(ctx, n - k0 - kb, n - k0 - kb, kb, A + k0 + kb + (int64_t)k0 * lda, lda,
-                       sA, A12, lda, sA, A + k0 + kb + (int64_t)(k0 + kb) * lda, lda, sA, batch);
+    if (k0 + kb >= n) break;
+    const int k1 = k0 + kb, m = n - k1;
+    const int kb1 = m < NB ? m : NB;
+    T* A11 = A + k0 + (int64_t)k0 * lda;
+    T* A12 = A + k0 + (int64_t)k1 * lda;
+    T* A21 = A + k1 + (int64_t)k0 * lda;
+    rc = trsm_lower(ctx, kb, m, A11, lda, sA, A12, lda, sA, batch);
+    if (rc) return rc;
+    // next panel's columns first
+    rc = gemm_sub<T>(ctx, m, kb1, kb, A21, lda, sA, A12, lda, sA, A + k1 + (int64_t)k1 * lda, lda,
+                     sA, batch);
+    if (rc) return rc;
+    QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
+    QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
+    rc = on_aux([&] { return panel_rec(ctx, A, lda, sA, n, k1, kb1, ipiv, sP, info, batch); });
+    if (rc) return rc;
+    QB_HIP(ctx, hipEventRecord(ctx->ev_aux, aux));
+    if (m > kb1) {
+      rc = gemm_sub<T>(ctx, m, m - kb1, kb, A21, lda, sA, A12 + (int64_t)kb1 * lda, lda, sA,
+                       A + k1 + (int64_t)(k1 + kb1) * lda, lda, sA, batch);
       if (rc) return rc;
     }
+    QB_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_aux, 0));
   }
   return QB_OK;
 }
@@ -663,11 +689,15 @@ __global__ void k_add_cf_to_cd(const cf* __restrict__ d, cd* __restrict__ x, int
     x[i] = v;
   }
 }
-// Copy V (one N x r block) into every node's slot of R.
-__global__ void k_bcast(const cd* __restrict__ V, cd* __restrict__ R, int64_t nv, int nb) {
+// Copy the first nv entries of V into every node's slot of R (slot stride ns).
+__global__ void k_bcast(const cd* __restrict__ V, cd* __restrict__ R, int64_t nv, int64_t ns,
+                        int nb) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < nv * nb; i += st) R[i] = V[i % nv];
+  for (; i < nv * nb; i += st) {
+    int64_t b = i / nv, e = i - b * nv;
+    R[b * ns + e] = V[e];
+  }
 }
 // A0 = sum_j w_j X_j, A1 = sum_j w_j z_j X_j over nq nodes (beyn.jl:282-283).
 __global__ void k_moments(const cd* __restrict__ X, int64_t nv, int nq, const cd* __restrict__ w,
@@ -684,9 +714,10 @@ __global__ void k_moments(const cd* __restrict__ X, int64_t nv, int nq, const cd
   A1[i] = a1;
 }
 // Per-node max |.|_inf of n_per entries (|re|+|im| bound not used: true modulus).
-__global__ void k_node_maxabs(const cd* __restrict__ X, int64_t n_per, double* __restrict__ out) {
+__global__ void k_node_maxabs(const cd* __restrict__ X, int64_t n_per, int64_t stride,
+                              double* __restrict__ out) {
   const int64_t b = blockIdx.y;
-  X += b * n_per;
+  X += b * stride;
   double m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_per;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -736,8 +767,8 @@ int qb_add_cf_to_cd(qb_ctx* ctx, const cf* d, cd* x, int64_t n) {
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
-int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int nb) {
-  hipLaunchKernelGGL(k_bcast, dim3(grid1(nv * nb)), dim3(256), 0, ctx->stream, V, R, nv, nb);
+int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int64_t ns, int nb) {
+  hipLaunchKernelGGL(k_bcast, dim3(grid1(nv * nb)), dim3(256), 0, ctx->stream, V, R, nv, ns, nb);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
@@ -748,11 +779,12 @@ int qb_moments(qb_ctx* ctx, const cd* X, int64_t nv, int nq, const cd* w, const 
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
-int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int nb, double* out_dev) {
+int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int64_t stride, int nb,
+                   double* out_dev) {
   QB_HIP(ctx, hipMemsetAsync(out_dev, 0, nb * sizeof(double), ctx->stream));
   int64_t g = (n_per + 255) / 256;
   dim3 grid((unsigned)(g > 1024 ? 1024 : g), nb);
-  hipLaunchKernelGGL(k_node_maxabs, grid, dim3(256), 0, ctx->stream, X, n_per, out_dev);
+  hipLaunchKernelGGL(k_node_maxabs, grid, dim3(256), 0, ctx->stream, X, n_per, stride, out_dev);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }

@@ -12,5 +12,5 @@ from .geometry import (Billiard, BoundaryPointsCFIE, CircularHoleBilliard, evalu
                        make_star_polar_boundary, n_kress)
 from .solvers import (CFIE_kress, DLP_kress, Geometry, assemble, bessel_eval,
                       boundary_matrix_size, chebyshev_params, construct_boundary_matrices)
-from .beyn import (Window, construct_B_matrix, footprint, probe_block, probe_count, solve,
+from .beyn import (B_from_moments, Window, construct_B_matrix, footprint, probe_block, probe_count, solve,
                    solve_vect, weyl_count)

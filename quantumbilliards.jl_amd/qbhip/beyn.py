@@ -120,6 +120,20 @@ def probe_count(billiard, k0: float, R: float, nq: int, svd_tol: float, r0: int 
     return max(r0, r)
 
 
+def B_from_moments(A0: np.ndarray, A1: np.ndarray, svd_tol: float):
+    """Host half of construct_B_matrix (beyn.jl:286-321): thin SVD of A0,
+    absolute rank threshold, B = U_k^H A1 W_k Sigma_k^{-1}.  Returns
+    (B, Uk, sigma, rk)."""
+    U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
+    if rk == 0:
+        e = np.zeros((A0.shape[0], 0), dtype=np.complex128)
+        return np.zeros((0, 0), np.complex128), e, S, 0
+    Uk = U[:, :rk]
+    Wk = Wh.conj().T[:, :rk]
+    B = Uk.conj().T @ ((A1 @ Wk) / S[:rk][None, :])
+    return B, Uk, S, rk
+
+
 def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 64, r: int = 48,
                        svd_tol: float = 1e-14, rng: Optional[np.random.Generator] = None,
                        use_chebyshev: bool = True, n_panels_h: int = 15000, M_h: int = 5,

@@ -112,11 +112,12 @@ def CircularHoleBilliard(R: float = 1.0, holes: Sequence = ((0.3, 0.35, 0.0),)) 
     return Billiard(comps, area, L, "circle_hole")
 
 
-def make_full_stadium(radius: float = 1.0, half_length: float = 1.0) -> Billiard:
-    """Full stadium (src/billiards/stadium.jl:23-41): two straight segments of
-    length 2*half_length joined by two half circles, one C1 closed component
-    traversed counter-clockwise starting at the right end of the bottom edge."""
-    a, r = half_length, radius
+def make_full_stadium(half_width: float = 1.0, radius: float = 1.0) -> Billiard:
+    """Full stadium (src/billiards/stadium.jl:23-41): one C1 closed component
+    of five segments starting at (half_width + radius, 0), counter-clockwise:
+    quarter arc, top edge (right to left), half arc, bottom edge (left to
+    right), quarter arc."""
+    a, r = half_width, radius
 
     def line(p0, p1):
         p0, p1 = np.asarray(p0, float), np.asarray(p1, float)
@@ -127,17 +128,19 @@ def make_full_stadium(radius: float = 1.0, half_length: float = 1.0) -> Billiard
             return (p0[0] + d[0] * u, p0[1] + d[1] * u, d[0] + z, d[1] + z, z, z)
         return Segment(g, float(np.hypot(*d)))
 
-    def arc(cx, cy, th0):
+    def arc(cx, cy, arc_angle, shift):
+        # CircleSegment(radius, arc_angle, shift_angle, x0, y0) (circlesegment.jl:16-18)
         def g(u):
             u = np.asarray(u, float)
-            th = th0 + math.pi * u
+            th = arc_angle * u + shift
             c, s = np.cos(th), np.sin(th)
-            return (cx + r * c, cy + r * s, -math.pi * r * s, math.pi * r * c,
-                    -math.pi ** 2 * r * c, -math.pi ** 2 * r * s)
-        return Segment(g, math.pi * r)
+            return (cx + r * c, cy + r * s, -arc_angle * r * s, arc_angle * r * c,
+                    -arc_angle ** 2 * r * c, -arc_angle ** 2 * r * s)
+        return Segment(g, arc_angle * r)
 
-    segs = [arc(a, 0.0, -math.pi / 2), line((a, r), (-a, r)),
-            arc(-a, 0.0, math.pi / 2), line((-a, -r), (a, -r))]
+    segs = [arc(a, 0.0, math.pi / 2, 0.0), line((a, r), (-a, r)),
+            arc(-a, 0.0, math.pi, math.pi / 2), line((-a, -r), (a, -r)),
+            arc(a, 0.0, math.pi / 2, 3 * math.pi / 2)]
     L = sum(s.length for s in segs)
     return Billiard([segs], math.pi * r * r + 4 * a * r, L, "stadium")
 

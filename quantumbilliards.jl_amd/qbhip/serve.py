@@ -1,0 +1,266 @@
+"""Continuous-batching server for Beyn spectral windows.
+
+A request is one disk (geometry, k0, contour radius R, tolerance).  The
+server keeps the boundary quadrature of every (shape, N) resident on the
+device and runs windows in `slots` device contexts (each with its own HIP
+streams), so windows admitted together overlap on the GPU.  A request is
+admitted when a slot is free and its HBM footprint (the N^2 matrices of all
+live contour nodes plus the N x r solve buffers, `qbhip.footprint`) fits in
+the budget next to the windows already running; requests that do not fit
+wait while smaller ones behind them may start (first fit).  The host part of
+each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
+windows' device work.  Latency is measured from arrival to result.
+
+No counterpart in the reference: its driver (compute_spectrum_beyn,
+beyn.jl:778-921) loops over disks serially on the CPU.  The per-window
+computation is the same construct_B_matrix + eigen!(B) as qbhip.beyn.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from concurrent.futures import Future, ThreadPoolExecutor
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import geometry as G
+from .beyn import B_from_moments, Window, footprint, probe_block, probe_count
+from .context import Context
+from .solvers import CFIE_kress, Geometry, chebyshev_params
+
+SHAPES = {
+    "circle": lambda: G.make_circle(1.0),
+    "ellipse": lambda: G.make_full_ellipse(1.0, 0.6),
+    "stadium": lambda: G.make_full_stadium(1.0, 1.0),
+    "star": lambda: G.make_star_polar_boundary(1.0, 0.2, 5),
+    "circle_hole": lambda: G.CircularHoleBilliard(1.0, holes=[(0.3, 0.35, 0.0)]),
+}
+
+
+@dataclass
+class WindowRequest:
+    rid: int
+    shape: str
+    k0: float
+    R: float
+    tol: float = 1e-12      # table tolerance and refinement backward error
+    nq: int = 32
+    ppw: float = 10.0       # points per wavelength: N = ppw k L / 2pi (rounded up, even)
+    precision: str = "mixed"
+    svd_tol: float = 1e-10
+    seed: int = 0
+    arrival: float = 0.0    # seconds after the stream starts
+    N: Optional[List[int]] = None  # explicit node counts per component
+
+
+@dataclass
+class WindowResult:
+    rid: int
+    lam: np.ndarray             # all eigenvalues of B
+    inside: np.ndarray          # those with |lam - k0| <= R
+    rk: int
+    r: int
+    N: int
+    t_arrival: float
+    t_admit: float
+    t_done: float
+    device_s: float
+    host_s: float
+    stats: dict = field(default_factory=dict)
+
+    @property
+    def latency(self) -> float:
+        return self.t_done - self.t_arrival
+
+
+def node_counts(billiard: G.Billiard, k: float, ppw: float, mult: int = 2) -> List[int]:
+    out = []
+    for comp in billiard.full_boundary:
+        L = sum(c.length for c in comp)
+        n = int(math.ceil(ppw * k * L / (2 * math.pi)))
+        out.append(max(16, n + (-n) % mult))
+    return out
+
+
+class Server:
+    def __init__(self, slots: int = 2, hbm_budget: Optional[int] = None, device: int = 0,
+                 host_threads: int = 4):
+        self.slots = [Context(device) for _ in range(slots)]
+        self.free_slots = list(range(slots))
+        self.budget = int(hbm_budget if hbm_budget is not None else 200 * 2 ** 30)
+        self.live_bytes = 0
+        self.lock = threading.Condition()
+        self.pending: List = []          # (req, future, t_arrival)
+        self.geoms: Dict = {}            # (slot, shape, N tuple) -> Geometry
+        self.pts_cache: Dict = {}        # (shape, N tuple) -> pts
+        self.cheb_cache: Dict = {}       # (shape, N tuple, k0 bucket, tol) -> cheb params
+        self.host_pool = ThreadPoolExecutor(max_workers=host_threads)
+        self.workers: List[threading.Thread] = []
+        self.closed = False
+        self.dispatcher = threading.Thread(target=self._dispatch, daemon=True)
+        self.dispatcher.start()
+
+    # -- public --------------------------------------------------------------
+    def submit(self, req: WindowRequest, t_arrival: Optional[float] = None) -> Future:
+        fut: Future = Future()
+        with self.lock:
+            if self.closed:
+                raise RuntimeError("server closed")
+            self.pending.append((req, fut, time.perf_counter() if t_arrival is None else t_arrival))
+            self.lock.notify_all()
+        return fut
+
+    def run_stream(self, requests: List[WindowRequest]) -> List[WindowResult]:
+        """Submit requests at their arrival offsets and wait for all."""
+        t0 = time.perf_counter()
+        futs = []
+        for req in sorted(requests, key=lambda r: r.arrival):
+            dt = t0 + req.arrival - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            futs.append(self.submit(req, t0 + req.arrival))
+        return [f.result() for f in futs]
+
+    def close(self):
+        with self.lock:
+            self.closed = True
+            self.lock.notify_all()
+        self.dispatcher.join()
+        for w in self.workers:
+            w.join()
+        self.host_pool.shutdown(wait=True)
+        for g in self.geoms.values():
+            g.close()
+        for c in self.slots:
+            c.close()
+
+    # -- internals -----------------------------------------------------------
+    def _prepare(self, req: WindowRequest):
+        b = SHAPES[req.shape]()
+        Ns = tuple(req.N) if req.N is not None else tuple(node_counts(b, req.k0 + req.R, req.ppw))
+        key = (req.shape, Ns)
+        if key not in self.pts_cache:
+            self.pts_cache[key] = G.evaluate_points(CFIE_kress(req.ppw, b), b, req.k0, N=list(Ns))
+        r = probe_count(b, req.k0, req.R, req.nq, req.svd_tol)
+        N = int(sum(Ns))
+        r = min(r, N)
+        fp = footprint(N, req.nq, req.precision, r)
+        return b, Ns, N, r, fp
+
+    def _dispatch(self):
+        while True:
+            with self.lock:
+                while True:
+                    if self.closed and not self.pending:
+                        return
+                    pick = None
+                    if self.free_slots:
+                        for idx, (req, fut, ta) in enumerate(self.pending):
+                            prep = self._prepare(req)
+                            fp = prep[4]
+                            if fp > self.budget:
+                                self.pending.pop(idx)
+                                fut.set_exception(MemoryError(
+                                    f"request {req.rid}: footprint {fp} B exceeds the budget {self.budget} B"))
+                                pick = "rejected"
+                                break
+                            if self.live_bytes + fp <= self.budget:
+                                pick = (idx, prep)
+                                break
+                    if pick == "rejected":
+                        continue
+                    if pick is not None:
+                        idx, prep = pick
+                        req, fut, ta = self.pending.pop(idx)
+                        slot = self.free_slots.pop(0)
+                        self.live_bytes += prep[4]
+                        break
+                    self.lock.wait(timeout=0.05)
+            t = threading.Thread(target=self._run, args=(slot, req, fut, ta, prep), daemon=True)
+            self.workers.append(t)
+            t.start()
+
+    def _run(self, slot, req, fut, t_arrival, prep):
+        b, Ns, N, r, fp = prep
+        t_admit = time.perf_counter()
+        try:
+            ctx = self.slots[slot]
+            pts = self.pts_cache[(req.shape, Ns)]
+            gkey = (slot, req.shape, Ns)
+            if gkey not in self.geoms:
+                self.geoms[gkey] = Geometry(pts, ctx)
+            geom = self.geoms[gkey]
+            solver = CFIE_kress(req.ppw, b)
+            zj = req.k0 + req.R * np.exp(2j * np.pi * np.arange(req.nq) / req.nq)
+            ck = (req.shape, Ns, round(req.k0, 3), round(req.R, 3), req.nq, req.tol)
+            if ck not in self.cheb_cache:
+                nh, mh, nj, mj, _ = chebyshev_params(solver, pts, zj, tol=max(req.tol, 1e-13),
+                                                     geom=geom)
+                self.cheb_cache[ck] = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
+            cheb = self.cheb_cache[ck]
+            win = Window(solver, pts, req.k0, req.R, req.nq, precision=req.precision, geom=geom,
+                         **cheb)
+            V = probe_block(np.random.default_rng(req.seed), N, r)
+            A0, A1 = win.moments(V, max(req.tol, 1e-13))
+            st = dict(win.stats)
+            win.close()
+        except Exception as e:  # release and report
+            self._release(slot, fp)
+            fut.set_exception(e)
+            return
+        t_dev = time.perf_counter() - t_admit
+        self._release(slot, fp)
+
+        def host():
+            th = time.perf_counter()
+            B, Uk, S, rk = B_from_moments(A0, A1, req.svd_tol)
+            lam = np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128)
+            t_done = time.perf_counter()
+            inside = np.sort_complex(lam[np.abs(lam - req.k0) <= req.R])
+            return WindowResult(req.rid, lam, inside, rk, r, N, t_arrival, t_admit, t_done, t_dev,
+                                t_done - th, st)
+
+        hf = self.host_pool.submit(host)
+        hf.add_done_callback(lambda f: fut.set_exception(f.exception()) if f.exception()
+                             else fut.set_result(f.result()))
+
+    def _release(self, slot, fp):
+        with self.lock:
+            self.live_bytes -= fp
+            self.free_slots.append(slot)
+            self.free_slots.sort()
+            self.lock.notify_all()
+
+
+def summarize(results: List[WindowResult], t_span: Optional[float] = None) -> dict:
+    lat = np.array([r.latency for r in results])
+    n_eig = int(sum(len(r.inside) for r in results))
+    if t_span is None:
+        t_span = max(r.t_done for r in results) - min(r.t_arrival for r in results)
+    return {"windows": len(results), "eigenvalues": n_eig,
+            "eigenvalues_per_s": n_eig / t_span if t_span > 0 else None,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "span_s": t_span}
+
+
+def poisson_stream(n: int, rate: float, seed: int = 0, shapes=("ellipse", "stadium", "star"),
+                   kmin: float = 50.0, kmax: float = 1000.0, nq: int = 32,
+                   precision: str = "mixed") -> List[WindowRequest]:
+    """Seeded request stream of the serving workload: Poisson arrivals, shapes
+    drawn from `shapes`, k0 log-uniform in [kmin, kmax], R uniform in
+    [0.25, 0.5], tol 1e-8 or 1e-12."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        out.append(WindowRequest(rid=i, shape=str(rng.choice(shapes)),
+                                 k0=float(np.exp(rng.uniform(np.log(kmin), np.log(kmax)))),
+                                 R=float(rng.uniform(0.25, 0.5)),
+                                 tol=float(rng.choice([1e-8, 1e-12])), nq=nq,
+                                 precision=precision, seed=1000 + i, arrival=t))
+    return out

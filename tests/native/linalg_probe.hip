@@ -111,3 +111,30 @@ static double bench_gemm(qb_ctx* ctx, int M, int N, int K, int batch, int iters)
 extern "C" double qbt_gemm_bench(qb_ctx* c, int M, int N, int K, int batch, int iters, int dbl) {
   return dbl ? bench_gemm<cd>(c, M, N, K, batch, iters) : bench_gemm<cf>(c, M, N, K, batch, iters);
 }
+
+// Plain copy of `bytes` bytes (16 B per thread per step): the reference
+// kernel for scaling FETCH_SIZE / WRITE_SIZE counters.  Returns GB/s.
+__global__ void k_copy16(const double2* __restrict__ a, double2* __restrict__ b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) b[i] = a[i];
+}
+extern "C" double qbt_copy_bench(qb_ctx* ctx, int64_t bytes, int iters) {
+  double2 *a, *b;
+  int64_t n = bytes / 16;
+  if (hipMalloc(&a, n * 16) || hipMalloc(&b, n * 16)) return -1;
+  hipMemsetAsync(a, 0, n * 16, ctx->stream);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, ctx->stream);
+  for (int i = 0; i < iters; ++i)
+    hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, ctx->stream, a, b, n);
+  hipEventRecord(e1, ctx->stream);
+  hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipFree(a);
+  hipFree(b);
+  return 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
+}

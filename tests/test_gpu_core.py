@@ -112,3 +112,31 @@ def test_beyn_circle_gives_bessel_zeros(precision):
     print(precision, inside - ex if len(inside) == len(ex) else (inside, ex))
     assert len(inside) == len(ex)
     assert np.max(np.abs(inside - ex) / ex) < 1e-12
+
+
+def test_dlp_ellipse_full_size_sampled_columns():
+    """Config 2 at full size: DLP-Kress on the ellipse (1, 0.6), N = 4096,
+    z = 400 + 0.5 e^{i theta} (8 of the 64 contour nodes), tables tuned to
+    1e-12 on the device; sampled columns against the oracle."""
+    q = qb()
+    b = q.make_full_ellipse(1.0, 0.6)
+    solver = q.DLP_kress(10.0, b)
+    pts = q.evaluate_points(solver, b, 400.0, N=[4096])
+    zall, _ = O.contour(400.0, 0.5, 64)
+    nh, mh, nj, mj, errs = q.chebyshev_params(solver, pts, zall, tol=1e-12)
+    assert max(errs) < 1e-12
+    zs = zall[::8]
+    T = q.assemble(solver, pts, zs, n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
+    comp = O.make_geometry("ellipse", [4096])[0]
+    rmin, rmax = O._rrange_single_fast(comp)
+    cp = O.ChebParams(nh, mh, nj, mj)
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for qi, z in enumerate(zs):
+        j0 = int(rng.integers(0, 4096 - 32))
+        F = O.dlp_assemble_cols(comp, z, cp, j0, j0 + 32, rmin, rmax)
+        G = np.asarray(T[qi][:, j0:j0 + 32])
+        s = np.max(np.abs(F))
+        worst = max(worst, np.max(np.abs(G - F)) / s)
+    print("cfg2 worst normwise", worst, (nh, mh, nj, mj))
+    assert worst < 1e-12

@@ -1,0 +1,66 @@
+"""Serving loop: continuous batching with HBM-footprint admission.  Each
+window's result must be the same when it runs alone and when it runs in a
+mix with others (same probes, same kernels)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _reqs():
+    from qbhip.serve import WindowRequest
+    return [
+        WindowRequest(0, "circle", 20.0, 0.5, tol=1e-12, nq=24, ppw=13, seed=11),
+        WindowRequest(1, "ellipse", 24.0, 0.4, tol=1e-8, nq=24, ppw=12, seed=12),
+        WindowRequest(2, "star", 18.0, 0.45, tol=1e-12, nq=24, ppw=12, seed=13, precision="fp64"),
+        WindowRequest(3, "stadium", 15.0, 0.3, tol=1e-8, nq=24, ppw=12, seed=14),
+        WindowRequest(4, "circle_hole", 16.0, 0.35, tol=1e-12, nq=24, ppw=12, seed=15),
+        WindowRequest(5, "circle", 26.0, 0.25, tol=1e-12, nq=24, ppw=12, seed=16),
+    ]
+
+
+def test_alone_equals_mixed_and_circle_is_exact():
+    from qbhip.serve import Server, summarize
+    reqs = _reqs()
+    srv = Server(slots=1)
+    alone = {q.rid: srv.submit(q).result() for q in reqs}
+    srv.close()
+    srv = Server(slots=3)
+    futs = [srv.submit(q) for q in reqs]
+    mixed = {f.result().rid: f.result() for f in futs}
+    srv.close()
+    for rid in alone:
+        a, m = alone[rid].inside, mixed[rid].inside
+        assert len(a) == len(m) and len(a) > 0, rid
+        assert np.max(np.abs(a - m) / np.abs(a)) <= 1e-10, rid
+    # the circle windows are checked against the zeros of J_m
+    for rid, k0, R in ((0, 20.0, 0.5), (5, 26.0, 0.25)):
+        ex = O.jm_zeros_in_disk(k0, R)
+        got = np.sort(mixed[rid].inside.real)
+        assert len(got) == len(ex)
+        assert np.max(np.abs(got - ex) / ex) < 1e-10
+    s = summarize(list(mixed.values()))
+    assert s["eigenvalues"] > 0 and s["p99_latency_s"] >= s["p50_latency_s"]
+
+
+def test_admission_by_footprint():
+    from qbhip.beyn import footprint
+    from qbhip.serve import Server, WindowRequest
+    reqs = _reqs()[:3]
+    srv0 = Server(slots=1)
+    fps = [srv0._prepare(q)[4] for q in reqs]
+    srv0.close()
+    # budget for the largest one only: windows must run one at a time
+    srv = Server(slots=3, hbm_budget=max(fps))
+    res = [f.result() for f in [srv.submit(q) for q in reqs]]
+    srv.close()
+    spans = sorted((r.t_admit, r.t_admit + r.device_s) for r in res)
+    for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+        assert b0 >= a1 - 1e-3, "admitted while the budget was full"
+    # a request whose footprint exceeds the whole budget is rejected
+    srv = Server(slots=1, hbm_budget=min(fps) // 2)
+    with pytest.raises(MemoryError):
+        srv.submit(reqs[0]).result()
+    srv.close()
