@@ -192,6 +192,8 @@ def main():
         print(f"# warmup {w}: {dt:.2f}s rk={info['rk']} r={r_used} svd={info['t_host_svd_s']:.2f}s "
               f"eig={info['t_host_eig_s']:.2f}s stats={info['window_stats']}",
               file=sys.stderr, flush=True)
+    rep = ctx.autotune(a.N, a.nq, r_used, "fp64") + ctx.autotune(a.N, a.nq, r_used, "mixed")
+    print("# gemm autotune:\n" + rep, file=sys.stderr, flush=True)
     ref = None
     fp64_s = None
     if not a.no_fp64:
@@ -241,6 +243,7 @@ def main():
         "fp64_window_s": fp64_s,
         "speedup_vs_fp64_path": (fp64_s / mean) if fp64_s else None,
         "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]} for k, v in ks.items()},
+        "gemm_autotune": [l for l in rep.splitlines() if l],
         "roofline": {"kernel": "k_gemm_sub<complex64> (LU trailing update, MFMA)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK["c64"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK["c64"], "avg_launch_ms": avg_ms, "traffic": None},

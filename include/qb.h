@@ -66,8 +66,11 @@ typedef struct {
 int qb_ctx_create(int device, qb_ctx** out);
 void qb_ctx_destroy(qb_ctx* ctx);
 const char* qb_last_error(const qb_ctx* ctx);
-/* Bytes currently held on the device by this context's objects. */
+/* Bytes currently held on the device by this context (objects plus the
+ * allocation cache that keeps freed buffers for windows of the same size). */
 int64_t qb_ctx_device_bytes(const qb_ctx* ctx);
+/* Set the cache limit in bytes (0 releases the cache and disables it). */
+int qb_ctx_set_cache_limit(qb_ctx* ctx, int64_t bytes);
 
 /* Kernel classes for the per-context launch accounting below. */
 enum {
@@ -89,6 +92,15 @@ int qb_ctx_set_profiling(qb_ctx* ctx, int32_t enable);
  * launches (complex GEMM: 8 M N K flops; assembly: 16 B per entry written). */
 int qb_ctx_kernel_stats(qb_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches,
                         double* flops, double* bytes);
+
+/* Tile autotuning: times each MFMA GEMM tile variant on the shapes a window
+ * of size N with nq nodes and r probes launches (LU trailing/panel updates,
+ * triangular-solve halves, the FP64 refinement residual) and keeps the
+ * fastest per shape class for every later launch on this context.  report
+ * (may be NULL) receives one line per tuned shape.  No counterpart in the
+ * reference (its BLAS picks kernels internally). */
+int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, int32_t precision,
+                    char* report, int64_t report_len);
 
 /* Boundary discretisation, one or more periodic components, holes already
  * orientation-reversed by the caller.  Replaces the inputs the reference
@@ -125,7 +137,9 @@ int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
                      const qb_cheb_params* cheb, int32_t precision,
                      qb_window** out, qb_stats* stats);
 void qb_window_destroy(qb_window* w);
-/* Device bytes a window of this size would hold (the admission footprint). */
+/* Device bytes a window of this size would hold (the admission footprint):
+ * matrices, pivots, solve buffers for r probes, and up to 32 MiB of tables
+ * per node. */
 int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r);
 
 /* The moment loop of construct_B_matrix (beyn.jl:278-285): X_j = T(z_j)^{-1} V,

@@ -19,10 +19,37 @@ int qb_fail(qb_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
+// Device memory of a context.  Freed blocks stay in a per-context cache keyed
+// by exact size (windows of one size allocate the same list of buffers), up
+// to cache_limit bytes; an allocation that fails releases the cache and
+// retries once.  `bytes` counts memory handed out plus cached.
+static void cache_release(qb_ctx* ctx) {
+  for (auto& e : ctx->cache) {
+    (void)hipFree(e.second);
+    ctx->bytes -= (int64_t)e.first;
+  }
+  ctx->cache.clear();
+  ctx->cache_bytes = 0;
+}
+
 int qb_dev_alloc(qb_ctx* ctx, void** p, size_t bytes) {
   *p = nullptr;
   if (bytes == 0) return QB_OK;
+  for (size_t i = 0; i < ctx->cache.size(); ++i) {
+    if (ctx->cache[i].first == bytes) {
+      *p = ctx->cache[i].second;
+      ctx->cache_bytes -= (int64_t)bytes;
+      ctx->cache.erase(ctx->cache.begin() + i);
+      return QB_OK;
+    }
+  }
   hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess && !ctx->cache.empty()) {
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(ctx->stream);
+    cache_release(ctx);
+    e = hipMalloc(p, bytes);
+  }
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
@@ -36,6 +63,14 @@ int qb_dev_alloc(qb_ctx* ctx, void** p, size_t bytes) {
 
 void qb_dev_free(qb_ctx* ctx, void* p, size_t bytes) {
   if (!p) return;
+  if (bytes > 0 && ctx->cache_bytes + (int64_t)bytes <= ctx->cache_limit) {
+    // the block may still be in use by queued work on this context's
+    // streams; reuse only happens through this context, whose later
+    // launches are ordered after that work
+    ctx->cache.push_back({bytes, p});
+    ctx->cache_bytes += (int64_t)bytes;
+    return;
+  }
   (void)hipFree(p);
   ctx->bytes -= (int64_t)bytes;
 }
@@ -103,6 +138,8 @@ int qb_ctx_create(int device, qb_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return QB_E_HIP;
   qb_ctx* c = new qb_ctx();
   c->device = device;
+  for (int t = 0; t < 2; ++t)
+    for (int k = 0; k < QB_GEMM_CLASSES; ++k) c->gemm_pick[t][k] = -1;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream_aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming) != hipSuccess ||
@@ -125,6 +162,8 @@ void qb_ctx_destroy(qb_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream_aux);
     (void)hipStreamDestroy(ctx->stream_aux);
   }
+  (void)hipStreamSynchronize(ctx->stream);
+  cache_release(ctx);
   if (ctx->ev_main) (void)hipEventDestroy(ctx->ev_main);
   if (ctx->ev_aux) (void)hipEventDestroy(ctx->ev_aux);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -134,6 +173,15 @@ void qb_ctx_destroy(qb_ctx* ctx) {
 const char* qb_last_error(const qb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int64_t qb_ctx_device_bytes(const qb_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int qb_ctx_set_cache_limit(qb_ctx* ctx, int64_t bytes) {
+  if (!ctx || bytes < 0) return QB_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->cache_limit = bytes;
+  if (ctx->cache_bytes > bytes) cache_release(ctx);
+  return QB_OK;
+}
 
 int qb_ctx_set_profiling(qb_ctx* ctx, int32_t enable) {
   if (!ctx) return QB_E_ARG;
@@ -491,6 +539,8 @@ extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision,
   // per node in mixed mode, V, A0, A1
   b += 2 * (size_t)nq * nv * sizeof(cd) + 3 * nv * sizeof(cd);
   if (precision == QB_MIXED) b += (size_t)nq * nv * sizeof(cf);
+  // Chebyshev tables, bounded by 1.2e5 panels of degree 7 per node
+  b += (size_t)nq * (32u << 20);
   return (int64_t)b;
 }
 
@@ -940,4 +990,55 @@ extern "C" int qb_cheb_tune(qb_ctx* ctx, const qb_geom* g, int32_t op, const dou
     for (int c = 0; c < 4; ++c) errs[c] = e4[c];
   qb_dev_free(ctx, zd, nz * sizeof(cd));
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// GEMM tile autotuning for the shapes of one window
+// ---------------------------------------------------------------------------
+extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, int32_t precision,
+                               char* report, int64_t report_len) {
+  if (!ctx) return QB_E_ARG;
+  if (N < 64 || N > 65535 || nq < 1 || r < 1 || r > N)
+    return qb_fail(ctx, QB_E_ARG, "autotune needs 64 <= N <= 65535, nq >= 1, 1 <= r <= N");
+  if (precision != QB_FP64 && precision != QB_MIXED)
+    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64 or QB_MIXED");
+  (void)hipSetDevice(ctx->device);
+  struct Shape {
+    int M, N, K, ty;
+  };
+  std::vector<Shape> shapes;
+  const int n = (int)N;
+  const int lu_ty = precision == QB_MIXED ? 0 : 1;
+  // LU: trailing updates (K = 256), block-row TRSM updates, panel updates
+  shapes.push_back({n - 256, n - 256, 256, lu_ty});
+  shapes.push_back({n / 2, n / 2, 256, lu_ty});
+  shapes.push_back({n / 4, n / 4, 256, lu_ty});
+  shapes.push_back({128, n / 2, 128, lu_ty});
+  for (int w = 128; w >= 32; w /= 2) shapes.push_back({n / 2, w, w, lu_ty});
+  // triangular solves with r right-hand sides (recursive halves)
+  for (int h = n / 2; h >= 32; h /= 2) shapes.push_back({h, r, h, lu_ty});
+  // FP64 refinement residual
+  if (precision == QB_MIXED) shapes.push_back({n, r, n, 1});
+  std::string rep;
+  std::vector<char> seen(2 * QB_GEMM_CLASSES, 0);
+  for (const Shape& sh : shapes) {
+    if (sh.M < 1 || sh.N < 1 || sh.K < 1) continue;
+    int cls = qb_gemm_class(sh.M, sh.N, sh.K);
+    if (seen[sh.ty * QB_GEMM_CLASSES + cls]) continue;
+    seen[sh.ty * QB_GEMM_CLASSES + cls] = 1;
+    double tf = 0;
+    int v = sh.ty == 0 ? qb_gemm_tune<cf>(ctx, sh.M, sh.N, sh.K, nq, &tf)
+                       : qb_gemm_tune<cd>(ctx, sh.M, sh.N, sh.K, nq, &tf);
+    if (v < 0) return v;
+    char line[160];
+    snprintf(line, sizeof(line), "%s %dx%dx%d -> %s (%.1f TF/s)\n", sh.ty == 0 ? "c64" : "c128",
+             sh.M, sh.N, sh.K, qb_gemm_variant_name(sh.ty, v), tf);
+    rep += line;
+  }
+  if (report && report_len > 0) {
+    size_t m = std::min((size_t)(report_len - 1), rep.size());
+    memcpy(report, rep.data(), m);
+    report[m] = 0;
+  }
+  return QB_OK;
 }

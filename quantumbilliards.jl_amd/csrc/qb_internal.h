@@ -9,6 +9,8 @@
 #include "../../include/qb.h"
 
 #define QB_MAX_COMP 8
+#define QB_GEMM_VARIANTS 5
+#define QB_GEMM_CLASSES 45
 
 // Boundary geometry resident on the device.  Node arrays are concatenated
 // over components; the Kress R matrix and the log term are circulant per
@@ -43,6 +45,11 @@ struct qb_ctx {
   hipEvent_t ev_main = nullptr, ev_aux = nullptr;  // cross-stream ordering
   std::string err;
   int64_t bytes = 0;  // device bytes held by objects of this context
+  int gemm_pick[2][QB_GEMM_CLASSES];  // autotuned tile variant per shape class (-1: default)
+  // caching allocator: freed blocks are kept (by exact size) for reuse
+  std::vector<std::pair<size_t, void*>> cache;
+  int64_t cache_bytes = 0;
+  int64_t cache_limit = (int64_t)96 << 30;
   bool prof = false;
   std::vector<QbProfRec> prof_recs;      // recorded, not yet reduced
   std::vector<hipEvent_t> prof_pool;     // reusable events

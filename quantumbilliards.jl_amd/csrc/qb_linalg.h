@@ -32,3 +32,10 @@ int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int64_t stride, int 
                    double* out_dev);
 int qb_node_maxabs32(qb_ctx* ctx, const cf* X, int64_t n_per, int nb, double* out_dev);
 int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev);
+
+// GEMM autotuning (qb_linalg.hip): shape class, tuning of one shape (returns
+// the chosen variant or a negative error), and variant names (ty 0: complex64).
+int qb_gemm_class(int M, int N, int K);
+template <typename T>
+int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflops);
+const char* qb_gemm_variant_name(int ty, int v);

@@ -206,27 +206,125 @@ static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t ld
                      A, lda, sA, B, ldb, sB, C, ldc, sC);
 }
 
+// Tile variants and the per-context choice.  Variant 0/1 are the defaults
+// (wide / narrow N); the autotuner (qb_gemm_tune) may pick another per shape
+// class, measured on the device.
+typedef void (*gemm_fn_c64)(qb_ctx*, int, int, int, const cf*, int64_t, int64_t, const cf*,
+                            int64_t, int64_t, cf*, int64_t, int64_t, int);
+typedef void (*gemm_fn_c128)(qb_ctx*, int, int, int, const cd*, int64_t, int64_t, const cd*,
+                             int64_t, int64_t, cd*, int64_t, int64_t, int);
+static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
+    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 256, 64, 16, 4, 1>,
+    launch_gemm<cf, 64, 256, 16, 1, 4>,  launch_gemm<cf, 128, 64, 16, 2, 2>,
+    launch_gemm<cf, 128, 128, 32, 2, 2>};
+static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
+    launch_gemm<cd, 128, 64, 16, 2, 2>, launch_gemm<cd, 128, 32, 16, 4, 1>,
+    launch_gemm<cd, 64, 64, 16, 2, 2>,  launch_gemm<cd, 64, 128, 16, 2, 2>,
+    launch_gemm<cd, 256, 32, 16, 4, 1>};
+static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
+    {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32"},
+    {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16"}};
+
+int qb_gemm_class(int M, int N, int K) {
+  int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);
+  int cn = N <= 32 ? 0 : N <= 64 ? 1 : N <= 256 ? 2 : N <= 1024 ? 3 : 4;
+  int ck = K <= 64 ? 0 : K <= 256 ? 1 : 2;
+  return (cm * 5 + cn) * 3 + ck;
+}
+
+template <typename T>
+static int gemm_default(int N) {
+  if constexpr (sizeof(T) == sizeof(cf))
+    return N <= 64 ? 1 : 0;
+  else
+    return N <= 32 ? 1 : 0;
+}
+
+template <typename T>
+static void gemm_launch_variant(qb_ctx* ctx, int v, int M, int N, int K, const T* A, int64_t lda,
+                                int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C,
+                                int64_t ldc, int64_t sC, int batch) {
+  if constexpr (sizeof(T) == sizeof(cf))
+    kGemmC64[v](ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
+  else
+    kGemmC128[v](ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
+}
+
 template <typename T>
 int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA, const T* B,
              int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
-  int slot = qb_prof_begin(ctx, sizeof(T) == sizeof(cf) ? QB_K_GEMM_C64 : QB_K_GEMM_C128,
+  const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
+  int v = ctx->gemm_pick[ty][qb_gemm_class(M, N, K)];
+  if (v < 0) v = gemm_default<T>(N);
+  int slot = qb_prof_begin(ctx, ty == 0 ? QB_K_GEMM_C64 : QB_K_GEMM_C128,
                            8.0 * M * N * K * (double)batch,
                            (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
-  if constexpr (sizeof(T) == sizeof(cf)) {
-    if (N <= 64)
-      launch_gemm<T, 256, 64, 16, 4, 1>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
-    else
-      launch_gemm<T, 128, 128, 16, 2, 2>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
-  } else {
-    if (N <= 32)
-      launch_gemm<T, 128, 32, 16, 4, 1>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
-    else
-      launch_gemm<T, 128, 64, 16, 2, 2>(ctx, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
-  }
+  gemm_launch_variant<T>(ctx, v, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
   qb_prof_end(ctx, slot);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
+}
+
+// Autotuner: time every variant on synthetic operands of the given shape
+// (batch capped so the operands stay below ~1.5 GiB) and keep the fastest
+// for the shape's class.  Returns the chosen variant.
+__global__ void k_fill_bits(double* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 1e-3 * (double)((i * 2654435761ull) % 997) / 997.0;
+}
+
+template <typename T>
+int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflops) {
+  const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
+  size_t per = ((size_t)M * K + (size_t)K * N + (size_t)M * N) * sizeof(T);
+  int b = batch;
+  while (b > 1 && per * b > ((size_t)3 << 29)) --b;
+  T *A = nullptr, *B = nullptr, *C = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&A, (size_t)M * K * b * sizeof(T));
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&B, (size_t)K * N * b * sizeof(T));
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&C, (size_t)M * N * b * sizeof(T));
+  int best = gemm_default<T>(N);
+  double best_ms = 1e300;
+  if (!rc) {
+    int64_t na = (int64_t)M * K * b * sizeof(T) / 8, nb = (int64_t)K * N * b * sizeof(T) / 8,
+            nc = (int64_t)M * N * b * sizeof(T) / 8;
+    hipLaunchKernelGGL(k_fill_bits, dim3((na + 255) / 256), dim3(256), 0, ctx->stream, (double*)A, na);
+    hipLaunchKernelGGL(k_fill_bits, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, (double*)B, nb);
+    hipLaunchKernelGGL(k_fill_bits, dim3((nc + 255) / 256), dim3(256), 0, ctx->stream, (double*)C, nc);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    for (int v = 0; v < QB_GEMM_VARIANTS; ++v) {
+      gemm_launch_variant<T>(ctx, v, M, N, K, A, M, (int64_t)M * K, B, K, (int64_t)K * N, C, M,
+                             (int64_t)M * N, b);
+      (void)hipEventRecord(e0, ctx->stream);
+      for (int it = 0; it < 3; ++it)
+        gemm_launch_variant<T>(ctx, v, M, N, K, A, M, (int64_t)M * K, B, K, (int64_t)K * N, C, M,
+                               (int64_t)M * N, b);
+      (void)hipEventRecord(e1, ctx->stream);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+          ms > 0.f && ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->gemm_pick[ty][qb_gemm_class(M, N, K)] = best;
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, A, (size_t)M * K * b * sizeof(T));
+  qb_dev_free(ctx, B, (size_t)K * N * b * sizeof(T));
+  qb_dev_free(ctx, C, (size_t)M * N * b * sizeof(T));
+  if (best_tflops) *best_tflops = best_ms < 1e299 ? 8.0 * M * N * (double)K * b * 3 / (best_ms * 1e-3) / 1e12 : 0;
+  return rc ? rc : best;
+}
+
+const char* qb_gemm_variant_name(int ty, int v) {
+  return (ty >= 0 && ty < 2 && v >= 0 && v < QB_GEMM_VARIANTS) ? kGemmNames[ty][v] : "?";
 }
 
 // ---------------------------------------------------------------------------
@@ -811,6 +909,8 @@ template int qb_trsm_lu_batched<cf>(qb_ctx*, const cf*, int, int64_t, int64_t, c
                                     int64_t, int);
 template int qb_trsm_lu_batched<cd>(qb_ctx*, const cd*, int, int64_t, int64_t, cd*, int, int64_t,
                                     int64_t, int);
+template int qb_gemm_tune<cf>(qb_ctx*, int, int, int, int, double*);
+template int qb_gemm_tune<cd>(qb_ctx*, int, int, int, int, double*);
 template int gemm_sub<cf>(qb_ctx*, int, int, int, const cf*, int64_t, int64_t, const cf*, int64_t,
                           int64_t, cf*, int64_t, int64_t, int);
 template int gemm_sub<cd>(qb_ctx*, int, int, int, const cd*, int64_t, int64_t, const cd*, int64_t,

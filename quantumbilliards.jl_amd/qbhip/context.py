@@ -34,6 +34,19 @@ class Context:
             out[name] = {"ms": ms.value, "launches": n.value, "flops": fl.value, "bytes": by.value}
         return out
 
+    def set_cache_limit(self, nbytes: int):
+        """Bytes of freed device buffers kept for reuse (0 disables caching)."""
+        _lib.check(self.ptr, _lib.lib().qb_ctx_set_cache_limit(self.ptr, int(nbytes)))
+
+    def autotune(self, N: int, nq: int, r: int, precision: str = "mixed") -> str:
+        """Pick GEMM tile variants for the shapes of an (N, nq, r) window
+        (qb_ctx_autotune); returns the report."""
+        buf = C.create_string_buffer(8192)
+        prec = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED}[precision]
+        _lib.check(self.ptr, _lib.lib().qb_ctx_autotune(self.ptr, int(N), int(nq), int(r), prec,
+                                                        buf, len(buf)))
+        return buf.value.decode()
+
     def close(self):
         if getattr(self, "ptr", None):
             _lib.lib().qb_ctx_destroy(self.ptr)

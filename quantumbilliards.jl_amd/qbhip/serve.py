@@ -89,6 +89,10 @@ class Server:
     def __init__(self, slots: int = 2, hbm_budget: Optional[int] = None, device: int = 0,
                  host_threads: int = 4):
         self.slots = [Context(device) for _ in range(slots)]
+        for c in self.slots:
+            # windows of different sizes come and go: no per-slot buffer cache,
+            # so the footprint accounting is the memory actually held
+            c.set_cache_limit(0)
         self.free_slots = list(range(slots))
         self.budget = int(hbm_budget if hbm_budget is not None else 200 * 2 ** 30)
         self.live_bytes = 0

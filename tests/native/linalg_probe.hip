@@ -138,3 +138,9 @@ extern "C" double qbt_copy_bench(qb_ctx* ctx, int64_t bytes, int iters) {
   hipFree(b);
   return 2.0 * n * 16 * iters / (ms * 1e-3) / 1e9;
 }
+
+// Force every shape class of one operand type to tile variant v (-1 restores
+// the default choice), so tests can exercise each variant.
+extern "C" void qbt_force_variant(qb_ctx* ctx, int ty, int v) {
+  for (int k = 0; k < QB_GEMM_CLASSES; ++k) ctx->gemm_pick[ty][k] = v;
+}

@@ -85,3 +85,30 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         err = np.max(np.abs(X[i].T - ref)) / np.max(np.abs(ref))
         print("trsm", dt.__name__, n, i, err)
         assert err < tol * 10
+
+
+@pytest.mark.parametrize("variant", range(5))
+def test_every_tile_variant(probe, variant):
+    """Each MFMA tile variant the autotuner may pick computes the same GEMM,
+    LU and triangular solves."""
+    L, ctx = probe
+    c = C.c_void_p(ctx.value)
+    try:
+        for ty in (0, 1):
+            L.qbt_force_variant(c, ty, variant)
+        for dt, tol in ((np.complex128, 1e-13), (np.complex64, 3e-5)):
+            for (M, N, K, b) in [(100, 70, 33, 2), (300, 129, 128, 3), (64, 300, 40, 1)]:
+                test_gemm_sub(probe, dt, tol, M, N, K, b)
+        test_getrf_and_solve(probe, np.complex128, 1e-12, 300, 1)
+        test_getrf_and_solve(probe, np.complex64, 1e-4, 160, 2)
+    finally:
+        for ty in (0, 1):
+            L.qbt_force_variant(c, ty, -1)
+
+
+def test_autotune_report_and_results():
+    import qbhip
+    ctx = qbhip.Context(0)
+    rep = ctx.autotune(512, 4, 64, "mixed")
+    assert "c64" in rep and "c128" in rep
+    ctx.close()
