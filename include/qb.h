@@ -151,6 +151,20 @@ int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r)
 int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0,
                       double* A1, double refine_tol, qb_stats* stats);
 
+/* Residual check of candidate eigenpairs (the spurious-root filter): for
+ * q < nl, with T_q = T(lambda_q) assembled on the device exactly as by
+ * qb_assemble (same tables, same entries) and phi_q column q of phi,
+ *   res2[q] = ||T_q phi_q||_2,  resn1[q] = ||T_q phi_q||_1,  resninf[q] = ||T_q phi_q||_inf,
+ *   opn1[q] = ||T_q||_1 (max column sum),  opninf[q] = ||T_q||_inf (max row sum).
+ * lam: nl complex; phi: host N x nl complex128, column-major.  Outputs may be
+ * NULL.  The matrices are assembled `batch` at a time (0: as many as fit in
+ * 8 GiB).  Replaces construct_boundary_matrices! + mul!(y, Tbufs[j], phi) +
+ * norm/opnorm in residual_and_norm_select (beyn.jl:557-593) and in
+ * imag_k_check_EXPERIMENTAL (beyn.jl:666-687). */
+int qb_residuals(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* lam, int64_t nl,
+                 const qb_cheb_params* cheb, const double* phi, int32_t batch, double* res2,
+                 double* resn1, double* resninf, double* opn1, double* opninf);
+
 /* Device evaluation of H0^(1), H1^(1), J0, J1 at n complex points (the values
  * the reference takes from SpecialFunctions.besselh/besselj when sampling its
  * tables, chebyshev_bessels.jl:331-372).  Host arrays of n complex each. */

@@ -677,3 +677,36 @@ def jm_zeros_in_disk(k0: float, R: float, mmax: int = 200):
         if zs[0] > k0 + R:
             break
     return np.sort(np.array(out))
+
+
+# --------------------------------------------------------------------------
+# Residual check of Beyn eigenpairs
+# --------------------------------------------------------------------------
+
+
+def residual_select(T_of, lam, Uk, Y, k0: complex, R: float, res_tol: float, epss: float = 1e-15,
+                    auto_discard_spurious: bool = True):
+    """residual_and_norm_select with matnorm=:one (beyn.jl:557-593): for every
+    lam_j inside the disk, phi_j = Uk Y[:, j], y = T(lam_j) phi_j,
+    tens = ||y||_2, tensN = ||y||_1 / (||T||_1 (||phi||_1 + epss) + epss);
+    kept if not (auto_discard_spurious and tens >= res_tol).  `T_of(z)`
+    returns the N x N matrix T(z).  Returns (idx, tens[idx], tensN[idx]) and
+    the per-candidate (y norms, operator norms) for every inside index."""
+    idx, tens, tensN, detail = [], [], [], {}
+    for j, lj in enumerate(lam):
+        if abs(lj - k0) > R:
+            continue
+        T = T_of(lj)
+        phi = Uk @ Y[:, j]
+        y = T @ phi
+        r2 = float(np.linalg.norm(y))
+        nA = float(np.max(np.sum(np.abs(T), axis=0)))
+        ninf = float(np.max(np.sum(np.abs(T), axis=1)))
+        tn = float(np.sum(np.abs(y))) / (nA * (float(np.sum(np.abs(phi))) + epss) + epss)
+        detail[j] = (r2, float(np.sum(np.abs(y))), float(np.max(np.abs(y))), nA, ninf)
+        if auto_discard_spurious and r2 >= res_tol:
+            continue
+        idx.append(j)
+        tens.append(r2)
+        tensN.append(tn)
+    return np.array(idx, int), np.array(tens), np.array(tensN), detail

@@ -2,6 +2,8 @@
 // construct_boundary_matrices!, and the Beyn window (assembly + LU of every
 // contour node, then the moment solves).
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -498,6 +500,111 @@ extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const doub
 }
 
 // ---------------------------------------------------------------------------
+// residual check of candidate eigenpairs
+// ---------------------------------------------------------------------------
+extern "C" int qb_residuals(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* lam,
+                            int64_t nl, const qb_cheb_params* cheb, const double* phi,
+                            int32_t batch, double* res2, double* resn1, double* resninf,
+                            double* opn1, double* opninf) {
+  if (!ctx) return QB_E_ARG;
+  if (!g || (nl > 0 && (!lam || !phi)) || nl < 0 || batch < 0)
+    return qb_fail(ctx, QB_E_ARG, "null argument or negative count");
+  if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1)
+    return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  int rc = check_cheb(ctx, cheb);
+  if (rc) return rc;
+  if (nl == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  double rmin, rmax;
+  op_range(g, op, &rmin, &rmax);
+  const int64_t N = g->d.N;
+  const size_t per = (size_t)N * N * sizeof(cd);
+  int64_t G = batch > 0 ? batch : (int64_t)std::max<size_t>(1, ((size_t)8 << 30) / per);
+  G = std::min<int64_t>(G, nl);
+  const int nsplit = 8;
+  cd *zd = nullptr, *Td = nullptr, *phid = nullptr, *part = nullptr;
+  double *rowabs = nullptr, *colsum = nullptr;
+  const size_t b_z = G * sizeof(cd), b_T = G * per, b_phi = (size_t)G * N * sizeof(cd);
+  const size_t b_part = (size_t)nsplit * G * N * sizeof(cd);
+  const size_t b_row = (size_t)nsplit * G * N * sizeof(double), b_col = (size_t)G * N * sizeof(double);
+  rc = qb_dev_alloc(ctx, (void**)&zd, b_z);
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&Td, b_T);
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&phid, b_phi);
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&part, b_part);
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&rowabs, b_row);
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&colsum, b_col);
+  std::vector<cd> part_h;
+  std::vector<double> row_h, col_h;
+  for (int64_t q0 = 0; !rc && q0 < nl; q0 += G) {
+    const int64_t nb = std::min<int64_t>(G, nl - q0);
+    hipError_t e = hipMemcpyAsync(zd, lam + 2 * q0, nb * sizeof(cd), hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(phid, phi + 2 * q0 * N, (size_t)nb * N * sizeof(cd),
+                         hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) {
+      rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+      break;
+    }
+    DevTables t;
+    rc = qb_tables_build(ctx, (const double*)zd, nb, rmin, rmax, *cheb, true, &t);
+    if (!rc) rc = qb_assemble_dev(ctx, g, op, zd, nb, t, Td, nullptr);
+    if (!rc) rc = qb_gemv_norms(ctx, Td, phid, N, (int)nb, nsplit, part, rowabs, colsum);
+    part_h.resize((size_t)nsplit * nb * N);
+    row_h.resize((size_t)nsplit * nb * N);
+    col_h.resize((size_t)nb * N);
+    if (!rc) {
+      e = hipMemcpyAsync(part_h.data(), part, part_h.size() * sizeof(cd), hipMemcpyDeviceToHost,
+                         ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(row_h.data(), rowabs, row_h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(col_h.data(), colsum, col_h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+    } else {
+      (void)hipStreamSynchronize(ctx->stream);
+    }
+    qb_tables_free(ctx, &t);
+    if (rc) break;
+    for (int64_t b = 0; b < nb; ++b) {
+      double s2 = 0, s1 = 0, smx = 0, rmx = 0, cmx = 0;
+      for (int64_t i = 0; i < N; ++i) {
+        double yr = 0, yi = 0, ra = 0;
+        for (int s = 0; s < nsplit; ++s) {
+          const size_t o = ((size_t)s * nb + b) * N + i;
+          yr += part_h[o].re;
+          yi += part_h[o].im;
+          ra += row_h[o];
+        }
+        const double a = hypot(yr, yi);
+        s2 += a * a;
+        s1 += a;
+        smx = std::max(smx, a);
+        rmx = std::max(rmx, ra);
+        cmx = std::max(cmx, col_h[(size_t)b * N + i]);
+      }
+      if (res2) res2[q0 + b] = sqrt(s2);
+      if (resn1) resn1[q0 + b] = s1;
+      if (resninf) resninf[q0 + b] = smx;
+      if (opn1) opn1[q0 + b] = cmx;
+      if (opninf) opninf[q0 + b] = rmx;
+    }
+  }
+  qb_dev_free(ctx, colsum, b_col);
+  qb_dev_free(ctx, rowabs, b_row);
+  qb_dev_free(ctx, part, b_part);
+  qb_dev_free(ctx, phid, b_phi);
+  qb_dev_free(ctx, Td, b_T);
+  qb_dev_free(ctx, zd, b_z);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
 // Beyn window
 // ---------------------------------------------------------------------------
 struct qb_window {
@@ -569,10 +676,14 @@ extern "C" void qb_window_destroy(qb_window* w) {
   delete w;
 }
 
-#define QB_TRY(x)        \
-  do {                   \
-    int _rc = (x);       \
-    if (_rc) goto fail;  \
+// on error: keep the status in the caller's `rc` and jump to its cleanup
+#define QB_TRY(x)      \
+  do {                 \
+    int _rc = (x);     \
+    if (_rc) {         \
+      rc = _rc;        \
+      goto fail;       \
+    }                  \
   } while (0)
 
 extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
@@ -749,6 +860,7 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
   double t0 = now_ms();
   cd *Vd = nullptr, *X = nullptr, *Rr = nullptr, *A0d = nullptr, *A1d = nullptr;
   cf* D32 = nullptr;
+  int* actd = nullptr;
   int rc = QB_OK;
   int iters_max = 0;
   double resid_max = 0.0;
@@ -787,67 +899,94 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
     std::vector<char> stag(nq, 0);
     std::vector<int> stagnant;
     const int maxit = 8;
+    // Only the nodes still above the tolerance take part in an iteration:
+    // their indices go to the device (ctx->bix) and every batched launch of
+    // the iteration (gather, complex64 TRSM, update, FP64 residual GEMM,
+    // norms) covers just them.  Nodes far from the real axis converge in
+    // one or two steps; the few close to an eigenvalue need more.
+    std::vector<int> act(nq);
+    for (int j = 0; j < nq; ++j) act[j] = j;
+    QB_TRY(qb_dev_alloc(ctx, (void**)&actd, nq * sizeof(int)));
     // The last correction is usually predictable: once a node's backward
-    // error contracts by rho per step, be * rho^... is known.  When every
-    // unfinished node is predicted below tol/100 after the next correction,
-    // that correction is applied and checked on the first rs columns only
-    // (the probes are i.i.d., so a column subset measures the same backward
+    // error contracts by rho per step, be * rho is known.  When every active
+    // node is predicted below tol/100 after the next correction, that
+    // correction is applied and checked on the first rs columns only (the
+    // probes are i.i.d., so a column subset measures the same backward
     // error); a failed check falls back to the full residual.
     const int rs = r < 16 ? r : 16;
     const int64_t nvs = N * (int64_t)rs;
+    const bool trace = getenv("QB_REFINE_TRACE") != nullptr;
     int it = 0;
-    for (; it < maxit; ++it) {
+    for (; it < maxit && !act.empty(); ++it) {
+      const int na = (int)act.size();
+      if (hipMemcpyAsync(actd, act.data(), na * sizeof(int), hipMemcpyHostToDevice,
+                         ctx->stream) != hipSuccess) {
+        rc = qb_fail(ctx, QB_E_HIP, "node list upload failed");
+        goto fail;
+      }
+      ctx->bix = actd;
       // D = LU32^{-1} P R (complex64), X += D
-      QB_TRY(qb_gather_rows_cd2cf(ctx, Rr, nv, w->perm, D32, (int)N, r, nq));
-      QB_TRY(qb_trsm_lu_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, D32, r, N, nv, nq));
-      QB_TRY(qb_add_cf_to_cd(ctx, D32, X, (int64_t)nq * nv));
+      QB_TRY(qb_gather_rows_cd2cf(ctx, Rr, nv, w->perm, D32, (int)N, r, na));
+      QB_TRY(qb_trsm_lu_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, D32, r, N, nv, na));
+      QB_TRY(qb_add_cf_to_cd_nodes(ctx, D32, X, nv, nv, na));
       bool predicted = it >= 1;
-      for (int j = 0; j < nq && predicted; ++j)
-        if (!(be[j] <= refine_tol) && !stag[j] && !(be[j] * rho[j] <= 1e-2 * refine_tol))
-          predicted = false;
+      for (int j : act)
+        if (!(be[j] * rho[j] <= 1e-2 * refine_tol)) predicted = false;
+      bool subset_ok = false;
       for (int pass = predicted ? 0 : 1; pass < 2; ++pass) {
         // pass 0: residual of the first rs columns; pass 1: full residual
         const int rr = pass == 0 ? rs : r;
         const int64_t nvr = pass == 0 ? nvs : nv;
-        QB_TRY(qb_node_maxabs(ctx, X, nvr, nv, nq, w->scratch_d + nq));
-        QB_TRY(qb_bcast(ctx, Vd, Rr, nvr, nv, nq));
+        ctx->bix = actd;
+        QB_TRY(qb_node_maxabs(ctx, X, nvr, nv, na, w->scratch_d + nq, nq));
+        QB_TRY(qb_bcast(ctx, Vd, Rr, nvr, nv, na));
         QB_TRY(gemm_sub<cd>(ctx, (int)N, rr, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N,
-                            nv, nq));
-        QB_TRY(qb_node_maxabs(ctx, Rr, nvr, nv, nq, w->scratch_d + 2 * nq));
+                            nv, na));
+        QB_TRY(qb_node_maxabs(ctx, Rr, nvr, nv, na, w->scratch_d + 2 * nq, nq));
+        ctx->bix = nullptr;
         if (hipMemcpyAsync(h3.data(), w->scratch_d, 3 * nq * sizeof(double),
                            hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess) {
           rc = qb_fail(ctx, QB_E_HIP, "refinement norms failed");
           goto fail;
         }
-        std::vector<double> bnew(nq);
         bool ok = true;
-        for (int j = 0; j < nq; ++j) {
-          bnew[j] = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
-          if (!(bnew[j] <= refine_tol)) ok = false;
+        for (int j : act) {
+          double bn = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
+          if (!(bn <= refine_tol)) ok = false;
+          if (pass == 1) {
+            if (be[j] < 1e299) rho[j] = bn / be[j];
+            prev[j] = be[j];
+            be[j] = bn;
+          } else {
+            h3[j] = bn;  // slot j of the first third is free: keep the subset value
+          }
         }
         if (pass == 0) {
           if (ok) {  // subset verified: done
-            be = bnew;
+            for (int j : act) be[j] = h3[j];
+            subset_ok = true;
             break;
           }
           continue;  // subset failed: full residual
         }
-        for (int j = 0; j < nq; ++j) {
-          if (be[j] < 1e299) rho[j] = bnew[j] / be[j];
-          prev[j] = be[j];
-          be[j] = bnew[j];
-        }
       }
-      bool active = false;
-      for (int j = 0; j < nq; ++j) {
+      ctx->bix = nullptr;
+      if (trace) {
+        double bmax = 0;
+        for (int j : act) bmax = std::max(bmax, be[j]);
+        fprintf(stderr, "[qb refine] it %d: %d active nodes, %s, max backward error %.3e\n", it,
+                na, subset_ok ? "subset check" : "full residual", bmax);
+      }
+      std::vector<int> next;
+      for (int j : act) {
         if (be[j] <= refine_tol) continue;
         if (it >= 1 && !(be[j] < 0.5 * prev[j])) stag[j] = 1;
-        if (!stag[j]) active = true;
+        if (!stag[j]) next.push_back(j);
       }
-      if (!active) break;
+      act.swap(next);
     }
-    iters_max = it + 1 > maxit ? maxit : it + 1;
+    iters_max = it;
     // nodes that did not reach the tolerance: FP64 refactorisation and solve
     for (int j = 0; j < nq; ++j)
       if (!(be[j] <= refine_tol)) stagnant.push_back(j);
@@ -882,7 +1021,9 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
   w->st.t_solve_ms = now_ms() - t0;
   if (stats) *stats = w->st;
 fail:
+  ctx->bix = nullptr;
   (void)hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, actd, nq * sizeof(int));
   qb_dev_free(ctx, Vd, nv * sizeof(cd));
   qb_dev_free(ctx, X, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));

@@ -46,6 +46,10 @@ struct qb_ctx {
   std::string err;
   int64_t bytes = 0;  // device bytes held by objects of this context
   int gemm_pick[2][QB_GEMM_CLASSES];  // autotuned tile variant per shape class (-1: default)
+  // Node list (device array) for batched launches: when set, batch entry z
+  // of a GEMM / TRSM / gather / node-wise helper works on matrix bix[z]
+  // (strides unchanged), so a subset of a window's nodes runs in one launch.
+  const int* bix = nullptr;
   // caching allocator: freed blocks are kept (by exact size) for reuse
   std::vector<std::pair<size_t, void*>> cache;
   int64_t cache_bytes = 0;

@@ -22,14 +22,17 @@ int qb_gather_rows_cd(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm
 int qb_gather_rows_cd2cf(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm, cf* dst,
                          int n, int nrhs, int batch);
 int qb_add_cf_to_cd(qb_ctx* ctx, const cf* d, cd* x, int64_t n);
+// x[b*ns + i] += d[b*ns + i], i < n, for the nb batch entries (ctx->bix aware)
+int qb_add_cf_to_cd_nodes(qb_ctx* ctx, const cf* d, cd* x, int64_t n, int64_t ns, int nb);
 int qb_cd_to_cf(qb_ctx* ctx, const cd* x, cf* y, int64_t n);
 // R[b*ns + e] = V[e] for e < nv, b < nb
 int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int64_t ns, int nb);
 int qb_moments(qb_ctx* ctx, const cd* X, int64_t nv, int nq, const cd* w, const cd* wz, cd* A0,
                cd* A1);
-// out[b] = max_{e < n_per} |X[b*stride + e]|
+// out[b] = max_{e < n_per} |X[b*stride + e]| for the physical node b of each
+// of the nb batch entries (ctx->bix aware); out[0..max(n_out, nb)) cleared first
 int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int64_t stride, int nb,
-                   double* out_dev);
+                   double* out_dev, int n_out);
 int qb_node_maxabs32(qb_ctx* ctx, const cf* X, int64_t n_per, int nb, double* out_dev);
 int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev);
 
@@ -39,3 +42,10 @@ int qb_gemm_class(int M, int N, int K);
 template <typename T>
 int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflops);
 const char* qb_gemm_variant_name(int ty, int v);
+
+// Residual check (qb_residual.hip): for nb matrices T (N x N, stride N*N) and
+// vectors phi (stride N), partial products part[s][b][i] and partial row
+// sums rowabs[s][b][i] of |T| over nsplit column chunks, and column sums
+// colsum[b][j] of |T|.
+int qb_gemv_norms(qb_ctx* ctx, const cd* T, const cd* phi, int64_t N, int nb, int nsplit,
+                  cd* part, double* rowabs, double* colsum);

@@ -69,7 +69,7 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
                                                   int64_t lda, int64_t sA,
                                                   const T* __restrict__ B, int64_t ldb,
                                                   int64_t sB, T* __restrict__ C, int64_t ldc,
-                                                  int64_t sC) {
+                                                  int64_t sC, const int* __restrict__ bix) {
   typedef typename Scal<T>::S S;
   typedef typename Scal<T>::V4 V4;
   static_assert(WGM * WGN == 4, "4 waves");
@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
   const int lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WGM) * WM, wn0 = (wave / WGM) * WN;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-  const int64_t b = blockIdx.z;
+  const int64_t b = bix ? bix[blockIdx.z] : blockIdx.z;
   A += b * sA;
   B += b * sB;
   C += b * sC;
@@ -203,7 +203,7 @@ static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t ld
                         int batch) {
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
   hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN>), g, dim3(256), 0, ctx->stream, M, N, K,
-                     A, lda, sA, B, ldb, sB, C, ldc, sC);
+                     A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
 // Tile variants and the per-context choice.  Variant 0/1 are the defaults
@@ -329,36 +329,107 @@ const char* qb_gemm_variant_name(int ty, int v) {
 
 // ---------------------------------------------------------------------------
 // Leaf panel factorisation: columns [c0, c0+w) of rows [c0, n), w <= 32.
-// One workgroup per matrix.  Partial pivoting on max |re|+|im| (LAPACK's
-// izamax measure); a zero pivot leaves the column unscaled and counts in
-// info, the behaviour of zgetf2 behind lu!(check=false).
+// One workgroup per matrix, left-looking (Crout) order inside the leaf:
+// step c first forms the U part of column c (rows c0..c0+c-1) by forward
+// substitution with the unit-lower top block kept in LDS, then updates the
+// rest of column c with all previous L columns at once (only loads until the
+// single store per row, so a thread keeps many loads in flight), picks the
+// pivot on max |re|+|im| (LAPACK's izamax measure), swaps the pivot row into
+// place across the w columns and scales the column.  Each step reads c+1
+// columns and writes two, against ~2(w-c) read-modify-written columns in the
+// right-looking order.  A zero pivot leaves the column unscaled and counts
+// in info, the behaviour of zgetf2 behind lu!(check=false).
 // ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T cmul_(T a, T b) {
+  return T{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, int64_t sA,
                                                 int n, int c0, int w, int* __restrict__ ipiv,
                                                 int64_t sP, int* __restrict__ info) {
   typedef typename Scal<T>::S S;
-  __shared__ S red_v[32];
-  __shared__ int red_i[32];
+  __shared__ T Lt[32][33];  // Lt[j][k] = L(c0 + j, c0 + k), k < j
+  __shared__ T u[32];       // U(c0 + k, col), k < c
   __shared__ T prow[32];
+  __shared__ S red_v[16];
+  __shared__ int red_i[16];
   __shared__ int s_piv;
   const int64_t b = blockIdx.x;
   A += b * sA;
   ipiv += b * sP;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int lane = tid & 63, wv = tid >> 6;
+  const T* L0 = A + (int64_t)c0 * lda;  // first panel column
   for (int c = 0; c < w; ++c) {
     const int col = c0 + c;
     T* Ac = A + (int64_t)col * lda;
-    // pivot search over rows col..n-1
+    // (1) U(c0 + k, col) = L11^{-1} A(c0.., col), k < c, on wave 0
+    if (wv == 0) {
+      T a = (lane < c) ? Ac[c0 + lane] : T{0, 0};
+      for (int k = 0; k < c; ++k) {
+        T uk = T{__shfl(a.re, k), __shfl(a.im, k)};
+        if (lane > k && lane < c) {
+          T l = Lt[lane][k];
+          a.re -= l.re * uk.re - l.im * uk.im;
+          a.im -= l.re * uk.im + l.im * uk.re;
+        }
+      }
+      if (lane < c) {
+        u[lane] = a;
+        Ac[c0 + lane] = a;
+      }
+    }
+    __syncthreads();
+    // (2) rows col..n-1: A(r, col) -= sum_k L(r, c0 + k) U(c0 + k, col); pivot search
     S best = -1;
     int bi = n;
-    for (int r = col + tid; r < n; r += nt) {
-      T v = Ac[r];
-      S a = fabs(v.re) + fabs(v.im);
-      if (a > best) {
-        best = a;
+    for (int r = col + tid; r < n; r += 2 * nt) {
+      const int r2 = r + nt;
+      const bool two = r2 < n;
+      T a0 = Ac[r];
+      T a1 = two ? Ac[r2] : T{0, 0};
+      const T* Lr0 = L0 + r;
+      int k = 0;
+      for (; k + 4 <= c; k += 4) {
+        T l[4], m[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          l[q] = Lr0[(int64_t)(k + q) * lda];
+          m[q] = two ? Lr0[(int64_t)(k + q) * lda + nt] : T{0, 0};
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          T uk = u[k + q];
+          a0.re -= l[q].re * uk.re - l[q].im * uk.im;
+          a0.im -= l[q].re * uk.im + l[q].im * uk.re;
+          a1.re -= m[q].re * uk.re - m[q].im * uk.im;
+          a1.im -= m[q].re * uk.im + m[q].im * uk.re;
+        }
+      }
+      for (; k < c; ++k) {
+        T uk = u[k];
+        T l = Lr0[(int64_t)k * lda];
+        T m = two ? Lr0[(int64_t)k * lda + nt] : T{0, 0};
+        a0.re -= l.re * uk.re - l.im * uk.im;
+        a0.im -= l.re * uk.im + l.im * uk.re;
+        a1.re -= m.re * uk.re - m.im * uk.im;
+        a1.im -= m.re * uk.im + m.im * uk.re;
+      }
+      Ac[r] = a0;
+      S v0 = fabs(a0.re) + fabs(a0.im);
+      if (v0 > best) {
+        best = v0;
         bi = r;
+      }
+      if (two) {
+        Ac[r2] = a1;
+        S v1 = fabs(a1.re) + fabs(a1.im);
+        if (v1 > best) {
+          best = v1;
+          bi = r2;
+        }
       }
     }
     for (int off = 32; off >= 1; off >>= 1) {
@@ -388,43 +459,27 @@ __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, 
     }
     __syncthreads();
     const int piv = s_piv;
-    // swap rows col and piv across the w panel columns, keep the pivot row
+    // (3) swap rows col and piv across the w panel columns; the pivot row's
+    // L part becomes row c of the top block
     for (int cc = tid; cc < w; cc += nt) {
       T* Acc = A + (int64_t)(c0 + cc) * lda;
-      T u = Acc[piv];
+      T x = Acc[piv];
       if (piv != col) {
-        T d = Acc[col];
-        Acc[piv] = d;
-        Acc[col] = u;
+        Acc[piv] = Acc[col];
+        Acc[col] = x;
       }
-      prow[cc] = u;
+      prow[cc] = x;
+      if (cc < c) Lt[c][cc] = x;
     }
     __syncthreads();
-    T p = prow[c];
-    bool nz = (p.re != 0 || p.im != 0);
+    // (4) scale the column below the pivot
+    const T p = prow[c];
+    const bool nz = (p.re != 0 || p.im != 0);
     if (!nz && tid == 0 && info) atomicCAS(info + b, 0, col + 1);
-    // scale the column below the pivot
-    T inv = T{1, 0};
     if (nz) {
       S den = p.re * p.re + p.im * p.im;
-      inv = T{p.re / den, -p.im / den};
-    }
-    for (int r = col + 1 + tid; r < n; r += nt) {
-      T v = Ac[r];
-      if (nz) {
-        T s = T{v.re * inv.re - v.im * inv.im, v.re * inv.im + v.im * inv.re};
-        Ac[r] = s;
-        v = s;
-      }
-      // rank-1 update of the remaining panel columns of this row
-      for (int cc = c + 1; cc < w; ++cc) {
-        T* q = A + (int64_t)(c0 + cc) * lda + r;
-        T u = prow[cc];
-        T x = *q;
-        x.re -= v.re * u.re - v.im * u.im;
-        x.im -= v.re * u.im + v.im * u.re;
-        *q = x;
-      }
+      T inv = T{p.re / den, -p.im / den};
+      for (int r = col + 1 + tid; r < n; r += nt) Ac[r] = cmul_(Ac[r], inv);
     }
     __syncthreads();
   }
@@ -458,10 +513,11 @@ __global__ void k_laswp(T* __restrict__ A, int64_t lda, int64_t sA, int cb, int 
 template <typename T, bool UNIT_LOWER>
 __global__ void __launch_bounds__(64) k_trsm_leaf(int n, int nrhs, const T* __restrict__ L,
                                                   int64_t ldl, int64_t sL, T* __restrict__ B,
-                                                  int64_t ldb, int64_t sB) {
+                                                  int64_t ldb, int64_t sB,
+                                                  const int* __restrict__ bix) {
   __shared__ T Ls[32][33];
   __shared__ T Bt[64][33];
-  const int64_t b = blockIdx.y;
+  const int64_t b = bix ? bix[blockIdx.y] : blockIdx.y;
   L += b * sL;
   B += b * sB;
   const int tid = threadIdx.x;
@@ -580,7 +636,7 @@ static int trsm_lower(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int
     dim3 g((nrhs + 63) / 64, batch);
     int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
     hipLaunchKernelGGL((k_trsm_leaf<T, true>), g, dim3(64), 0, ctx->stream, n, nrhs, L, ldl, sL,
-                       B, ldb, sB);
+                       B, ldb, sB, ctx->bix);
     qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
@@ -603,7 +659,7 @@ static int trsm_upper(qb_ctx* ctx, int n, int nrhs, const T* U, int64_t ldu, int
     dim3 g((nrhs + 63) / 64, batch);
     int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
     hipLaunchKernelGGL((k_trsm_leaf<T, false>), g, dim3(64), 0, ctx->stream, n, nrhs, U, ldu,
-                       sU, B, ldb, sB);
+                       sU, B, ldb, sB, ctx->bix);
     qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
@@ -737,8 +793,9 @@ int qb_perm_from_ipiv(qb_ctx* ctx, const int* ipiv, int* perm, int n, int batch)
 
 template <typename TO>
 __global__ void k_gather_rows(const cd* __restrict__ src, int64_t s_src,
-                              const int* __restrict__ perm, TO* __restrict__ dst, int n, int nrhs) {
-  const int64_t b = blockIdx.z;
+                              const int* __restrict__ perm, TO* __restrict__ dst, int n, int nrhs,
+                              const int* __restrict__ bix) {
+  const int64_t b = bix ? bix[blockIdx.z] : blockIdx.z;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int c = blockIdx.y;
   if (i >= n) return;
@@ -755,7 +812,7 @@ int qb_gather_rows_cd(qb_ctx* ctx, const cd* src, int64_t s_src, const int* perm
                       int nrhs, int batch) {
   dim3 g((n + 255) / 256, nrhs, batch);
   hipLaunchKernelGGL(k_gather_rows<cd>, g, dim3(256), 0, ctx->stream, src, s_src, perm, dst, n,
-                     nrhs);
+                     nrhs, ctx->bix);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
@@ -763,7 +820,7 @@ int qb_gather_rows_cd2cf(qb_ctx* ctx, const cd* src, int64_t s_src, const int* p
                          int n, int nrhs, int batch) {
   dim3 g((n + 255) / 256, nrhs, batch);
   hipLaunchKernelGGL(k_gather_rows<cf>, g, dim3(256), 0, ctx->stream, src, s_src, perm, dst, n,
-                     nrhs);
+                     nrhs, ctx->bix);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
@@ -787,13 +844,29 @@ __global__ void k_add_cf_to_cd(const cf* __restrict__ d, cd* __restrict__ x, int
     x[i] = v;
   }
 }
+// X_b += D_b for the nodes of the batch (slot stride ns).
+__global__ void k_add_cf_to_cd_nodes(const cf* __restrict__ d, cd* __restrict__ x, int64_t n,
+                                     int64_t ns, const int* __restrict__ bix) {
+  const int64_t b = bix ? bix[blockIdx.y] : blockIdx.y;
+  d += b * ns;
+  x += b * ns;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    cd v = x[i];
+    cf e = d[i];
+    v.re += (double)e.re;
+    v.im += (double)e.im;
+    x[i] = v;
+  }
+}
 // Copy the first nv entries of V into every node's slot of R (slot stride ns).
 __global__ void k_bcast(const cd* __restrict__ V, cd* __restrict__ R, int64_t nv, int64_t ns,
-                        int nb) {
+                        int nb, const int* __restrict__ bix) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t st = (int64_t)gridDim.x * blockDim.x;
   for (; i < nv * nb; i += st) {
-    int64_t b = i / nv, e = i - b * nv;
+    int64_t z = i / nv, e = i - z * nv;
+    int64_t b = bix ? bix[z] : z;
     R[b * ns + e] = V[e];
   }
 }
@@ -813,8 +886,8 @@ __global__ void k_moments(const cd* __restrict__ X, int64_t nv, int nq, const cd
 }
 // Per-node max |.|_inf of n_per entries (|re|+|im| bound not used: true modulus).
 __global__ void k_node_maxabs(const cd* __restrict__ X, int64_t n_per, int64_t stride,
-                              double* __restrict__ out) {
-  const int64_t b = blockIdx.y;
+                              double* __restrict__ out, const int* __restrict__ bix) {
+  const int64_t b = bix ? bix[blockIdx.y] : blockIdx.y;
   X += b * stride;
   double m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_per;
@@ -865,8 +938,16 @@ int qb_add_cf_to_cd(qb_ctx* ctx, const cf* d, cd* x, int64_t n) {
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
+int qb_add_cf_to_cd_nodes(qb_ctx* ctx, const cf* d, cd* x, int64_t n, int64_t ns, int nb) {
+  int64_t g = (n + 255) / 256;
+  dim3 grid((unsigned)(g > 4096 ? 4096 : g), nb);
+  hipLaunchKernelGGL(k_add_cf_to_cd_nodes, grid, dim3(256), 0, ctx->stream, d, x, n, ns, ctx->bix);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
 int qb_bcast(qb_ctx* ctx, const cd* V, cd* R, int64_t nv, int64_t ns, int nb) {
-  hipLaunchKernelGGL(k_bcast, dim3(grid1(nv * nb)), dim3(256), 0, ctx->stream, V, R, nv, ns, nb);
+  hipLaunchKernelGGL(k_bcast, dim3(grid1(nv * nb)), dim3(256), 0, ctx->stream, V, R, nv, ns, nb,
+                     ctx->bix);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
@@ -878,11 +959,14 @@ int qb_moments(qb_ctx* ctx, const cd* X, int64_t nv, int nq, const cd* w, const 
   return QB_OK;
 }
 int qb_node_maxabs(qb_ctx* ctx, const cd* X, int64_t n_per, int64_t stride, int nb,
-                   double* out_dev) {
-  QB_HIP(ctx, hipMemsetAsync(out_dev, 0, nb * sizeof(double), ctx->stream));
+                   double* out_dev, int n_out) {
+  // out_dev[b] for the physical node b of every batch entry; all n_out
+  // entries are cleared first
+  QB_HIP(ctx, hipMemsetAsync(out_dev, 0, (n_out > nb ? n_out : nb) * sizeof(double), ctx->stream));
   int64_t g = (n_per + 255) / 256;
   dim3 grid((unsigned)(g > 1024 ? 1024 : g), nb);
-  hipLaunchKernelGGL(k_node_maxabs, grid, dim3(256), 0, ctx->stream, X, n_per, stride, out_dev);
+  hipLaunchKernelGGL(k_node_maxabs, grid, dim3(256), 0, ctx->stream, X, n_per, stride, out_dev,
+                     ctx->bix);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }

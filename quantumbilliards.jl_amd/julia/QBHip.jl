@@ -3,7 +3,8 @@
 #
 #     using QuantumBilliards
 #     include("QBHip.jl"); using .QBHip
-#     QBHip.enable!()          # route construct_boundary_matrices!/construct_B_matrix here
+#     QBHip.enable!()          # route construct_boundary_matrices!, construct_B_matrix and
+#                              # residual_and_norm_select here
 #
 # Not executed in this repository (there is no Julia toolchain here); the C ABI
 # it calls is exercised by the Python tests.
@@ -159,6 +160,53 @@ function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float6
     end
 end
 
+# residual_and_norm_select (beyn.jl:557-593): T(λ_j) assembled and applied to
+# φ_j = U_k Y[:, j] on the device; only the norms come back
+function residual_and_norm_select(solver, λ::AbstractVector{ComplexF64},
+                                  Uk::AbstractMatrix{ComplexF64}, Y::AbstractMatrix{ComplexF64},
+                                  k0::ComplexF64, R::Float64, pts; res_tol::Float64,
+                                  matnorm::Symbol=:one, epss::Real=1e-15,
+                                  auto_discard_spurious::Bool=true, collect_logs::Bool=false,
+                                  use_chebyshev::Bool=true, n_panels_h::Int=15000, M_h::Int=5,
+                                  n_panels_j::Int=3000, M_j::Int=5, multithreaded::Bool=true)
+    matnorm in (:one, :inf) || error("matnorm :two is not provided by libqbhip")
+    N, rk = size(Uk)
+    Φ = Uk * Y
+    inside = findall(j -> abs(λ[j] - k0) <= R, eachindex(λ))
+    tens = fill(NaN, rk); tensN = fill(NaN, rk); keep = falses(rk)
+    logs = String[]
+    if !isempty(inside)
+        nl = length(inside)
+        r2 = zeros(nl); r1 = zeros(nl); rinf = zeros(nl); n1 = zeros(nl); ninf = zeros(nl)
+        g = geom(pts)
+        try
+            cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
+            check(ccall((:qb_residuals, LIB), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Int64, Ref{ChebParams},
+                         Ptr{ComplexF64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                         Ptr{Float64}, Ptr{Float64}),
+                        ctx(), g, opcode(solver), λ[inside], Int64(nl), cp, Φ[:, inside],
+                        Int32(0), r2, r1, rinf, n1, ninf))
+        finally
+            geom_destroy(g)
+        end
+        for (q, j) in pairs(inside)
+            nA, φn, yn = matnorm === :one ? (n1[q], norm(Φ[:, j], 1), r1[q]) :
+                                           (ninf[q], norm(Φ[:, j], Inf), rinf[q])
+            tens[j] = r2[q]
+            tensN[j] = yn / (nA * (φn + epss) + epss)
+            if auto_discard_spurious && r2[q] >= res_tol
+                collect_logs && push!(logs, "λ=$(λ[j]) ||Aφ||=$(r2[q]) > $res_tol → DROP")
+            else
+                keep[j] = true
+                collect_logs && push!(logs, "λ=$(λ[j]) ||Aφ||=$(r2[q]) < $res_tol ← KEEP")
+            end
+        end
+    end
+    idx = findall(keep)
+    return idx, Φ[:, idx], tens[idx], tensN[idx], logs
+end
+
 # Route the reference's entry points here for the Kress solvers.
 function enable!()
     @eval QB function construct_boundary_matrices!(Tbufs::Vector{Matrix{ComplexF64}},
@@ -169,6 +217,13 @@ function enable!()
     @eval QB function construct_B_matrix(solver::Union{CFIE_kress,DLP_kress}, pts, N::Int,
                                          k0::Complex{Float64}, R::Float64; kw...)
         return $(QBHip).construct_B_matrix(solver, pts, N, k0, R; kw...)
+    end
+    @eval QB function residual_and_norm_select(solver::Union{CFIE_kress,DLP_kress},
+                                               λ::AbstractVector{ComplexF64},
+                                               Uk::AbstractMatrix{ComplexF64},
+                                               Y::AbstractMatrix{ComplexF64}, k0::ComplexF64,
+                                               R::Float64, pts; kw...)
+        return $(QBHip).residual_and_norm_select(solver, λ, Uk, Y, k0, R, pts; kw...)
     end
     return nothing
 end

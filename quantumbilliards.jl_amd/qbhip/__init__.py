@@ -14,3 +14,6 @@ from .solvers import (CFIE_kress, DLP_kress, Geometry, assemble, bessel_eval,
                       boundary_matrix_size, chebyshev_params, construct_boundary_matrices)
 from .beyn import (B_from_moments, Window, construct_B_matrix, footprint, probe_block, probe_count, solve,
                    solve_vect, weyl_count)
+from .spectrum import (beyn_disks_from_windows, compute_spectrum_beyn, delta_weyl,
+                       imag_k_check_EXPERIMENTAL, plan_weyl_windows, residual_and_norm_select,
+                       residual_norms)

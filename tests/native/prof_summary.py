@@ -1,0 +1,92 @@
+"""Condense rocprofv3 CSV output into the short text summaries kept under
+profiles/.
+
+  prof_summary.py stats <dir>/<prefix>          kernel-trace run of bench.py
+  prof_summary.py pmc <fetch_csv> <write_csv>   FETCH_SIZE / WRITE_SIZE runs
+
+`stats` prints the whole-run per-kernel table and, from the trace, the last
+Beyn window alone (the dispatches from the last table build on), which is the
+window bench.py times.  `pmc` scales each counter by what it reads for the
+plain copy kernel k_copy16 (known bytes, 1 GiB read + 1 GiB written) and
+prints HBM bytes per dispatch beside the algorithmic bytes of the GEMM
+shapes the probe launches.
+"""
+import collections
+import csv
+import sys
+
+
+def _short(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def stats(prefix):
+    rows = list(csv.DictReader(open(prefix + "_kernel_stats.csv")))
+    print("whole run (rocprofv3 --stats):")
+    print(f"{'total ms':>10} {'calls':>7} {'avg ms':>9}  kernel")
+    for r in rows[:24]:
+        print(f"{int(r['TotalDurationNs']) / 1e6:10.3f} {int(r['Calls']):7d} "
+              f"{float(r['AverageNs']) / 1e6:9.4f}  {_short(r['Name'])}")
+    tr = list(csv.DictReader(open(prefix + "_kernel_trace.csv")))
+    start = max(i for i, r in enumerate(tr) if "k_build_tables<0>" in r["Kernel_Name"])
+    sel = tr[start:]
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    for r in sel:
+        d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+        a = agg[_short(r["Kernel_Name"])]
+        a[0] += 1
+        a[1] += d
+    span = (int(sel[-1]["End_Timestamp"]) - int(sel[0]["Start_Timestamp"])) / 1e6
+    busy = sum(v[1] for v in agg.values())
+    print(f"\nlast window (trace, dispatches {start}..{len(tr) - 1}): span {span:.1f} ms, "
+          f"kernel busy {busy:.1f} ms (two streams overlap)")
+    print(f"{'total ms':>10} {'calls':>7} {'avg ms':>9}  kernel")
+    for k, v in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"{v[1]:10.3f} {v[0]:7d} {v[1] / v[0]:9.4f}  {k}")
+    c64 = [v for k, v in agg.items() if k.startswith("k_gemm_sub<cf")]
+    n = sum(v[0] for v in c64)
+    ms = sum(v[1] for v in c64)
+    print(f"\ncomplex64 GEMM (all tile variants) in the last window: {n} launches, {ms:.2f} ms, "
+          f"average {ms / max(n, 1):.4f} ms per launch")
+
+
+def pmc(fetch_csv, write_csv):
+    val = {}
+    for path, cname in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == cname]
+        val[cname] = [(_short(r["Kernel_Name"]), float(r["Counter_Value"]),
+                       (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6) for r in rows]
+    copy_bytes = float(1 << 30)
+    f_copy = [v for k, v, _ in val["FETCH_SIZE"] if k == "k_copy16"][0] * 1024
+    w_copy = [v for k, v, _ in val["WRITE_SIZE"] if k == "k_copy16"][0] * 1024
+    sf, sw = copy_bytes / f_copy, copy_bytes / w_copy
+    print(f"copy kernel k_copy16, 1 GiB read + 1 GiB written: FETCH_SIZE {f_copy / 2**30:.4f} GiB, "
+          f"WRITE_SIZE {w_copy / 2**30:.4f} GiB")
+    print(f"scale factors: FETCH x{sf:.3f}, WRITE x{sw:.3f}\n")
+    shapes = {"k_gemm_sub<cf, 128, 128, 16, 2, 2>": (7936, 7936, 256, 4, 8),
+              "k_gemm_sub<cd, 128, 64, 16, 2, 2>": None}
+    print(f"{'kernel':44} {'ms':>8} {'fetch GB':>9} {'write GB':>9} {'HBM GB':>8} {'alg GB':>8} "
+          f"{'GB/s':>7}")
+    seen = collections.Counter()
+    for (k, fv, ms), (k2, wv, _) in zip(val["FETCH_SIZE"], val["WRITE_SIZE"]):
+        if not k.startswith("k_gemm_sub"):
+            continue
+        seen[k] += 1
+        fb, wb = fv * 1024 * sf, wv * 1024 * sw
+        alg = None
+        if k in shapes and shapes[k]:
+            M, N, K, b, es = shapes[k]
+            alg = b * es * (M * K + K * N + 2 * M * N)
+        elif k.startswith("k_gemm_sub<cd"):
+            # probe order: two trailing-update launches, then two residual launches
+            M, N, K, b, es = ((7936, 7936, 256, 4, 16) if seen[k] <= 2 else (8192, 768, 8192, 4, 16))
+            alg = b * es * (M * K + K * N + 2 * M * N)
+        print(f"{k:44} {ms:8.3f} {fb / 1e9:9.3f} {wb / 1e9:9.3f} {(fb + wb) / 1e9:8.3f} "
+              f"{(alg or 0) / 1e9:8.3f} {(fb + wb) / 1e6 / ms:7.0f}")
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        stats(sys.argv[2])
+    else:
+        pmc(sys.argv[2], sys.argv[3])

@@ -49,7 +49,7 @@ def test_gemm_sub(probe, dt, tol, M, N, K, b):
 
 
 @pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-12), (np.complex64, 1e-4)])
-@pytest.mark.parametrize("n,b", [(8, 1), (40, 2), (160, 2), (300, 1)])
+@pytest.mark.parametrize("n,b", [(8, 1), (40, 2), (160, 2), (300, 1), (2200, 2)])
 def test_getrf_and_solve(probe, dt, tol, n, b):
     L, ctx = probe
     rng = np.random.default_rng(n)
@@ -71,6 +71,11 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         err = np.max(np.abs(Ti[perm] - Lm @ Um)) / np.max(np.abs(Ti))
         print("getrf", dt.__name__, n, i, err)
         assert err < tol
+        # partial pivoting on |re| + |im| (izamax): below the diagonal, the
+        # unscaled entries l_ik u_kk never exceed the pivot u_kk in that measure
+        a1 = lambda x: np.abs(x.real) + np.abs(x.imag)
+        lm = np.tril(LU, -1) * np.diag(LU)[None, :]
+        assert np.all(a1(lm) <= a1(np.diag(LU))[None, :] * (1 + 1e-5))
     # triangular solves with the device factors
     nr = 7
     Bm = _rand(rng, (b, nr, n), dt)
