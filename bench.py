@@ -25,6 +25,10 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "quantumbilliards.jl_amd")]
 import numpy as np  # noqa: E402
 
 PEAK = {"c64": 157.3, "c128": 78.6}  # dense MFMA TFLOP/s (fp32, fp64)
+# HBM bytes per complex64 GEMM launch over one window of the default workload,
+# from FETCH_SIZE / WRITE_SIZE counter runs scaled by a copy kernel of known
+# size (profiles/beyn_window_star_N8192_mixed_pmc_traffic.txt)
+TRAFFIC_C64_PER_LAUNCH = 699.9e6
 
 
 def parse():
@@ -225,6 +229,7 @@ def main():
     g = ks["gemm_c64"]
     avg_ms = g["ms"] / max(g["launches"], 1)
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    default_workload = (a.N, a.nq, a.k0, a.R, r_used) == (8192, 48, 800.0, 0.5, 768)
     mean = t_span / a.steps
     value = float(np.sum(counts) / t_span)
     out = {
@@ -246,7 +251,10 @@ def main():
         "gemm_autotune": [l for l in rep.splitlines() if l],
         "roofline": {"kernel": "k_gemm_sub<complex64> (LU trailing update, MFMA)", "bound": "mfma",
                      "achieved": achieved, "peak": PEAK["c64"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK["c64"], "avg_launch_ms": avg_ms, "traffic": None},
+                     "frac": achieved / PEAK["c64"], "avg_launch_ms": avg_ms,
+                     "traffic": TRAFFIC_C64_PER_LAUNCH if default_workload else None,
+                     "alg_bytes_per_launch": g["bytes"] / max(g["launches"], 1),
+                     "flops_per_launch": g["flops"] / max(g["launches"], 1)},
     }
     if not a.no_cpu:
         try:

@@ -3,6 +3,7 @@ profiles/.
 
   prof_summary.py stats <dir>/<prefix>          kernel-trace run of bench.py
   prof_summary.py pmc <fetch_csv> <write_csv>   FETCH_SIZE / WRITE_SIZE runs
+  prof_summary.py pmcwin <fetch_csv> <write_csv>  the same over one Beyn window
 
 `stats` prints the whole-run per-kernel table and, from the trace, the last
 Beyn window alone (the dispatches from the last table build on), which is the
@@ -85,8 +86,38 @@ def pmc(fetch_csv, write_csv):
               f"{(alg or 0) / 1e9:8.3f} {(fb + wb) / 1e6 / ms:7.0f}")
 
 
+def pmcwin(fetch_csv, write_csv):
+    """Window run of pmc_probe.py: a 1 GiB k_copy16 (scale), a 1 MiB k_copy16
+    (marker), then one mixed-precision window.  Per kernel family: launches,
+    summed HBM bytes (scaled counters) and bytes per launch."""
+    per = {}
+    for path, cname in ((fetch_csv, "FETCH_SIZE"), (write_csv, "WRITE_SIZE")):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == cname]
+        copies = [i for i, r in enumerate(rows) if _short(r["Kernel_Name"]) == "k_copy16"]
+        big, mark = copies[0], copies[1]
+        scale = float(1 << 30) / (float(rows[big]["Counter_Value"]) * 1024)
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for r in rows[mark + 1:]:
+            k = _short(r["Kernel_Name"])
+            k = "k_gemm_sub<cf> (all tiles)" if k.startswith("k_gemm_sub<cf") else \
+                "k_gemm_sub<cd> (all tiles)" if k.startswith("k_gemm_sub<cd") else k
+            agg[k][0] += 1
+            agg[k][1] += float(r["Counter_Value"]) * 1024 * scale
+        per[cname] = (scale, agg)
+    sf, af = per["FETCH_SIZE"]
+    sw, aw = per["WRITE_SIZE"]
+    print(f"scale factors from the 1 GiB copy: FETCH x{sf:.3f}, WRITE x{sw:.3f}")
+    print(f"{'launches':>8} {'fetch GB':>9} {'write GB':>9} {'MB/launch':>10}  kernel")
+    for k in sorted(af, key=lambda k: -(af[k][1] + aw.get(k, [0, 0])[1])):
+        n = af[k][0]
+        f, w = af[k][1], aw.get(k, [0, 0.0])[1]
+        print(f"{n:8d} {f / 1e9:9.3f} {w / 1e9:9.3f} {(f + w) / 1e6 / max(n, 1):10.3f}  {k}")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2])
+    elif sys.argv[1] == "pmcwin":
+        pmcwin(sys.argv[2], sys.argv[3])
     else:
         pmc(sys.argv[2], sys.argv[3])
