@@ -143,9 +143,19 @@ def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
     res = [None] * steps
 
     def host_stage():
+        # LAPACK on all but two of this process's cores: one core keeps
+        # driving the device (launches, stream waits) for the next window
+        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 4
+        try:
+            from threadpoolctl import threadpool_limits
+            lim = threadpool_limits(limits=max(1, ncpu - 2), user_api="blas")
+        except Exception:
+            lim = None
         while True:
             item = qu.get()
             if item is None:
+                if lim is not None:
+                    lim.restore_original_limits()
                 return
             s, A0, A1, t_start, t_dev, st = item
             th = time.perf_counter()

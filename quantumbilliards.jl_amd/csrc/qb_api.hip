@@ -906,16 +906,40 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
     // one or two steps; the few close to an eigenvalue need more.
     std::vector<int> act(nq);
     for (int j = 0; j < nq; ++j) act[j] = j;
-    QB_TRY(qb_dev_alloc(ctx, (void**)&actd, nq * sizeof(int)));
-    // The last correction is usually predictable: once a node's backward
-    // error contracts by rho per step, be * rho is known.  When every active
-    // node is predicted below tol/100 after the next correction, that
-    // correction is applied and checked on the first rs columns only (the
-    // probes are i.i.d., so a column subset measures the same backward
-    // error); a failed check falls back to the full residual.
+    QB_TRY(qb_dev_alloc(ctx, (void**)&actd, 2 * nq * sizeof(int)));
+    // The last correction of a node is usually predictable: once its
+    // backward error contracts by rho per step, be * rho is known.  A node
+    // predicted below tol/10 after the current correction is checked on the
+    // first rs columns only (the probes are i.i.d., so a column subset
+    // measures the same backward error); it must come out below tol/4 there,
+    // otherwise it joins the full residual of the unpredicted nodes.
     const int rs = r < 16 ? r : 16;
-    const int64_t nvs = N * (int64_t)rs;
     const bool trace = getenv("QB_REFINE_TRACE") != nullptr;
+    std::vector<int> P, F;
+    // residual pass over the nodes of `list` (uploaded to slot `slot` of
+    // actd) on the first `rr` columns; bn[j] = backward error per node
+    std::vector<double> bn(nq, 0.0);
+    auto residual_pass = [&](const std::vector<int>& list, int slot, int rr) -> int {
+      const int n = (int)list.size();
+      const int64_t nvr = N * (int64_t)rr;
+      if (hipMemcpyAsync(actd + slot * nq, list.data(), n * sizeof(int), hipMemcpyHostToDevice,
+                         ctx->stream) != hipSuccess)
+        return qb_fail(ctx, QB_E_HIP, "node list upload failed");
+      ctx->bix = actd + slot * nq;
+      int e = qb_node_maxabs(ctx, X, nvr, nv, n, w->scratch_d + nq, nq);
+      if (!e) e = qb_bcast(ctx, Vd, Rr, nvr, nv, n);
+      if (!e)
+        e = gemm_sub<cd>(ctx, (int)N, rr, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N, nv, n);
+      if (!e) e = qb_node_maxabs(ctx, Rr, nvr, nv, n, w->scratch_d + 2 * nq, nq);
+      ctx->bix = nullptr;
+      if (e) return e;
+      if (hipMemcpyAsync(h3.data(), w->scratch_d, 3 * nq * sizeof(double),
+                         hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess)
+        return qb_fail(ctx, QB_E_HIP, "refinement norms failed");
+      for (int j : list) bn[j] = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
+      return QB_OK;
+    };
     int it = 0;
     for (; it < maxit && !act.empty(); ++it) {
       const int na = (int)act.size();
@@ -925,58 +949,40 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
         goto fail;
       }
       ctx->bix = actd;
-      // D = LU32^{-1} P R (complex64), X += D
+      // D = LU32^{-1} P R (complex64), X += D on the active nodes
       QB_TRY(qb_gather_rows_cd2cf(ctx, Rr, nv, w->perm, D32, (int)N, r, na));
       QB_TRY(qb_trsm_lu_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, D32, r, N, nv, na));
       QB_TRY(qb_add_cf_to_cd_nodes(ctx, D32, X, nv, nv, na));
-      bool predicted = it >= 1;
+      ctx->bix = nullptr;
+      P.clear();
+      F.clear();
       for (int j : act)
-        if (!(be[j] * rho[j] <= 1e-2 * refine_tol)) predicted = false;
-      bool subset_ok = false;
-      for (int pass = predicted ? 0 : 1; pass < 2; ++pass) {
-        // pass 0: residual of the first rs columns; pass 1: full residual
-        const int rr = pass == 0 ? rs : r;
-        const int64_t nvr = pass == 0 ? nvs : nv;
-        ctx->bix = actd;
-        QB_TRY(qb_node_maxabs(ctx, X, nvr, nv, na, w->scratch_d + nq, nq));
-        QB_TRY(qb_bcast(ctx, Vd, Rr, nvr, nv, na));
-        QB_TRY(gemm_sub<cd>(ctx, (int)N, rr, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N,
-                            nv, na));
-        QB_TRY(qb_node_maxabs(ctx, Rr, nvr, nv, na, w->scratch_d + 2 * nq, nq));
-        ctx->bix = nullptr;
-        if (hipMemcpyAsync(h3.data(), w->scratch_d, 3 * nq * sizeof(double),
-                           hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess) {
-          rc = qb_fail(ctx, QB_E_HIP, "refinement norms failed");
-          goto fail;
-        }
-        bool ok = true;
-        for (int j : act) {
-          double bn = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
-          if (!(bn <= refine_tol)) ok = false;
-          if (pass == 1) {
-            if (be[j] < 1e299) rho[j] = bn / be[j];
-            prev[j] = be[j];
-            be[j] = bn;
-          } else {
-            h3[j] = bn;  // slot j of the first third is free: keep the subset value
-          }
-        }
-        if (pass == 0) {
-          if (ok) {  // subset verified: done
-            for (int j : act) be[j] = h3[j];
-            subset_ok = true;
-            break;
-          }
-          continue;  // subset failed: full residual
+        (it >= 1 && be[j] * rho[j] <= 0.1 * refine_tol ? P : F).push_back(j);
+      const int n_sub = (int)P.size();
+      if (!P.empty()) {
+        QB_TRY(residual_pass(P, 1, rs));
+        for (int j : P) {
+          if (bn[j] <= 0.25 * refine_tol)
+            be[j] = bn[j];  // verified on the subset: converged
+          else
+            F.push_back(j);
         }
       }
-      ctx->bix = nullptr;
+      if (!F.empty()) {
+        QB_TRY(residual_pass(F, 0, r));
+        for (int j : F) {
+          if (be[j] < 1e299) rho[j] = bn[j] / be[j];
+          prev[j] = be[j];
+          be[j] = bn[j];
+        }
+      }
       if (trace) {
         double bmax = 0;
         for (int j : act) bmax = std::max(bmax, be[j]);
-        fprintf(stderr, "[qb refine] it %d: %d active nodes, %s, max backward error %.3e\n", it,
-                na, subset_ok ? "subset check" : "full residual", bmax);
+        fprintf(stderr,
+                "[qb refine] it %d: %d active nodes, %d full residuals, %d subset checks, max "
+                "backward error %.3e\n",
+                it, na, (int)F.size(), n_sub, bmax);
       }
       std::vector<int> next;
       for (int j : act) {
@@ -1023,7 +1029,7 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
 fail:
   ctx->bix = nullptr;
   (void)hipStreamSynchronize(ctx->stream);
-  qb_dev_free(ctx, actd, nq * sizeof(int));
+  qb_dev_free(ctx, actd, 2 * nq * sizeof(int));
   qb_dev_free(ctx, Vd, nv * sizeof(cd));
   qb_dev_free(ctx, X, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));

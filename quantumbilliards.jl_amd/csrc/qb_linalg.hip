@@ -64,7 +64,7 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
 // coalesced global loads and the fragment reads are conflict-free.  A is
 // negated when it is written to LDS, so the accumulators hold -A*B and the
 // epilogue adds them to C.
-template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD>
 __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
                                                   int64_t lda, int64_t sA,
                                                   const T* __restrict__ B, int64_t ldb,
@@ -87,7 +87,25 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave % WGM) * WM, wn0 = (wave / WGM) * WN;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  int mt = blockIdx.x, ntile = blockIdx.y;
+  if constexpr (XCD) {
+    // Workgroups are dispatched round-robin over the 8 XCDs (each with its
+    // own L2).  Give XCD x a contiguous range of tile ids, and order tile
+    // ids in groups of 8 m-tiles sweeping n, so the workgroups resident on
+    // one XCD share A and B panels in its L2.  Only when the tile count is a
+    // multiple of 8 (every batch entry then starts on XCD 0).
+    const unsigned gx = gridDim.x, gy = gridDim.y, ntiles = gx * gy;
+    if ((ntiles & 7u) == 0) {
+      const unsigned L = blockIdx.x + gx * blockIdx.y;
+      const unsigned t = (L & 7u) * (ntiles >> 3) + (L >> 3);
+      const unsigned grp = t / (8u * gy), first = grp * 8u;
+      const unsigned gm = gx - first < 8u ? gx - first : 8u;
+      const unsigned loc = t - first * gy;
+      mt = (int)(first + loc % gm);
+      ntile = (int)(loc / gm);
+    }
+  }
+  const int m0 = mt * BM, n0 = ntile * BN;
   const int64_t b = bix ? bix[blockIdx.z] : blockIdx.z;
   A += b * sA;
   B += b * sB;
@@ -197,12 +215,12 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
     }
 }
 
-template <typename T, int BM, int BN, int BK, int WGM, int WGN>
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD = false>
 static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
                         const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                         int batch) {
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN>), g, dim3(256), 0, ctx->stream, M, N, K,
+  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD>), g, dim3(256), 0, ctx->stream, M, N, K,
                      A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
@@ -214,16 +232,20 @@ typedef void (*gemm_fn_c64)(qb_ctx*, int, int, int, const cf*, int64_t, int64_t,
 typedef void (*gemm_fn_c128)(qb_ctx*, int, int, int, const cd*, int64_t, int64_t, const cd*,
                              int64_t, int64_t, cd*, int64_t, int64_t, int);
 static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
-    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 256, 64, 16, 4, 1>,
-    launch_gemm<cf, 64, 256, 16, 1, 4>,  launch_gemm<cf, 128, 64, 16, 2, 2>,
-    launch_gemm<cf, 128, 128, 32, 2, 2>};
+    launch_gemm<cf, 128, 128, 16, 2, 2>,      launch_gemm<cf, 256, 64, 16, 4, 1>,
+    launch_gemm<cf, 64, 256, 16, 1, 4>,       launch_gemm<cf, 128, 64, 16, 2, 2>,
+    launch_gemm<cf, 128, 128, 32, 2, 2>,      launch_gemm<cf, 128, 64, 16, 2, 2, true>,
+    launch_gemm<cf, 128, 128, 16, 2, 2, true>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
-    launch_gemm<cd, 128, 64, 16, 2, 2>, launch_gemm<cd, 128, 32, 16, 4, 1>,
-    launch_gemm<cd, 64, 64, 16, 2, 2>,  launch_gemm<cd, 64, 128, 16, 2, 2>,
-    launch_gemm<cd, 256, 32, 16, 4, 1>};
+    launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
+    launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
+    launch_gemm<cd, 256, 32, 16, 4, 1>,       launch_gemm<cd, 64, 64, 16, 2, 2, true>,
+    launch_gemm<cd, 128, 64, 16, 2, 2, true>};
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
-    {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32"},
-    {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16"}};
+    {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
+     "128x128x16/xcd"},
+    {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
+     "128x64x16/xcd"}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);

@@ -92,17 +92,20 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(5))
+@pytest.mark.parametrize("variant", range(7))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
-    LU and triangular solves."""
+    LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
+    are multiples of 8 with a partial group of m-tiles, the case in which the
+    XCD-aware variants (5, 6) remap tiles."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
         for ty in (0, 1):
             L.qbt_force_variant(c, ty, variant)
         for dt, tol in ((np.complex128, 1e-13), (np.complex64, 3e-5)):
-            for (M, N, K, b) in [(100, 70, 33, 2), (300, 129, 128, 3), (64, 300, 40, 1)]:
+            for (M, N, K, b) in [(100, 70, 33, 2), (300, 129, 128, 3), (64, 300, 40, 1),
+                                 (1300, 1000, 40, 2)]:
                 test_gemm_sub(probe, dt, tol, M, N, K, b)
         test_getrf_and_solve(probe, np.complex128, 1e-12, 300, 1)
         test_getrf_and_solve(probe, np.complex64, 1e-4, 160, 2)
