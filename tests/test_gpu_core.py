@@ -140,3 +140,36 @@ def test_dlp_ellipse_full_size_sampled_columns():
         worst = max(worst, np.max(np.abs(G - F)) / s)
     print("cfg2 worst normwise", worst, (nh, mh, nj, mj))
     assert worst < 1e-12
+
+
+def test_device_matches_golden_fixtures():
+    """Device Bessel values and the device-assembled unit-circle operators
+    against the mpmath fixtures of tests/golden (values and Fourier
+    multipliers)."""
+    import json
+    import os
+    q = qb()
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    c = lambda v: complex(v[0], v[1])
+    rows = json.load(open(os.path.join(gold, "bessel_mpmath.json")))
+    z = np.array([c(r["z"]) for r in rows])
+    got = q.bessel_eval(z)
+    for k, g in zip(("h0", "h1", "j0", "j1"), got):
+        ref = np.array([c(r[k]) for r in rows])
+        assert np.max(np.abs(g - ref) / np.maximum(np.abs(ref), 1e-3)) < 3e-14, k
+    mult = json.load(open(os.path.join(gold, "circle_multipliers.json")))
+    b = q.make_circle(1.0)
+    N = 128
+    m_all = np.fft.fftfreq(N, 1.0 / N).astype(int)
+    cp = dict(n_panels_h=6000, M_h=7, n_panels_j=3000, M_j=7)
+    for key, ent in mult.items():
+        zz = c(ent["z"])
+        T = q.assemble(q.CFIE_kress(10.0, b), q.evaluate_points(q.CFIE_kress(10.0, b), b, 1.0,
+                                                                 N=[N]), [zz], **cp)[0]
+        F = q.assemble(q.DLP_kress(10.0, b), q.evaluate_points(q.DLP_kress(10.0, b), b, 1.0,
+                                                                N=[N]), [zz], **cp)[0]
+        lt, lf = np.fft.fft(np.asarray(T[:, 0])), np.fft.fft(np.asarray(F[:, 0]))
+        for e in ent["multipliers"]:
+            i = int(np.flatnonzero(m_all == e["m"])[0])
+            assert abs(lt[i] - c(e["cfie"])) < 1e-10, (key, e["m"])
+            assert abs(lf[i] - c(e["dlp"])) < 1e-10, (key, e["m"])

@@ -104,24 +104,25 @@ def test_imag_check_matches_full_select():
     assert np.all(np.abs(resid[0] - tens) <= 1e-13)
 
 
-def test_compute_spectrum_beyn_circle():
+@pytest.mark.parametrize("imag_check", [False, True])
+def test_compute_spectrum_beyn_circle(imag_check):
     """Weyl-planned sweep of [19.2, 20.8] on the unit circle (CFIE-Kress,
-    b = 10): every kept wavenumber is a zero of J_m, and every zero of J_m in
-    the interval is found with its multiplicity."""
+    b = 10), with the full residual check or the |Im lambda| filter: every
+    kept wavenumber is a zero of J_m, and every zero of J_m in the interval
+    is found with its multiplicity."""
     q = qb()
     b = q.make_circle(1.0)
     solver = q.CFIE_kress(10.0, b)
     ks, tens, us, pts, tensN = q.compute_spectrum_beyn(
         solver, b, 19.2, 20.8, m=10, Rmax=0.5, nq=32, r=40, svd_tol=1e-10, res_tol=1e-6,
-        cheb_tol=1e-12, use_imag_check_EXPERIMENTAL=False)
+        cheb_tol=1e-12, use_imag_check_EXPERIMENTAL=imag_check)
     ks = np.sort(ks)
-    ex = []
-    for m in range(60):
-        from scipy.special import jn_zeros
-        for z in jn_zeros(m, 20):
-            if 19.2 <= z <= 20.8:
-                ex += [z] * (1 if m == 0 else 2)
-    ex = np.sort(ex)
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                    "jm_zeros.json")))
+    assert g["interval"] == [19.2, 20.8]
+    ex = np.sort([e["k"] for e in g["interval_zeros"] for _ in range(e["mult"])])
     print(len(ks), len(ex), ks, ex)
     assert len(ks) == len(ex)
     assert np.max(np.abs(ks - ex) / ex) < 1e-10
