@@ -49,7 +49,41 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-cols", type=int, default=512)
+    ap.add_argument("--serve", action="store_true",
+                    help="serving stream (Poisson arrivals of mixed windows) instead")
+    ap.add_argument("--serve-requests", type=int, default=24)
+    ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second")
+    ap.add_argument("--serve-kmax", type=float, default=1000.0)
+    ap.add_argument("--serve-slots", type=int, default=2)
     return ap.parse_args()
+
+
+def run_serve(a):
+    """Serving workload: a seeded Poisson stream of window requests mixing the
+    ellipse (1, 0.6), the stadium (radius 1, half-length 1) and the star,
+    k0 log-uniform in [50, kmax], R in [0.25, 0.5], tol 1e-8 or 1e-12, N from
+    10 points per wavelength, 32 contour nodes, mixed precision; admission by
+    HBM footprint into `slots` device contexts.  One untimed warmup window
+    first.  Prints one JSON line with eigenvalues/s and p50/p99 latency."""
+    from qbhip import serve as S
+    reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax)
+    srv = S.Server(slots=a.serve_slots)
+    srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32)).result()
+    t0 = time.perf_counter()
+    res = srv.run_stream(reqs)
+    span = time.perf_counter() - t0
+    srv.close()
+    summ = S.summarize(res)
+    Ns = [r.N for r in res]
+    print(json.dumps({
+        "metric": "serving: eigenvalues/sec + p50/p99 window latency (Poisson stream of Beyn windows)",
+        "value": summ["eigenvalues_per_s"], "unit": "eigenvalues/s", "n_gpus": 1,
+        "p50_window_s": summ["p50_latency_s"], "p99_window_s": summ["p99_latency_s"],
+        "windows": summ["windows"], "eigenvalues": summ["eigenvalues"], "span_s": span,
+        "higher_is_better": True, "dtype": "c64+c128-refine", "data": "synthetic (seeded)",
+        "config": {"workload": "serving stream ellipse/stadium/star", "requests": len(reqs),
+                   "rate_per_s": a.serve_rate, "k0": [50.0, a.serve_kmax], "nq": 32,
+                   "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns)}}), flush=True)
 
 
 def match_count(lam, ref, k0, R, tol=1e-10):
@@ -186,6 +220,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding of contour nodes is a design note only)"}))
         sys.exit(2)
+    if a.serve:
+        run_serve(a)
+        return
     import qbhip as q
     ctx = q.default_context()
     b = q.make_star_polar_boundary(1.0, 0.2, 5)

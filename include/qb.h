@@ -39,7 +39,7 @@ enum { QB_CFIE_KRESS = 1, QB_DLP_KRESS = 2 };
 enum { QB_FP64 = 0, QB_MIXED = 1 };
 
 /* flags for qb_assemble */
-enum { QB_OUT_DEVICE = 1 };
+enum { QB_OUT_DEVICE = 1, QB_DERIV_REFERENCE_SIGN = 2 };
 
 /* Chebyshev table sizes: the n_panels_h/M_h/n_panels_j/M_j keywords of
  * construct_boundary_matrices! (chebyshev_euclidian_helmholtz_cfie_kress.jl:1431,
@@ -150,6 +150,19 @@ int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r)
  * halving is refactored and solved in FP64 (ignored for QB_FP64). */
 int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0,
                       double* A1, double refine_tol, qb_stats* stats);
+
+/* construct_boundary_matrices_with_derivatives!: T(z_q), dT/dk(z_q) and
+ * d2T/dk2(z_q) for the expanded boundary integral method, each in the layout
+ * of qb_assemble (CFIE: cheb-CFIE:1507-1530 with the entries of
+ * _all_k_nosymm_CFIE_chebyshev_deriv!, :881-1059; DLP: cheb-DLP:1126-1144
+ * with _construct_dlp_kress_matrices_derivatives_chebyshev!, :881-952, and
+ * its sign flip + identity, :1097-1108).  Between boundary components the
+ * default is the wavenumber derivative of the matrix; with
+ * QB_DERIV_REFERENCE_SIGN in flags the two double-layer derivative terms take
+ * the reference's opposite sign (cheb-CFIE:1031-1032). */
+int qb_assemble_deriv(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z, int64_t nz,
+                      const qb_cheb_params* cheb, double* out, double* dout, double* ddout,
+                      int32_t flags, double* rminmax);
 
 /* Residual check of candidate eigenpairs (the spurious-root filter): for
  * q < nl, with T_q = T(lambda_q) assembled on the device exactly as by

@@ -590,6 +590,163 @@ def dlp_assemble(comp: Comp, zs, cp: ChebParams = ChebParams(), tables=None):
     return out
 
 
+def _tables4(k, rmin, rmax, cp):
+    return (build_table("h", 0, k, rmin, rmax, cp.n_panels_h, cp.M_h),
+            build_table("h", 1, k, rmin, rmax, cp.n_panels_h, cp.M_h),
+            build_table("j", 0, k, 0.0, rmax, cp.n_panels_j, cp.M_j),
+            build_table("j", 1, k, 0.0, rmax, cp.n_panels_j, cp.M_j))
+
+
+def _h_cut(k, r, h0, h1):
+    """|kr| < 0.2: direct values (h0_h1_j0_j1_multi_ks_at_r!, bessels:1155-1160);
+    r == 0 (the diagonal, handled by its limit formulas) is left alone."""
+    z = k * r
+    dm = (np.abs(z) < H_CUT) & (r > 0)
+    if np.any(dm):
+        hh0, hh1 = _hankel_direct(z[dm])
+        h0[dm], h1[dm] = hh0, hh1
+    return h0, h1
+
+
+def cfie_assemble_deriv(comps, zs, cp: ChebParams = ChebParams(),
+                        reference_offblock_sign: bool = False):
+    """construct_boundary_matrices_with_derivatives! for CFIE_kress
+    (cheb-CFIE:1507-1530 -> _all_k_nosymm_CFIE_chebyshev_deriv!, :881-1059).
+    Returns (A, A', A'') each [nz, N, N], derivatives in the wavenumber.
+
+    Between components the reference writes the double-layer derivative
+    terms with a minus sign, dval1 = -(i/2) w_j inn k H0 and
+    dval2 = -(i/2) w_j inn (H0 - k r H1) (cheb-CFIE:1031-1032), while the
+    derivative of its own entry dval = w_j (inn/r)(i k/2) H1(kr) (:641) is
+    +(i/2) w_j inn k H0.  The default here is the derivative of the matrix;
+    reference_offblock_sign=True reproduces the reference's sign."""
+    zs = np.asarray(zs, dtype=np.complex128)
+    rmin, rmax = _rrange_blocks(comps)
+    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+    offs = np.concatenate([[0], np.cumsum([len(c.x) for c in comps])])
+    N = int(offs[-1])
+    out = [np.zeros((len(zs), N, N), np.complex128) for _ in range(3)]
+    caches = [geom_cache(c) for c in comps]
+    for q, k in enumerate(zs):
+        H0, H1, J0, J1 = _tables4(k, rmin, rmax, cp)
+        aL1, aL2, aM1, aM2, ik = -k * INV_TWO_PI, 0.5j * k, -INV_TWO_PI, 0.5j, 1j * k
+        for a, ca in enumerate(comps):
+            for b, cb in enumerate(comps):
+                ro, co = offs[a], offs[b]
+                na, nb = len(ca.x), len(cb.x)
+                if a == b:
+                    G = caches[a]
+                    Rk = kress_R(na)
+                    r = G.R
+                    pidx, th = ph.locate(r)
+                    pjx, tj = pj.locate(r)
+                    h0, h1 = _h_cut(k, r, clenshaw(H0[pidx], th), clenshaw(H1[pidx], th))
+                    j0, j1 = clenshaw(J0[pjx], tj), clenshaw(J1[pjx], tj)
+                    sj = G.speed[None, :]
+                    wj = cb.ws[None, :]
+                    lt = G.logterm
+                    inn, invr = G.inner, G.invR
+                    kr = k * r
+                    L1, L2, M1, M2 = aL1 * j1, aL2 * h1, aM1 * j0, aM2 * h0
+                    cR, cW = Rk * inn * INV_TWO_PI, wj * inn * INV_TWO_PI
+                    dval = (Rk * inn * invr) * L1 + (wj * inn * invr) * L2 - (wj * lt * inn * invr) * L1
+                    dval1 = -cR * k * j0 + cW * k * (lt * j0 + 1j * np.pi * h0)
+                    dval2 = cR * (kr * j1 - j0) + cW * (lt * (j0 - kr * j1) + 1j * np.pi * (h0 - kr * h1))
+                    sval = (Rk * sj) * M1 + (wj * sj) * M2 - (wj * sj * lt) * M1
+                    sval1 = (r * sj * INV_TWO_PI) * (Rk * j1 - wj * (lt * j1 + 1j * np.pi * h1))
+                    sval2 = (r * sj * INV_TWO_PI / k) * (Rk * (kr * j0 - j1) + wj * (
+                        lt * (j1 - kr * j0) + 1j * np.pi * (h1 - kr * h0)))
+                    A = -(dval + ik * sval)
+                    A1 = -(dval1 + 1j * sval + ik * sval1)
+                    A2 = -(dval2 + 2j * sval1 + ik * sval2)
+                    # diagonal (cheb-CFIE:922-941)
+                    s, w = G.speed, ca.ws
+                    m2 = ((0.5j - EULER_OVER_PI) - INV_TWO_PI * np.log((k * k / 4.0) * (s * s))) * s
+                    sv = np.diag(Rk) * (aM1 * s) + w * m2
+                    sv1 = w * (-s / (np.pi * k))
+                    sv2 = w * (s / (np.pi * k * k))
+                    ii = np.arange(na)
+                    A[ii, ii] = 1.0 - (w * G.kappa + ik * sv)
+                    A1[ii, ii] = -(1j * sv + ik * sv1)
+                    A2[ii, ii] = -(2j * sv1 + ik * sv2)
+                else:
+                    DX = ca.x[:, None] - cb.x[None, :]
+                    DY = ca.y[:, None] - cb.y[None, :]
+                    r = np.hypot(DX, DY)
+                    invr = 1.0 / r
+                    inn = cb.dy[None, :] * DX - cb.dx[None, :] * DY
+                    pidx, th = ph.locate(r)
+                    h0, h1 = _h_cut(k, r, clenshaw(H0[pidx], th), clenshaw(H1[pidx], th))
+                    sjb = np.sqrt(cb.dx ** 2 + cb.dy ** 2)[None, :]
+                    wj = cb.ws[None, :]
+                    cD, cD1, cS = wj * inn * invr, wj * inn, wj * sjb
+                    sg = -1.0 if reference_offblock_sign else 1.0
+                    dval = cD * aL2 * h1
+                    dval1 = sg * 0.5j * cD1 * k * h0
+                    dval2 = sg * 0.5j * cD1 * (h0 - k * r * h1)
+                    sval = cS * aM2 * h0
+                    sval1 = -0.5j * cS * r * h1
+                    sval2 = 0.5j * cS * r * (h1 - k * r * h0) / k
+                    A = -(dval + ik * sval)
+                    A1 = -(dval1 + 1j * sval + ik * sval1)
+                    A2 = -(dval2 + 2j * sval1 + ik * sval2)
+                for o, X in zip(out, (A, A1, A2)):
+                    o[q, ro:ro + na, co:co + nb] = X
+    return tuple(out)
+
+
+def dlp_assemble_deriv(comp: Comp, zs, cp: ChebParams = ChebParams()):
+    """construct_boundary_matrices_with_derivatives! for DLP_kress
+    (cheb-DLP:1126-1144 -> :881-952, then F = I - D, F' = -D', F'' = -D'',
+    :1097-1108).  H0, H1, J0, J1 with the |kr| < 0.2 direct branch."""
+    zs = np.asarray(zs, dtype=np.complex128)
+    G = geom_cache(comp)
+    n = len(comp.x)
+    Rk = kress_R(n)
+    R = G.R.copy()
+    np.fill_diagonal(R, 0.0)
+    m = R > np.finfo(float).eps
+    rmin, rmax = float(np.min(R[m])) * 0.95, float(np.max(R[m])) * 1.05
+    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+    pidx, th = ph.locate(G.R)
+    pjx, tj = pj.locate(G.R)
+    inn, invr, lt, r = G.inner, G.invR, G.logterm, G.R
+    wj = comp.ws[None, :]
+    out = [np.zeros((len(zs), n, n), np.complex128) for _ in range(3)]
+    ii = np.arange(n)
+    for q, k in enumerate(zs):
+        H0, H1, J0, J1 = _tables4(k, rmin, rmax, cp)
+        h0, h1 = _h_cut(k, r, clenshaw(H0[pidx], th), clenshaw(H1[pidx], th))
+        j0, j1 = clenshaw(J0[pjx], tj), clenshaw(J1[pjx], tj)
+        cD = inn * invr
+        l1 = (-k * INV_TWO_PI) * j1
+        D = (Rk * cD) * l1 + (wj * cD) * ((0.5j * k) * h1) - (wj * lt * cD) * l1
+        D1 = (k * INV_TWO_PI) * (Rk * (-inn) * j0 + wj * (lt * inn * j0 + 1j * np.pi * inn * h0))
+        u, v = j0 - k * r * j1, h0 - k * r * h1
+        D2 = (INV_TWO_PI * inn) * (Rk * (-u) + wj * (lt * u + 1j * np.pi * v))
+        D[ii, ii] = comp.ws * G.kappa
+        D1[ii, ii] = 0.0
+        D2[ii, ii] = 0.0
+        out[0][q] = np.eye(n) - D
+        out[1][q] = -D1
+        out[2][q] = -D2
+    return tuple(out)
+
+
+def cauchy_derivatives(assemble, z: complex, rho: float = 0.25, n: int = 24):
+    """dT/dz and d2T/dz2 of an analytic matrix function from its values on
+    the circle z + rho e^{i theta_m} (trapezoid rule of Cauchy's integral
+    formula, exponentially accurate).  `assemble(zs)` returns [len(zs), N, N]."""
+    th = 2 * np.pi * np.arange(n) / n
+    e = np.exp(1j * th)
+    F = assemble(z + rho * e)
+    d1 = np.tensordot(np.exp(-1j * th), F, axes=(0, 0)) / (n * rho)
+    d2 = 2 * np.tensordot(np.exp(-2j * th), F, axes=(0, 0)) / (n * rho * rho)
+    return d1, d2
+
+
 def cfie_direct_real(comps, k: float):
     """Exact-Hankel CFIE at real k (cfie_kress.jl:416-510), one component
     (the twin of the Chebyshev path, with J_n = real(H_n) for real k)."""

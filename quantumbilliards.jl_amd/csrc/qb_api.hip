@@ -499,6 +499,58 @@ extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const doub
   return rc;
 }
 
+extern "C" int qb_assemble_deriv(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
+                                 int64_t nz, const qb_cheb_params* cheb, double* out,
+                                 double* dout, double* ddout, int32_t flags, double* rminmax) {
+  if (!ctx) return QB_E_ARG;
+  if (!g || !z || !out || !dout || !ddout || nz < 0) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1)
+    return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  int rc = check_cheb(ctx, cheb);
+  if (rc) return rc;
+  if (nz == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  double rmin, rmax;
+  op_range(g, op, &rmin, &rmax);
+  if (rminmax) {
+    rminmax[0] = rmin;
+    rminmax[1] = rmax;
+  }
+  const int64_t N = g->d.N;
+  const size_t obytes = (size_t)nz * N * N * sizeof(cd);
+  const bool on_dev = (flags & QB_OUT_DEVICE) != 0;
+  cd* zd = nullptr;
+  cd* od[3] = {nullptr, nullptr, nullptr};
+  double* host[3] = {out, dout, ddout};
+  DevTables t;
+  rc = qb_dev_alloc(ctx, (void**)&zd, nz * sizeof(cd));
+  if (!rc && hipMemcpyAsync(zd, z, nz * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) !=
+                 hipSuccess)
+    rc = qb_fail(ctx, QB_E_HIP, "upload failed");
+  if (!rc) rc = qb_tables_build(ctx, (const double*)zd, nz, rmin, rmax, *cheb, true, &t);
+  for (int m = 0; m < 3 && !rc; ++m) {
+    if (on_dev)
+      od[m] = (cd*)host[m];
+    else
+      rc = qb_dev_alloc(ctx, (void**)&od[m], obytes);
+  }
+  if (!rc)
+    rc = qb_assemble_deriv_dev(ctx, g, op, zd, nz, t, od[0], od[1], od[2],
+                               (flags & QB_DERIV_REFERENCE_SIGN) != 0);
+  for (int m = 0; m < 3 && !rc && !on_dev; ++m)
+    if (hipMemcpyAsync(host[m], od[m], obytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+      rc = qb_fail(ctx, QB_E_HIP, "download failed");
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess && !rc)
+    rc = qb_fail(ctx, QB_E_HIP, "assembly failed");
+  if (!on_dev)
+    for (int m = 0; m < 3; ++m) qb_dev_free(ctx, od[m], obytes);
+  qb_tables_free(ctx, &t);
+  qb_dev_free(ctx, zd, nz * sizeof(cd));
+  return rc;
+}
+
 // ---------------------------------------------------------------------------
 // residual check of candidate eigenpairs
 // ---------------------------------------------------------------------------

@@ -274,3 +274,277 @@ int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int6
   }
   return QB_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Assembly with the first and second wavenumber derivatives, T, dT/dk and
+// d2T/dk2, for the expanded boundary integral method:
+//   CFIE: _all_k_nosymm_CFIE_chebyshev_deriv! (cheb-CFIE:881-1059) behind
+//         construct_boundary_matrices_with_derivatives! (cheb-CFIE:1507-1530);
+//   DLP:  _construct_dlp_kress_matrices_derivatives_chebyshev! and the sign
+//         flip + identity of its caller (cheb-DLP:881-952, :1097-1108), behind
+//         construct_boundary_matrices_with_derivatives! (cheb-DLP:1126-1144).
+// Both read H0, H1, J0, J1 with the |kr| < 0.2 direct branch
+// (h0_h1_j0_j1_multi_ks_at_r!; the plain DLP path has no such branch).
+// ---------------------------------------------------------------------------
+struct D3 {
+  cd a, a1, a2;
+};
+
+template <int MH, int MJ>
+__device__ __forceinline__ Spec specials_all(const TabView& tv, double r, cd k, bool want_j) {
+  Spec s;
+  hank<MH, MJ>(tv, r, k, true, &s.h0, &s.h1);
+  if (want_j) {
+    bessj<MH, MJ>(tv, r, &s.j0, &s.j1);
+  } else {
+    s.j0 = mk(0, 0);
+    s.j1 = mk(0, 0);
+  }
+  return s;
+}
+
+// Off-diagonal entry (target i, source j) of T, T', T''.
+// Between components the derivative of the double-layer term
+// w_j (inn/r)(i k/2) H1(kr) is +(i/2) w_j inn k H0 (and its k-derivative);
+// the reference writes both with a minus sign (cheb-CFIE:1031-1032).
+// off_sign = -1 reproduces that, +1 (the default) is the derivative.
+template <int OP>
+__device__ __forceinline__ D3 offdiag_deriv(const DevGeom& g, int64_t j, double dx, double dy,
+                                            double r, double invr, bool same, double Rij,
+                                            double lt, cd k, const Spec& sp, double off_sign) {
+  const cd ik = mk(-k.im, k.re);
+  const cd I = mk(0.0, 1.0);
+  const cd aL1 = mk(-k.re * QB_INV_TWO_PI, -k.im * QB_INV_TWO_PI);
+  const cd aL2 = mk(-0.5 * k.im, 0.5 * k.re);
+  const cd aM2 = mk(0.0, 0.5);
+  const double aM1 = -QB_INV_TWO_PI;
+  const double wj = g.ws[j];
+  const double inn = g.dy[j] * dx - g.dx[j] * dy;
+  const cd kr = k * r;
+  D3 o;
+  if (OP == QB_DLP_KRESS) {
+    const double cD = inn * invr;
+    const double c1 = Rij * cD, c2 = wj * cD, c3 = wj * lt * cD;
+    const cd l1 = aL1 * sp.j1;
+    const cd D = c1 * l1 + c2 * (aL2 * sp.h1) - c3 * l1;
+    const cd kc = k * QB_INV_TWO_PI;
+    const cd D1 = kc * ((-Rij * inn) * sp.j0 + wj * ((lt * inn) * sp.j0 + (QB_PI * inn) * (I * sp.h0)));
+    const cd u = sp.j0 - kr * sp.j1, v = sp.h0 - kr * sp.h1;
+    const cd D2 = (inn * QB_INV_TWO_PI) * ((-Rij) * u + wj * (lt * u + QB_PI * (I * v)));
+    o.a = -D;
+    o.a1 = -D1;
+    o.a2 = -D2;
+    return o;
+  }
+  const double sj = g.speed[j];
+  const cd mh = mk(0.0, -0.5), ph = mk(0.0, 0.5);
+  if (!same) {
+    const double cD = wj * inn * invr, cD1 = wj * inn, cS = wj * sj;
+    const cd dval = cD * (aL2 * sp.h1);
+    const cd dval1 = off_sign * (ph * (cD1 * (k * sp.h0)));
+    const cd dval2 = off_sign * (ph * (cD1 * (sp.h0 - kr * sp.h1)));
+    const cd sval = cS * (aM2 * sp.h0);
+    const cd sval1 = mh * ((cS * r) * sp.h1);
+    const cd sval2 = ph * ((cS * r) * cdiv(sp.h1 - kr * sp.h0, k));
+    o.a = -(dval + ik * sval);
+    o.a1 = -(dval1 + I * sval + ik * sval1);
+    o.a2 = -(dval2 + 2.0 * (I * sval1) + ik * sval2);
+    return o;
+  }
+  const cd L1 = aL1 * sp.j1, L2 = aL2 * sp.h1;
+  const cd M1 = aM1 * sp.j0, M2 = aM2 * sp.h0;
+  const double cDR = Rij * inn * invr, cDW = wj * inn * invr, cDL = wj * lt * inn * invr;
+  const double cR = Rij * inn * QB_INV_TWO_PI, cW = wj * inn * QB_INV_TWO_PI;
+  const double cSR = Rij * sj, cS = wj * sj;
+  const cd dval = cDR * L1 + cDW * L2 - cDL * L1;
+  const cd dval1 = -(cR * (k * sp.j0)) + cW * (k * (lt * sp.j0 + QB_PI * (I * sp.h0)));
+  const cd dval2 = cR * (kr * sp.j1 - sp.j0) +
+                   cW * (lt * (sp.j0 - kr * sp.j1) + QB_PI * (I * (sp.h0 - kr * sp.h1)));
+  const cd sval = cSR * M1 + cS * M2 - (cS * lt) * M1;
+  const cd sval1 = (r * sj * QB_INV_TWO_PI) * (Rij * sp.j1 - wj * (lt * sp.j1 + QB_PI * (I * sp.h1)));
+  const cd sval2 = (r * sj * QB_INV_TWO_PI) *
+                   cdiv(Rij * (kr * sp.j0 - sp.j1) +
+                            wj * (lt * (sp.j1 - kr * sp.j0) + QB_PI * (I * (sp.h1 - kr * sp.h0))),
+                        k);
+  o.a = -(dval + ik * sval);
+  o.a1 = -(dval1 + I * sval + ik * sval1);
+  o.a2 = -(dval2 + 2.0 * (I * sval1) + ik * sval2);
+  return o;
+}
+
+// Diagonal entries (cheb-CFIE:922-941; cheb-DLP: D_ii = w_i kappa_i, no
+// derivative terms, then F = I - D).
+template <int OP>
+__device__ __forceinline__ D3 diag_deriv(const DevGeom& g, int64_t j, cd k) {
+  D3 o;
+  const double wj = g.ws[j];
+  if (OP == QB_DLP_KRESS) {
+    o.a = mk(1.0 - wj * g.kappa[j], 0.0);
+    o.a1 = mk(0, 0);
+    o.a2 = mk(0, 0);
+    return o;
+  }
+  const cd I = mk(0.0, 1.0);
+  const cd ik = mk(-k.im, k.re);
+  const double sj = g.speed[j];
+  const double rjj = g.rk[g.off[g.comp[j]]];
+  const cd dval = mk(wj * g.kappa[j], 0.0);
+  const double m1 = -QB_INV_TWO_PI * sj;
+  const cd k2 = k * k;
+  const cd lg = clog(mk(0.25 * k2.re * (sj * sj), 0.25 * k2.im * (sj * sj)));
+  const cd m2 = mk(-QB_EULER_OVER_PI - QB_INV_TWO_PI * lg.re, 0.5 - QB_INV_TWO_PI * lg.im) * sj;
+  const cd sval = mk(rjj * m1, 0.0) + wj * m2;
+  const cd sval1 = (-wj * sj / QB_PI) * crecip(k);
+  const cd sval2 = (wj * sj / QB_PI) * crecip(k2);
+  const cd v = dval + ik * sval;
+  o.a = mk(1.0 - v.re, -v.im);
+  o.a1 = -(I * sval + ik * sval1);
+  o.a2 = -(2.0 * (I * sval1) + ik * sval2);
+  return o;
+}
+
+// Tile pairs as in k_assemble; the mirrored tile of each of the three
+// outputs is staged in LDS 16 source columns at a time (3 x 16 x 65 complex
+// = 50 KB) and written 16 contiguous rows per column.
+template <int OP, int MH, int MJ>
+__global__ void __launch_bounds__(256) k_assemble_deriv(DevGeom g, const cd* __restrict__ zk,
+                                                        const cd* __restrict__ H,
+                                                        const cd* __restrict__ J, int np_h,
+                                                        int M_h, int np_j, int M_j,
+                                                        double rmin_h, double rmax, int64_t q0,
+                                                        int T, cd* __restrict__ o0,
+                                                        cd* __restrict__ o1,
+                                                        cd* __restrict__ o2, double off_sign) {
+  __shared__ cd mir[3][16][65];  // mir[out][j_local - 16 ch][i_local] = A[j, i]
+  const int64_t N = g.N;
+  const int64_t q = q0 + blockIdx.z;
+  int64_t p = blockIdx.x;
+  int d = 0;
+  while (p >= T - d) {
+    p -= T - d;
+    ++d;
+  }
+  const int Jt = (int)p, It = Jt + d;
+  TabView tv;
+  tv.H = H + q * (int64_t)np_h * (2 * (M_h + 1));
+  tv.J = J + q * (int64_t)np_j * (2 * (M_j + 1));
+  tv.np_h = np_h;
+  tv.M_h = M_h;
+  tv.np_j = np_j;
+  tv.M_j = M_j;
+  tv.rmin_h = rmin_h;
+  tv.rmax = rmax;
+  const cd k = zk[q];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t i = (int64_t)It * 64 + il;
+  const int64_t base = (q - q0) * N * N;
+  const bool diag_tile = (It == Jt);
+  const bool iok = i < N;
+  double xi = 0, yi = 0;
+  int ci = 0;
+  if (iok) {
+    xi = g.x[i];
+    yi = g.y[i];
+    ci = g.comp[i];
+  }
+#pragma unroll 1
+  for (int ch = 0; ch < 4; ++ch) {
+#pragma unroll 1
+    for (int c = 0; c < 4; ++c) {
+      const int jc = jl0 + 4 * c;  // 0..15 inside the chunk
+      const int64_t j = (int64_t)Jt * 64 + 16 * ch + jc;
+      if (!iok || j >= N) continue;
+      D3 vij, vji;
+      if (i == j) {
+        vij = diag_deriv<OP>(g, j, k);
+      } else {
+        const double dx = xi - g.x[j], dy = yi - g.y[j];
+        const double r = hypot(dx, dy);
+        const double invr = 1.0 / r;
+        const int cj = g.comp[j];
+        const bool same = (ci == cj);
+        double Rij = 0, lt = 0;
+        if (same) {
+          const int64_t o = g.off[ci];
+          const int64_t nc = g.off[ci + 1] - o;
+          int64_t dd = (i - j) % nc;
+          if (dd < 0) dd += nc;
+          Rij = g.rk[o + dd];
+          lt = g.lt[o + dd];
+        }
+        const Spec sp = specials_all<MH, MJ>(tv, r, k, same);
+        vij = offdiag_deriv<OP>(g, j, dx, dy, r, invr, same, Rij, lt, k, sp, off_sign);
+        if (!diag_tile)
+          vji = offdiag_deriv<OP>(g, i, -dx, -dy, r, invr, same, Rij, lt, k, sp, off_sign);
+      }
+      const int64_t idx = base + i + j * N;
+      o0[idx] = vij.a;
+      o1[idx] = vij.a1;
+      o2[idx] = vij.a2;
+      if (!diag_tile) {
+        mir[0][jc][il] = vji.a;
+        mir[1][jc][il] = vji.a1;
+        mir[2][jc][il] = vji.a2;
+      }
+    }
+    if (diag_tile) continue;  // uniform over the workgroup
+    __syncthreads();
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int jr = e & 15, il2 = e >> 4;
+      const int64_t ii = (int64_t)It * 64 + il2, jj = (int64_t)Jt * 64 + 16 * ch + jr;
+      if (ii >= N || jj >= N) continue;
+      const int64_t idx = base + jj + ii * N;
+      o0[idx] = mir[0][jr][il2];
+      o1[idx] = mir[1][jr][il2];
+      o2[idx] = mir[2][jr][il2];
+    }
+    __syncthreads();
+  }
+}
+
+template <int OP, int MH, int MJ>
+static void launch_asm_deriv(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, int64_t nzc,
+                             const DevTables& t, cd* o0, cd* o1, cd* o2, double off_sign) {
+  int64_t N = g->d.N;
+  int T = (int)((N + 63) / 64);
+  int64_t pairs = (int64_t)T * (T + 1) / 2;
+  dim3 grid((unsigned)pairs, 1, (unsigned)nzc);
+  int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0, 3.0 * (double)N * N * nzc * sizeof(cd));
+  hipLaunchKernelGGL((k_assemble_deriv<OP, MH, MJ>), grid, dim3(256), 0, ctx->stream, g->d, z,
+                     t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0, T, o0, o1, o2,
+                     off_sign);
+  qb_prof_end(ctx, slot);
+}
+
+int qb_assemble_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
+                          const DevTables& t, cd* out, cd* dout, cd* ddout, bool ref_sign) {
+  const double off_sign = ref_sign ? -1.0 : 1.0;
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS)
+    return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1)
+    return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  if (!t.J) return qb_fail(ctx, QB_E_ARG, "derivative assembly needs the J tables");
+  int64_t N = g->d.N;
+  for (int64_t q0 = 0; q0 < nz; q0 += 1024) {
+    int64_t nzc = nz - q0 < 1024 ? nz - q0 : 1024;
+    const int64_t o = q0 * N * N;
+    const int mh = t.M_h, mj = t.M_j;
+    if (op == QB_CFIE_KRESS) {
+      if (mh == 7 && mj == 7)
+        launch_asm_deriv<QB_CFIE_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                              ddout + o, off_sign);
+      else
+        launch_asm_deriv<QB_CFIE_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                              ddout + o, off_sign);
+    } else {
+      if (mh == 7 && mj == 7)
+        launch_asm_deriv<QB_DLP_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                             ddout + o, off_sign);
+      else
+        launch_asm_deriv<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                             ddout + o, off_sign);
+    }
+    QB_HIP(ctx, hipGetLastError());
+  }
+  return QB_OK;
+}

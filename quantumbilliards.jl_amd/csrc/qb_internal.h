@@ -113,6 +113,11 @@ void qb_lobatto_host(int M, double* t, double* C);  // nodes cospi(j/M), DCT mat
 // assembly (qb_assemble.hip) -------------------------------------------------
 // Writes T(z_q) for q in [0, nz) into out64 (ld = N, stride N*N), and, if
 // out32 != nullptr, a complex64 copy into out32.
+// T(z_q), dT/dk and d2T/dk2 for q in [0, nz) (same layout as qb_assemble_dev).
+// ref_sign: the reference's sign of the between-component double-layer
+// derivative terms (qb_assemble_deriv, QB_DERIV_REFERENCE_SIGN).
+int qb_assemble_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
+                          const DevTables& t, cd* out, cd* dout, cd* ddout, bool ref_sign);
 int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
                     int64_t nz, const DevTables& t, cd* out64, cf* out32);
 

@@ -10,8 +10,9 @@ from .context import Context, default_context
 from .geometry import (Billiard, BoundaryPointsCFIE, CircularHoleBilliard, evaluate_points,
                        make_circle, make_full_ellipse, make_full_stadium,
                        make_star_polar_boundary, n_kress)
-from .solvers import (CFIE_kress, DLP_kress, Geometry, assemble, bessel_eval,
-                      boundary_matrix_size, chebyshev_params, construct_boundary_matrices)
+from .solvers import (CFIE_kress, DLP_kress, Geometry, assemble, assemble_with_derivatives,
+                      bessel_eval, boundary_matrix_size, chebyshev_params,
+                      construct_boundary_matrices, construct_boundary_matrices_with_derivatives)
 from .beyn import (B_from_moments, Window, construct_B_matrix, footprint, probe_block, probe_count, solve,
                    solve_vect, weyl_count)
 from .spectrum import (beyn_disks_from_windows, compute_spectrum_beyn, delta_weyl,

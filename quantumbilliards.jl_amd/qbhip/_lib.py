@@ -16,13 +16,14 @@ QB_OK, QB_E_ARG, QB_E_HIP, QB_E_NOMEM, QB_E_STATE, QB_E_NUMERIC = 0, -1, -2, -3,
 QB_CFIE_KRESS, QB_DLP_KRESS = 1, 2
 QB_FP64, QB_MIXED = 0, 1
 QB_OUT_DEVICE = 1
+QB_DERIV_REFERENCE_SIGN = 2
 
 EXPORTS = (
     "qb_ctx_create", "qb_ctx_destroy", "qb_last_error", "qb_ctx_device_bytes",
     "qb_geom_create", "qb_geom_destroy", "qb_geom_size", "qb_assemble",
     "qb_window_create", "qb_window_destroy", "qb_window_footprint", "qb_window_moments",
     "qb_bessel_eval", "qb_cheb_tune", "qb_ctx_set_profiling", "qb_ctx_kernel_stats",
-    "qb_ctx_autotune", "qb_ctx_set_cache_limit", "qb_residuals",
+    "qb_ctx_autotune", "qb_ctx_set_cache_limit", "qb_residuals", "qb_assemble_deriv",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
@@ -74,6 +75,8 @@ def lib():
         "qb_geom_destroy": (None, [P]),
         "qb_geom_size": (I64, [P]),
         "qb_assemble": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P]),
+        "qb_assemble_deriv": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, P, P, I32,
+                                        P]),
         "qb_window_create": (C.c_int, [P, P, I32, D, D, D, I32, C.POINTER(ChebParams), I32, PP,
                                        C.POINTER(Stats)]),
         "qb_window_destroy": (None, [P]),

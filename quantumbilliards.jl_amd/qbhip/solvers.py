@@ -144,6 +144,62 @@ def construct_boundary_matrices(Tbufs, solver, pts, zj, *, multithreaded: bool =
     return None
 
 
+def assemble_with_derivatives(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_panels_j=3000,
+                              M_j=5, geom: Optional[Geometry] = None,
+                              ctx: Optional[Context] = None,
+                              reference_offblock_sign: bool = False):
+    """T(z_q), dT/dk(z_q) and d2T/dk2(z_q) in one launch (qb_assemble_deriv).
+    Returns three arrays of shape (nz, N, N) indexed like `assemble`.
+    reference_offblock_sign=True gives the between-component double-layer
+    derivative terms the reference's (opposite) sign, cheb-CFIE:1031-1032."""
+    ctx = ctx or (geom.ctx if geom is not None else default_context())
+    g = geom or Geometry(pts, ctx)
+    zj = np.ascontiguousarray(np.asarray(zj, dtype=np.complex128).ravel())
+    nz, N = len(zj), g.N
+    raws = [np.empty((nz, N, N), dtype=np.complex128) for _ in range(3)]
+    cp = _cheb(n_panels_h, M_h, n_panels_j, M_j)
+    rr = np.zeros(2)
+    flags = _lib.QB_DERIV_REFERENCE_SIGN if reference_offblock_sign else 0
+    rc = _lib.lib().qb_assemble_deriv(ctx.ptr, g.ptr, solver.op, zj.ctypes.data, nz, C.byref(cp),
+                                      *[r.ctypes.data for r in raws], flags, rr.ctypes.data)
+    _lib.check(ctx.ptr, rc)
+    return tuple(r.transpose(0, 2, 1) for r in raws)
+
+
+def construct_boundary_matrices_with_derivatives(Tbufs, dTbufs, ddTbufs, solver, pts, zj, *,
+                                                 multithreaded: bool = True,
+                                                 use_chebyshev: bool = True,
+                                                 n_panels_h: int = 15000, M_h: int = 5,
+                                                 n_panels_j: int = 3000, M_j: int = 5,
+                                                 timeit: bool = False,
+                                                 geom: Optional[Geometry] = None,
+                                                 ctx: Optional[Context] = None,
+                                                 reference_offblock_sign: bool = False):
+    """construct_boundary_matrices_with_derivatives!(Tbufs, dTbufs, ddTbufs,
+    solver, pts, zj; ...) (cheb-CFIE:1507-1530, cheb-DLP:1126-1144): the
+    three buffers of every wavenumber are overwritten with T, dT/dk and
+    d2T/dk2."""
+    if not use_chebyshev:
+        raise RuntimeError("Direct derivative matrix construction is only for real k")
+    zj = np.asarray(zj, dtype=np.complex128).ravel()
+    Mk = len(zj)
+    assert len(Tbufs) == Mk and len(dTbufs) == Mk and len(ddTbufs) == Mk
+    if isinstance(solver, DLP_kress) and not isinstance(pts, BoundaryPointsCFIE):
+        assert len(_as_list(pts)) == 1, "DLP_kress expects one BoundaryPointsCFIE component"
+    N = boundary_matrix_size(pts)
+    for bufs in (Tbufs, dTbufs, ddTbufs):
+        for q, T in enumerate(bufs):
+            assert T.shape == (N, N), f"buffer {q} has size {T.shape}, but ({N},{N}) is required"
+    A, A1, A2 = assemble_with_derivatives(solver, pts, zj, n_panels_h=n_panels_h, M_h=M_h,
+                                          n_panels_j=n_panels_j, M_j=M_j, geom=geom, ctx=ctx,
+                                          reference_offblock_sign=reference_offblock_sign)
+    for q in range(Mk):
+        np.copyto(Tbufs[q], A[q])
+        np.copyto(dTbufs[q], A1[q])
+        np.copyto(ddTbufs[q], A2[q])
+    return None
+
+
 def chebyshev_params(solver, pts, zj, *, npanels_h_init=15000, M_h_init=5, npanels_j_init=3000,
                      M_j_init=5, tol=1e-10, sampling_points=50000, max_iter=20,
                      geom: Optional[Geometry] = None, ctx: Optional[Context] = None):

@@ -173,3 +173,47 @@ def test_device_matches_golden_fixtures():
             i = int(np.flatnonzero(m_all == e["m"])[0])
             assert abs(lt[i] - c(e["cfie"])) < 1e-10, (key, e["m"])
             assert abs(lf[i] - c(e["dlp"])) < 1e-10, (key, e["m"])
+
+
+@pytest.mark.parametrize("kind,Ns,k0,refsign", [("circle_hole", [200, 60], 25.0, False),
+                                                ("circle_hole", [200, 60], 25.0, True),
+                                                ("star", [384], 40.0, False)])
+def test_cfie_derivative_assembly_matches_oracle(kind, Ns, k0, refsign):
+    """construct_boundary_matrices_with_derivatives!: T, dT/dk and d2T/dk2 at
+    complex wavenumbers against the oracle (same tables), with the
+    between-component sign of the derivative of the matrix and of the
+    reference."""
+    q = qb()
+    b, pts = _pts(kind, Ns)
+    zj = np.array([k0 + 0.3 + 0.2j, k0 - 0.4 - 0.1j])
+    cp = dict(n_panels_h=4000, M_h=7, n_panels_j=1500, M_j=7)
+    A, A1, A2 = q.assemble_with_derivatives(q.CFIE_kress(10.0, b), pts, zj,
+                                            reference_offblock_sign=refsign, **cp)
+    R = O.cfie_assemble_deriv(O.make_geometry(kind, Ns), zj, O.ChebParams(4000, 7, 1500, 7),
+                              reference_offblock_sign=refsign)
+    for got, ref in zip((A, A1, A2), R):
+        e = _relerr(got, ref)
+        print(kind, refsign, e.max(axis=0))
+        assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-10
+    # the undifferentiated output is the plain assembly
+    P = q.assemble(q.CFIE_kress(10.0, b), pts, zj, **cp)
+    assert np.max(np.abs(np.asarray(A) - np.asarray(P))) <= 1e-14 * np.max(np.abs(P))
+
+
+def test_dlp_derivative_assembly_matches_oracle():
+    q = qb()
+    b = q.make_full_ellipse(1.0, 0.6)
+    pts = q.evaluate_points(q.DLP_kress(10.0, b), b, 1.0, N=[320])
+    zj = np.array([35.0 + 0.25j, 34.6 - 0.3j])
+    Tb = [np.zeros((320, 320), complex, order="F") for _ in range(2)]
+    dT = [np.zeros((320, 320), complex, order="F") for _ in range(2)]
+    ddT = [np.zeros((320, 320), complex, order="F") for _ in range(2)]
+    q.construct_boundary_matrices_with_derivatives(Tb, dT, ddT, q.DLP_kress(10.0, b), pts, zj,
+                                                   n_panels_h=3000, M_h=7, n_panels_j=1000,
+                                                   M_j=7)
+    R = O.dlp_assemble_deriv(O.make_geometry("ellipse", [320])[0], zj,
+                             O.ChebParams(3000, 7, 1000, 7))
+    for got, ref in zip((np.array(Tb), np.array(dT), np.array(ddT)), R):
+        e = _relerr(got, ref)
+        print(e.max(axis=0))
+        assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-10

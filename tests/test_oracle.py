@@ -143,3 +143,60 @@ def test_jm_zero_fixture():
     np.testing.assert_allclose(np.unique(ex), [19.554536430997054, 19.615858510468243,
                                                19.615966903966921, 19.994430629816385,
                                                20.320789213566506], rtol=1e-14)
+
+
+@pytest.mark.parametrize("kind,Ns,k", [("circle_hole", [60, 20], 12.0 + 0.1j),
+                                       ("star", [96], 15.0 - 0.2j)])
+def test_cfie_derivative_formulas_match_cauchy_derivatives(kind, Ns, k):
+    """The wavenumber derivatives of the CFIE matrix (the reference's analytic
+    formulas, restated in cfie_assemble_deriv) equal the derivatives of the
+    plain assembly taken by Cauchy's integral formula on a circle around k;
+    the undifferentiated output equals the plain assembly."""
+    comps = O.make_geometry(kind, Ns)
+    cp = O.ChebParams(3000, 7, 1500, 7)
+    A, A1, A2 = O.cfie_assemble_deriv(comps, [k], cp)
+    P = O.cfie_assemble(comps, [k], cp)[0]
+    assert np.max(np.abs(A[0] - P)) < 1e-14 * np.max(np.abs(P)) * 10
+    d1, d2 = O.cauchy_derivatives(lambda zs: O.cfie_assemble(comps, zs, cp), k)
+    e1 = np.max(np.abs(A1[0] - d1)) / np.max(np.abs(d1))
+    e2 = np.max(np.abs(A2[0] - d2)) / np.max(np.abs(d2))
+    print(kind, e1, e2)
+    assert e1 < 1e-11 and e2 < 1e-10
+
+
+def test_dlp_derivative_formulas_match_cauchy_derivatives():
+    comp = O.make_geometry("ellipse", [64])[0]
+    k = 9.0 - 0.2j
+    cp = O.ChebParams(3000, 7, 1500, 7)
+    F, F1, F2 = O.dlp_assemble_deriv(comp, [k], cp)
+    P = O.dlp_assemble(comp, [k], cp)[0]
+    assert np.max(np.abs(F[0] - P)) < 1e-13
+    d1, d2 = O.cauchy_derivatives(lambda zs: O.dlp_assemble(comp, zs, cp), k)
+    e1 = np.max(np.abs(F1[0] - d1)) / np.max(np.abs(d1))
+    e2 = np.max(np.abs(F2[0] - d2)) / np.max(np.abs(d2))
+    print(e1, e2)
+    assert e1 < 1e-11 and e2 < 1e-10
+
+
+def test_cfie_derivative_reference_sign_differs_only_between_components():
+    """reference_offblock_sign flips exactly the two double-layer derivative
+    terms of the between-component blocks: the same-component blocks are
+    untouched and the difference there is 2 x (i/2) w_j inn k H0 for dT/dk."""
+    comps = O.make_geometry("circle_hole", [60, 20])
+    cp = O.ChebParams(3000, 7, 1500, 7)
+    k = 12.0 + 0.1j
+    A, A1, A2 = O.cfie_assemble_deriv(comps, [k], cp)
+    B, B1, B2 = O.cfie_assemble_deriv(comps, [k], cp, reference_offblock_sign=True)
+    assert np.array_equal(A, B)
+    for X, Y in ((A1, B1), (A2, B2)):
+        assert np.array_equal(X[0, :60, :60], Y[0, :60, :60])
+        assert np.array_equal(X[0, 60:, 60:], Y[0, 60:, 60:])
+    ca, cb = comps[1], comps[0]   # rows of the hole, columns of the outer boundary
+    DX = ca.x[:, None] - cb.x[None, :]
+    DY = ca.y[:, None] - cb.y[None, :]
+    inn = cb.dy[None, :] * DX - cb.dx[None, :] * DY
+    r = np.hypot(DX, DY)
+    h0 = sp.hankel1(0, k * r)
+    expect = -2 * (0.5j * cb.ws[None, :] * inn * k * h0)   # A1 - B1 = -(d1 - (-d1))
+    got = A1[0, 60:, :60] - B1[0, 60:, :60]
+    assert np.max(np.abs(got - expect)) < 1e-11 * np.max(np.abs(expect))
