@@ -69,6 +69,17 @@ const char* qb_last_error(const qb_ctx* ctx);
 /* Bytes currently held on the device by this context (objects plus the
  * allocation cache that keeps freed buffers for windows of the same size). */
 int64_t qb_ctx_device_bytes(const qb_ctx* ctx);
+
+/* Replay the batched LU of windows through cached hipGraphs (on != 0): the
+ * first window with given buffers and shape captures the launch sequence of
+ * both streams, later ones replay it; launch accounting bypasses the graphs.
+ * Off by default.  No reference counterpart. */
+int qb_ctx_set_graphs(qb_ctx* ctx, int32_t on);
+
+/* Free and total device memory of the context's GPU (hipMemGetInfo); the
+ * serving loop sizes its admission budget from it.  No reference
+ * counterpart. */
+int qb_ctx_mem_info(qb_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
 /* Set the cache limit in bytes (0 releases the cache and disables it). */
 int qb_ctx_set_cache_limit(qb_ctx* ctx, int64_t bytes);
 

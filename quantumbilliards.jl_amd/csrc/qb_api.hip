@@ -165,6 +165,7 @@ void qb_ctx_destroy(qb_ctx* ctx) {
     (void)hipStreamDestroy(ctx->stream_aux);
   }
   (void)hipStreamSynchronize(ctx->stream);
+  qb_graphs_clear(ctx);
   cache_release(ctx);
   if (ctx->ev_main) (void)hipEventDestroy(ctx->ev_main);
   if (ctx->ev_aux) (void)hipEventDestroy(ctx->ev_aux);
@@ -175,6 +176,26 @@ void qb_ctx_destroy(qb_ctx* ctx) {
 const char* qb_last_error(const qb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int64_t qb_ctx_device_bytes(const qb_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int qb_ctx_set_graphs(qb_ctx* ctx, int32_t on) {
+  if (!ctx) return QB_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->graphs = on != 0;
+  if (!ctx->graphs) qb_graphs_clear(ctx);
+  return QB_OK;
+}
+
+int qb_ctx_mem_info(qb_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+  if (!ctx || !free_bytes || !total_bytes) return QB_E_ARG;
+  (void)hipSetDevice(ctx->device);
+  size_t f = 0, t = 0;
+  hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
+  return QB_OK;
+}
 
 int qb_ctx_set_cache_limit(qb_ctx* ctx, int64_t bytes) {
   if (!ctx || bytes < 0) return QB_E_ARG;
@@ -817,10 +838,10 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   t1 = now_ms();
   // LU of every node (beyn.jl:270-277)
   if (precision == QB_FP64) {
-    QB_TRY(qb_getrf_batched<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
+    QB_TRY(qb_getrf_graph<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
   } else {
     QB_TRY(qb_node_rownorm(ctx, w->T64, N, nq, w->tnorm));
-    QB_TRY(qb_getrf_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
+    QB_TRY(qb_getrf_graph<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
   }
   QB_TRY(qb_perm_from_ipiv(ctx, w->ipiv, w->perm, (int)N, nq));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {

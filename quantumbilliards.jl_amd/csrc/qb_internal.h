@@ -9,7 +9,7 @@
 #include "../../include/qb.h"
 
 #define QB_MAX_COMP 8
-#define QB_GEMM_VARIANTS 7
+#define QB_GEMM_VARIANTS 9
 #define QB_GEMM_CLASSES 45
 
 // Boundary geometry resident on the device.  Node arrays are concatenated
@@ -50,6 +50,24 @@ struct qb_ctx {
   // of a GEMM / TRSM / gather / node-wise helper works on matrix bix[z]
   // (strides unchanged), so a subset of a window's nodes runs in one launch.
   const int* bix = nullptr;
+  // hipGraph replay of the batched LU (qb_getrf_graph): instantiated graphs
+  // keyed by operand type, buffers and shape; gemm_epoch changes whenever the
+  // autotuner changes a tile pick, so stale captures are not replayed.
+  bool graphs = false;
+  uint64_t gemm_epoch = 0;
+  struct GraphRec {
+    int ty;
+    const void* A;
+    int n;
+    int64_t lda, sA;
+    const int* ipiv;
+    int64_t sP;
+    const int* info;
+    int batch;
+    uint64_t epoch;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphRec> graph_cache;
   // caching allocator: freed blocks are kept (by exact size) for reuse
   std::vector<std::pair<size_t, void*>> cache;
   int64_t cache_bytes = 0;

@@ -8,6 +8,13 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
 template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
                      int* info, int batch);
+// qb_getrf_batched through a cached hipGraph when ctx->graphs is set and
+// launch accounting is off: the first call with given buffers captures the
+// launch sequence (both streams) and instantiates it, later calls replay it.
+template <typename T>
+int qb_getrf_graph(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
+                   int* info, int batch);
+void qb_graphs_clear(qb_ctx* ctx);
 // Triangular solves only (the row permutation is applied separately, fused
 // into the gather that fills B).
 template <typename T>

@@ -18,6 +18,10 @@
 // LDS and issues 4 real 16x16x4 MFMAs per complex product block
 // (v_mfma_f32_16x16x4_f32 for complex64, v_mfma_f64_16x16x4_f64 for
 // complex128) with accumulation in the operand precision.
+#include <stdlib.h>
+
+#include <vector>
+
 #include "qb_internal.h"
 #include "qb_linalg.h"
 
@@ -64,7 +68,7 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
 // coalesced global loads and the fragment reads are conflict-free.  A is
 // negated when it is written to LDS, so the accumulators hold -A*B and the
 // epilogue adds them to C.
-template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD>
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD, bool G3>
 __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
                                                   int64_t lda, int64_t sA,
                                                   const T* __restrict__ B, int64_t ldb,
@@ -112,17 +116,39 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
   C += b * sC;
   const int li = lane & 15, lk = lane >> 4;
 
-  V4 acc_re[FM][FN], acc_im[FM][FN];
+  // acc_re / acc_im hold -Re(AB) / -Im(AB); with G3 (Gauss's three-product
+  // form) they hold P1 = (-Ar) Br and P2 = (-Ai) Bi, and acc_s holds
+  // P3 = -(Ar + Ai)(Br + Bi): -Re = P1 - P2, -Im = P3 - P1 - P2.
+  V4 acc_re[FM][FN], acc_im[FM][FN], acc_s[G3 ? FM : 1][G3 ? FN : 1];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       acc_re[i][j] = V4{0, 0, 0, 0};
       acc_im[i][j] = V4{0, 0, 0, 0};
+      if constexpr (G3) acc_s[i][j] = V4{0, 0, 0, 0};
     }
 
   T ra[NA], rb[NB];
+  // tiles fully inside M x N (all but the edge tiles) load k-tiles that are
+  // fully inside K without per-element bounds checks
+  const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N);
   auto gload = [&](int k0) {
+    if (full_mn && k0 + BK <= K) {
+#pragma unroll
+      for (int p = 0; p < NA; ++p) {
+        int e = tid + 256 * p;
+        int m = e % BM, kk = e / BM;
+        ra[p] = A[(m0 + m) + (int64_t)(k0 + kk) * lda];
+      }
+#pragma unroll
+      for (int p = 0; p < NB; ++p) {
+        int e = tid + 256 * p;
+        int kk = e % BK, n = e / BK;
+        rb[p] = B[(k0 + kk) + (int64_t)(n0 + n) * ldb];
+      }
+      return;
+    }
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
       int e = tid + 256 * p;
@@ -182,15 +208,31 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
         br[j] = Br[off];
         bi[j] = Bi[off];
       }
+      if constexpr (G3) {
+        S as[FM], bs[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i) as[i] = ar[i] + ai[i];
 #pragma unroll
-        for (int j = 0; j < FN; ++j) {
-          acc_re[i][j] = mfma16(ar[i], br[j], acc_re[i][j]);
-          acc_re[i][j] = mfma16(nai[i], bi[j], acc_re[i][j]);
-          acc_im[i][j] = mfma16(ar[i], bi[j], acc_im[i][j]);
-          acc_im[i][j] = mfma16(ai[i], br[j], acc_im[i][j]);
-        }
+        for (int j = 0; j < FN; ++j) bs[j] = br[j] + bi[j];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc_re[i][j] = mfma16(ar[i], br[j], acc_re[i][j]);
+            acc_im[i][j] = mfma16(ai[i], bi[j], acc_im[i][j]);
+            acc_s[i][j] = mfma16(as[i], bs[j], acc_s[i][j]);
+          }
+      } else {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc_re[i][j] = mfma16(ar[i], br[j], acc_re[i][j]);
+            acc_re[i][j] = mfma16(nai[i], bi[j], acc_re[i][j]);
+            acc_im[i][j] = mfma16(ar[i], bi[j], acc_im[i][j]);
+            acc_im[i][j] = mfma16(ai[i], br[j], acc_im[i][j]);
+          }
+      }
     }
     if (t + 1 < nt) sstore(buf ^ 1);
     __syncthreads();
@@ -207,20 +249,27 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
         if (gm < M) {
           T* p = C + gm + (int64_t)gn * ldc;
           T c = *p;
-          c.re += acc_re[i][j][v];
-          c.im += acc_im[i][j][v];
+          if constexpr (G3) {
+            const S p1 = acc_re[i][j][v], p2 = acc_im[i][j][v];
+            c.re += p1 - p2;
+            c.im += acc_s[i][j][v] - p1 - p2;
+          } else {
+            c.re += acc_re[i][j][v];
+            c.im += acc_im[i][j][v];
+          }
           *p = c;
         }
       }
     }
 }
 
-template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD = false>
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD = false,
+          bool G3 = false>
 static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
                         const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                         int batch) {
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD>), g, dim3(256), 0, ctx->stream, M, N, K,
+  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD, G3>), g, dim3(256), 0, ctx->stream, M, N, K,
                      A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
@@ -235,17 +284,19 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 128, 128, 16, 2, 2>,      launch_gemm<cf, 256, 64, 16, 4, 1>,
     launch_gemm<cf, 64, 256, 16, 1, 4>,       launch_gemm<cf, 128, 64, 16, 2, 2>,
     launch_gemm<cf, 128, 128, 32, 2, 2>,      launch_gemm<cf, 128, 64, 16, 2, 2, true>,
-    launch_gemm<cf, 128, 128, 16, 2, 2, true>};
+    launch_gemm<cf, 128, 128, 16, 2, 2, true>, launch_gemm<cf, 128, 64, 16, 2, 2, false, true>,
+    launch_gemm<cf, 128, 128, 16, 2, 2, false, true>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
     launch_gemm<cd, 256, 32, 16, 4, 1>,       launch_gemm<cd, 64, 64, 16, 2, 2, true>,
-    launch_gemm<cd, 128, 64, 16, 2, 2, true>};
+    launch_gemm<cd, 128, 64, 16, 2, 2, true>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true>,
+    launch_gemm<cd, 128, 32, 16, 4, 1, false, true>};
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
-     "128x128x16/xcd"},
+     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
-     "128x64x16/xcd"}};
+     "128x64x16/xcd", "64x64x16/3m", "128x32x16/3m"}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);
@@ -317,7 +368,14 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
+    // QB_TUNE_SKIP: comma-separated variant indices the tuner leaves out
+    // (a diagnostic for comparing tile families in whole windows)
+    std::vector<char> skip(QB_GEMM_VARIANTS, 0);
+    if (const char* sk = getenv("QB_TUNE_SKIP"))
+      for (const char* c = sk; *c; ++c)
+        if (*c >= '0' && *c <= '9' && *c - '0' < QB_GEMM_VARIANTS) skip[*c - '0'] = 1;
     for (int v = 0; v < QB_GEMM_VARIANTS; ++v) {
+      if (skip[v]) continue;
       gemm_launch_variant<T>(ctx, v, M, N, K, A, M, (int64_t)M * K, B, K, (int64_t)K * N, C, M,
                              (int64_t)M * N, b);
       (void)hipEventRecord(e0, ctx->stream);
@@ -335,6 +393,7 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (ctx->gemm_pick[ty][qb_gemm_class(M, N, K)] != best) ++ctx->gemm_epoch;
     ctx->gemm_pick[ty][qb_gemm_class(M, N, K)] = best;
   }
   (void)hipStreamSynchronize(ctx->stream);
@@ -777,6 +836,46 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
   return QB_OK;
 }
 
+void qb_graphs_clear(qb_ctx* ctx) {
+  for (auto& g : ctx->graph_cache) (void)hipGraphExecDestroy(g.exec);
+  ctx->graph_cache.clear();
+}
+
+template <typename T>
+int qb_getrf_graph(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
+                   int* info, int batch) {
+  if (!ctx->graphs || ctx->prof || ctx->bix)
+    return qb_getrf_batched<T>(ctx, A, n, lda, sA, ipiv, sP, info, batch);
+  const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
+  for (auto& g : ctx->graph_cache)
+    if (g.ty == ty && g.A == (const void*)A && g.n == n && g.lda == lda && g.sA == sA &&
+        g.ipiv == ipiv && g.sP == sP && g.info == info && g.batch == batch &&
+        g.epoch == ctx->gemm_epoch) {
+      QB_HIP(ctx, hipGraphLaunch(g.exec, ctx->stream));
+      return QB_OK;
+    }
+  QB_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = qb_getrf_batched<T>(ctx, A, n, lda, sA, ipiv, sP, info, batch);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc || e != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (!rc) rc = qb_fail(ctx, QB_E_HIP, "LU graph capture failed");
+    return rc;
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  if (ctx->graph_cache.size() >= 16) {
+    (void)hipGraphExecDestroy(ctx->graph_cache.front().exec);
+    ctx->graph_cache.erase(ctx->graph_cache.begin());
+  }
+  ctx->graph_cache.push_back({ty, A, n, lda, sA, ipiv, sP, info, batch, ctx->gemm_epoch, exec});
+  QB_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+  return QB_OK;
+}
+
 // B <- U^{-1} L^{-1} B with the factors of qb_getrf_batched; the row
 // permutation of zgetrs is applied by the caller (fused into a gather).
 template <typename T>
@@ -1011,6 +1110,8 @@ int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev
 // explicit instantiations
 template int qb_getrf_batched<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int);
 template int qb_getrf_batched<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int);
+template int qb_getrf_graph<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int);
+template int qb_getrf_graph<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int);
 template int qb_trsm_lu_batched<cf>(qb_ctx*, const cf*, int, int64_t, int64_t, cf*, int, int64_t,
                                     int64_t, int);
 template int qb_trsm_lu_batched<cd>(qb_ctx*, const cd*, int, int64_t, int64_t, cd*, int, int64_t,

@@ -115,6 +115,45 @@ function construct_boundary_matrices!(Tbufs::Vector{Matrix{ComplexF64}}, solver,
     return nothing
 end
 
+# construct_boundary_matrices_with_derivatives! (cheb-CFIE:1507-1530,
+# cheb-DLP:1126-1144): T, dT/dk, d2T/dk2 in one launch.  Between boundary
+# components the default is the derivative of the matrix; reference_sign=true
+# reproduces the reference's sign of those terms (cheb-CFIE:1031-1032).
+function construct_boundary_matrices_with_derivatives!(Tbufs::Vector{Matrix{ComplexF64}},
+                                                       dTbufs::Vector{Matrix{ComplexF64}},
+                                                       ddTbufs::Vector{Matrix{ComplexF64}},
+                                                       solver, pts,
+                                                       zj::AbstractVector{ComplexF64};
+                                                       multithreaded::Bool=true,
+                                                       use_chebyshev::Bool=true,
+                                                       n_panels_h::Int=15000, M_h::Int=5,
+                                                       n_panels_j::Int=3000, M_j::Int=5,
+                                                       timeit::Bool=false,
+                                                       reference_sign::Bool=false)
+    use_chebyshev || error("Direct derivative matrix construction is only for real k currently")
+    Mk = length(zj)
+    @assert length(Tbufs) == Mk && length(dTbufs) == Mk && length(ddTbufs) == Mk
+    g = geom(pts)
+    try
+        N = Int(ccall((:qb_geom_size, LIB), Int64, (Ptr{Cvoid},), g))
+        outs = [Array{ComplexF64}(undef, N, N, Mk) for _ in 1:3]
+        cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
+        check(ccall((:qb_assemble_deriv, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Int64, Ref{ChebParams},
+                     Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int32, Ptr{Float64}),
+                    ctx(), g, opcode(solver), collect(zj), Int64(Mk), cp, outs[1], outs[2],
+                    outs[3], Int32(reference_sign ? 2 : 0), C_NULL))
+        for m in 1:Mk
+            copyto!(Tbufs[m], view(outs[1], :, :, m))
+            copyto!(dTbufs[m], view(outs[2], :, :, m))
+            copyto!(ddTbufs[m], view(outs[3], :, :, m))
+        end
+    finally
+        geom_destroy(g)
+    end
+    return nothing
+end
+
 # construct_B_matrix (beyn.jl:257-322): window on the device, SVD/eig on the host
 function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float64; nq::Int=64,
                             r::Int=48, svd_tol=1e-14, rng=QB.MersenneTwister(0),
@@ -217,6 +256,13 @@ function enable!()
     @eval QB function construct_B_matrix(solver::Union{CFIE_kress,DLP_kress}, pts, N::Int,
                                          k0::Complex{Float64}, R::Float64; kw...)
         return $(QBHip).construct_B_matrix(solver, pts, N, k0, R; kw...)
+    end
+    @eval QB function construct_boundary_matrices_with_derivatives!(
+            Tbufs::Vector{Matrix{ComplexF64}}, dTbufs::Vector{Matrix{ComplexF64}},
+            ddTbufs::Vector{Matrix{ComplexF64}}, solver::Union{CFIE_kress,DLP_kress}, pts,
+            zj::AbstractVector{ComplexF64}; kw...)
+        return $(QBHip).construct_boundary_matrices_with_derivatives!(Tbufs, dTbufs, ddTbufs,
+                                                                      solver, pts, zj; kw...)
     end
     @eval QB function residual_and_norm_select(solver::Union{CFIE_kress,DLP_kress},
                                                λ::AbstractVector{ComplexF64},

@@ -24,6 +24,7 @@ EXPORTS = (
     "qb_window_create", "qb_window_destroy", "qb_window_footprint", "qb_window_moments",
     "qb_bessel_eval", "qb_cheb_tune", "qb_ctx_set_profiling", "qb_ctx_kernel_stats",
     "qb_ctx_autotune", "qb_ctx_set_cache_limit", "qb_residuals", "qb_assemble_deriv",
+    "qb_ctx_mem_info", "qb_ctx_set_graphs",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
@@ -87,6 +88,8 @@ def lib():
         "qb_ctx_set_profiling": (C.c_int, [P, I32]),
         "qb_ctx_autotune": (C.c_int, [P, I64, I32, I32, I32, C.c_char_p, I64]),
         "qb_ctx_set_cache_limit": (C.c_int, [P, I64]),
+        "qb_ctx_mem_info": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64)]),
+        "qb_ctx_set_graphs": (C.c_int, [P, I32]),
         "qb_residuals": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P, P, P, P,
                                    P]),
         "qb_ctx_kernel_stats": (C.c_int, [P, I32, C.POINTER(D), C.POINTER(I64), C.POINTER(D),

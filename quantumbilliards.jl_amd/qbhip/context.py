@@ -34,6 +34,16 @@ class Context:
             out[name] = {"ms": ms.value, "launches": n.value, "flops": fl.value, "bytes": by.value}
         return out
 
+    def set_graphs(self, on: bool):
+        """Replay the batched LU of windows through cached hipGraphs."""
+        _lib.check(self.ptr, _lib.lib().qb_ctx_set_graphs(self.ptr, 1 if on else 0))
+
+    def mem_info(self):
+        """(free, total) device bytes of this context's GPU."""
+        f, t = C.c_int64(), C.c_int64()
+        _lib.check(self.ptr, _lib.lib().qb_ctx_mem_info(self.ptr, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     def set_cache_limit(self, nbytes: int):
         """Bytes of freed device buffers kept for reuse (0 disables caching)."""
         _lib.check(self.ptr, _lib.lib().qb_ctx_set_cache_limit(self.ptr, int(nbytes)))

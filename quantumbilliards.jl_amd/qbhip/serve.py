@@ -7,7 +7,11 @@ streams), so windows admitted together overlap on the GPU.  A request is
 admitted when a slot is free and its HBM footprint (the N^2 matrices of all
 live contour nodes plus the N x r solve buffers, `qbhip.footprint`) fits in
 the budget next to the windows already running; requests that do not fit
-wait while smaller ones behind them may start (first fit).  The host part of
+wait while smaller ones behind them may start (first fit).  A mixed-precision
+request larger than the whole budget runs in FP64 when that footprint (16
+instead of 24 bytes per matrix entry) fits; one that fits in neither is
+rejected with MemoryError.  The default budget is the device's free memory
+at start-up less 12 GiB.  The host part of
 each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
 windows' device work.  Latency is measured from arrival to result.
 
@@ -94,7 +98,10 @@ class Server:
             # so the footprint accounting is the memory actually held
             c.set_cache_limit(0)
         self.free_slots = list(range(slots))
-        self.budget = int(hbm_budget if hbm_budget is not None else 200 * 2 ** 30)
+        if hbm_budget is None:
+            free, _ = self.slots[0].mem_info()
+            hbm_budget = max(free - 12 * 2 ** 30, 1 << 30)
+        self.budget = int(hbm_budget)
         self.live_bytes = 0
         self.lock = threading.Condition()
         self.pending: List = []          # (req, future, t_arrival)
@@ -117,8 +124,10 @@ class Server:
             self.lock.notify_all()
         return fut
 
-    def run_stream(self, requests: List[WindowRequest]) -> List[WindowResult]:
-        """Submit requests at their arrival offsets and wait for all."""
+    def run_stream(self, requests: List[WindowRequest]) -> List:
+        """Submit requests at their arrival offsets and wait for all.  Returns
+        one entry per request in arrival order: its WindowResult, or the
+        exception it ended with (e.g. MemoryError for a window too large)."""
         t0 = time.perf_counter()
         futs = []
         for req in sorted(requests, key=lambda r: r.arrival):
@@ -126,7 +135,13 @@ class Server:
             if dt > 0:
                 time.sleep(dt)
             futs.append(self.submit(req, t0 + req.arrival))
-        return [f.result() for f in futs]
+        out = []
+        for f in futs:
+            try:
+                out.append(f.result())
+            except Exception as e:  # noqa: BLE001 - reported per request
+                out.append(e)
+        return out
 
     def close(self):
         with self.lock:
@@ -151,8 +166,13 @@ class Server:
         r = probe_count(b, req.k0, req.R, req.nq, req.svd_tol)
         N = int(sum(Ns))
         r = min(r, N)
-        fp = footprint(N, req.nq, req.precision, r)
-        return b, Ns, N, r, fp
+        prec = req.precision
+        fp = footprint(N, req.nq, prec, r)
+        if fp > self.budget and prec == "mixed":
+            fp64 = footprint(N, req.nq, "fp64", r)
+            if fp64 <= self.budget:
+                prec, fp = "fp64", fp64
+        return b, Ns, N, r, fp, prec
 
     def _dispatch(self):
         while True:
@@ -188,7 +208,7 @@ class Server:
             t.start()
 
     def _run(self, slot, req, fut, t_arrival, prep):
-        b, Ns, N, r, fp = prep
+        b, Ns, N, r, fp, prec = prep
         t_admit = time.perf_counter()
         try:
             ctx = self.slots[slot]
@@ -205,11 +225,11 @@ class Server:
                                                      geom=geom)
                 self.cheb_cache[ck] = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
             cheb = self.cheb_cache[ck]
-            win = Window(solver, pts, req.k0, req.R, req.nq, precision=req.precision, geom=geom,
-                         **cheb)
+            win = Window(solver, pts, req.k0, req.R, req.nq, precision=prec, geom=geom, **cheb)
             V = probe_block(np.random.default_rng(req.seed), N, r)
             A0, A1 = win.moments(V, max(req.tol, 1e-13))
             st = dict(win.stats)
+            st["precision_used"] = prec
             win.close()
         except Exception as e:  # release and report
             self._release(slot, fp)
@@ -239,12 +259,16 @@ class Server:
             self.lock.notify_all()
 
 
-def summarize(results: List[WindowResult], t_span: Optional[float] = None) -> dict:
+def summarize(results: List, t_span: Optional[float] = None) -> dict:
+    """p50/p99 latency and eigenvalues/s over the completed windows; entries
+    that are exceptions (rejected requests) are counted separately."""
+    failed = [r for r in results if not isinstance(r, WindowResult)]
+    results = [r for r in results if isinstance(r, WindowResult)]
     lat = np.array([r.latency for r in results])
     n_eig = int(sum(len(r.inside) for r in results))
     if t_span is None:
         t_span = max(r.t_done for r in results) - min(r.t_arrival for r in results)
-    return {"windows": len(results), "eigenvalues": n_eig,
+    return {"windows": len(results), "rejected": len(failed), "eigenvalues": n_eig,
             "eigenvalues_per_s": n_eig / t_span if t_span > 0 else None,
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),

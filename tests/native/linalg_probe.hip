@@ -143,4 +143,5 @@ extern "C" double qbt_copy_bench(qb_ctx* ctx, int64_t bytes, int iters) {
 // the default choice), so tests can exercise each variant.
 extern "C" void qbt_force_variant(qb_ctx* ctx, int ty, int v) {
   for (int k = 0; k < QB_GEMM_CLASSES; ++k) ctx->gemm_pick[ty][k] = v;
+  ++ctx->gemm_epoch;
 }

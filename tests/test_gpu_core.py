@@ -217,3 +217,32 @@ def test_dlp_derivative_assembly_matches_oracle():
         e = _relerr(got, ref)
         print(e.max(axis=0))
         assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-10
+
+
+def test_lu_graph_replay_matches_direct_launches():
+    """With hipGraph replay on, the first window of a shape captures the
+    batched LU and later windows replay it; every replay (with eager work in
+    between) gives bitwise the moments of direct launches, FP64 and mixed."""
+    q = qb()
+    b, pts = _pts("star", [320])
+    solver = q.CFIE_kress(10.0, b)
+    cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
+    V = q.probe_block(np.random.default_rng(5), 320, 16)
+    ctx = q.Context(0)
+    geom = q.Geometry(pts, ctx)
+    precs = ("fp64", "mixed")
+    ref = {}
+    for p in precs:
+        w = q.Window(solver, pts, 30.0 + 0j, 0.4, 6, precision=p, geom=geom, **cp)
+        ref[p] = w.moments(V, 1e-13)
+        w.close()
+    ctx.set_graphs(True)
+    for rep in range(3):
+        for p in precs:
+            w = q.Window(solver, pts, 30.0 + 0j, 0.4, 6, precision=p, geom=geom, **cp)
+            A0, A1 = w.moments(V, 1e-13)
+            w.close()
+            assert np.array_equal(A0, ref[p][0]) and np.array_equal(A1, ref[p][1]), (rep, p)
+    ctx.set_graphs(False)
+    geom.close()
+    ctx.close()

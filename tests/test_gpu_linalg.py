@@ -92,12 +92,13 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(7))
+@pytest.mark.parametrize("variant", range(9))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
     LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
     are multiples of 8 with a partial group of m-tiles, the case in which the
-    XCD-aware variants (5, 6) remap tiles."""
+    XCD-aware variants (5, 6) remap tiles; variants 7 and 8 form complex
+    products from three real ones (Gauss)."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:

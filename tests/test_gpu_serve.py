@@ -64,3 +64,31 @@ def test_admission_by_footprint():
     with pytest.raises(MemoryError):
         srv.submit(reqs[0]).result()
     srv.close()
+
+
+def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
+    """A mixed-precision window whose footprint exceeds the whole budget runs
+    in FP64 (16 instead of 24 bytes per matrix entry) when that fits, with
+    the same eigenvalues; the per-request outcome of a stream includes the
+    rejection of a window that fits in neither."""
+    from qbhip.beyn import footprint
+    from qbhip.serve import Server, WindowRequest, summarize
+    req = _reqs()[0]
+    srv0 = Server(slots=1)
+    b, Ns, N, r, fp_mixed, prec = srv0._prepare(req)
+    ref = srv0.submit(req).result()
+    srv0.close()
+    assert prec == "mixed"
+    fp64 = footprint(N, req.nq, "fp64", r)
+    assert fp64 < fp_mixed
+    srv = Server(slots=1, hbm_budget=(fp64 + fp_mixed) // 2)
+    got = srv.submit(req).result()
+    assert got.stats["precision_used"] == "fp64"
+    assert len(got.inside) == len(ref.inside)
+    assert np.max(np.abs(got.inside - ref.inside) / np.abs(ref.inside)) <= 1e-10
+    big = WindowRequest(9, "circle", 60.0, 0.5, nq=24, ppw=40)
+    out = srv.run_stream([req, big])
+    srv.close()
+    assert isinstance(out[1], MemoryError)
+    s = summarize(out)
+    assert s["windows"] == 1 and s["rejected"] == 1
