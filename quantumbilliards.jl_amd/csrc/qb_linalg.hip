@@ -64,8 +64,8 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
 // while the current one is multiplied out of LDS; the two LDS buffers
 // alternate, so each k-tile costs one barrier.  A is stored k-major
 // ([k][m], row pad of 16 words puts the four k rows a wave reads in
-// different bank halves), B n-major ([n][k], pad 1) so that both the
-// coalesced global loads and the fragment reads are conflict-free.  A is
+// different bank quarters), B n-major ([n][k], row stride below) so that
+// both the coalesced global loads and the fragment reads are conflict-free.  A is
 // negated when it is written to LDS, so the accumulators hold -A*B and the
 // epilogue adds them to C.
 template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD, bool G3>
@@ -81,7 +81,11 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int APAD = (sizeof(S) == 4) ? 16 : 8;
   constexpr int AST = BM + APAD;       // A row stride (elements)
-  constexpr int BST = BK + 1;          // B row stride
+  // B row stride: the f32 fragment read (li * BST + lk, li < 16, lk < 4)
+  // maps the 64 lanes of a wave onto 64 distinct 4-byte banks when BST is 4
+  // times an odd number (BK + 4 for BK = 16, 32); BK + 1 wraps lane 15 onto
+  // lane 0.
+  constexpr int BST = (sizeof(S) == 4) ? BK + 4 : BK + 1;
   constexpr int NA = BM * BK / 256;    // complex elements of A per thread per tile
   constexpr int NB = BN * BK / 256;
   static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "tile/thread mismatch");
@@ -291,12 +295,12 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
     launch_gemm<cd, 256, 32, 16, 4, 1>,       launch_gemm<cd, 64, 64, 16, 2, 2, true>,
     launch_gemm<cd, 128, 64, 16, 2, 2, true>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true>,
-    launch_gemm<cd, 128, 32, 16, 4, 1, false, true>};
+    launch_gemm<cd, 64, 32, 16, 2, 2, false, true>};
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
-     "128x64x16/xcd", "64x64x16/3m", "128x32x16/3m"}};
+     "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m"}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);

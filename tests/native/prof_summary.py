@@ -4,6 +4,7 @@ profiles/.
   prof_summary.py stats <dir>/<prefix>          kernel-trace run of bench.py
   prof_summary.py pmc <fetch_csv> <write_csv>   FETCH_SIZE / WRITE_SIZE runs
   prof_summary.py pmcwin <fetch_csv> <write_csv>  the same over one Beyn window
+  prof_summary.py mfma <csv>                    MFMA busy, LDS conflicts, waits per GEMM
 
 `stats` prints the whole-run per-kernel table and, from the trace, the last
 Beyn window alone (the dispatches from the last table build on), which is the
@@ -114,8 +115,31 @@ def pmcwin(fetch_csv, write_csv):
         print(f"{n:8d} {f / 1e9:9.3f} {w / 1e9:9.3f} {(f + w) / 1e6 / max(n, 1):10.3f}  {k}")
 
 
+def mfma(path, simds=1024, xcds=8):
+    """One counter run over pmc_probe.py: SQ_VALU_MFMA_BUSY_CYCLES,
+    SQ_LDS_BANK_CONFLICT, SQ_LDS_IDX_ACTIVE, SQ_WAIT_INST_ANY, SQ_WAVE_CYCLES,
+    GRBM_GUI_ACTIVE.  GRBM_GUI_ACTIVE is summed over the 8 XCDs, so the
+    kernel's active cycles are that sum / 8; MFMA busy = busy cycles /
+    (active cycles x 1024 SIMDs)."""
+    agg = collections.OrderedDict()
+    for r in csv.DictReader(open(path)):
+        k = (int(r["Dispatch_Id"]), _short(r["Kernel_Name"]))
+        agg.setdefault(k, {})[r["Counter_Name"]] = float(r["Counter_Value"])
+    print(f"{'dispatch':>8} {'MFMA busy':>9} {'LDS confl':>9} {'wait/wave':>9}  kernel")
+    for (d, k), v in agg.items():
+        if not k.startswith("k_gemm_sub"):
+            continue
+        act = v["GRBM_GUI_ACTIVE"] / xcds
+        busy = v["SQ_VALU_MFMA_BUSY_CYCLES"] / (act * simds)
+        lds = v["SQ_LDS_BANK_CONFLICT"] / max(v["SQ_LDS_IDX_ACTIVE"], 1.0)
+        wait = v["SQ_WAIT_INST_ANY"] / max(v["SQ_WAVE_CYCLES"], 1.0)
+        print(f"{d:8d} {100 * busy:8.1f}% {100 * lds:8.1f}% {100 * wait:8.1f}%  {k}")
+
+
 if __name__ == "__main__":
-    if sys.argv[1] == "stats":
+    if sys.argv[1] == "mfma":
+        mfma(sys.argv[2])
+    elif sys.argv[1] == "stats":
         stats(sys.argv[2])
     elif sys.argv[1] == "pmcwin":
         pmcwin(sys.argv[2], sys.argv[3])
