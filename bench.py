@@ -167,10 +167,12 @@ def q_probe(N, r):
 
 
 def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
-    """Device windows back to back on this thread; the host SVD / B / eig of
-    window s runs on a second thread while the device works on window s+1
-    (NumPy's LAPACK and the ctypes calls release the GIL).  Returns per-window
-    results and the span from the first window's start to the last result."""
+    """Device windows back to back on this thread (moments reduced on the
+    device to R and C = Q^H A1, qb_window_moments_qr); the host SVD of R, B
+    and eig(B) of window s run on a second thread while the device works on
+    window s+1 (NumPy's LAPACK and the ctypes calls release the GIL).  Returns
+    per-window results and the span from the first window's start to the last
+    result."""
     import queue
     import threading
     qu = queue.Queue()
@@ -191,9 +193,9 @@ def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
                 if lim is not None:
                     lim.restore_original_limits()
                 return
-            s, A0, A1, t_start, t_dev, st = item
+            s, Rm, Cm, t_start, t_dev, st = item
             th = time.perf_counter()
-            B, Uk, S, rk = q.B_from_moments(A0, A1, a.svd_tol)
+            B, Ur, S, rk = q.B_from_qr(Rm, Cm, a.svd_tol)
             lam = np.linalg.eigvals(B) if B.size else np.zeros(0, complex)
             t_end = time.perf_counter()
             res[s] = (lam, rk, t_start, t_end, t_dev, t_end - th, st)
@@ -205,10 +207,10 @@ def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
         t_start = time.perf_counter()
         win = q.Window(solver, pts, complex(a.k0), a.R, a.nq, precision=precision, geom=geom,
                        **cheb)
-        A0, A1 = win.moments(V, a.refine_tol)
+        Rm, Cm = win.moments_qr(V, a.refine_tol)
         st = dict(win.stats)
         win.close()
-        qu.put((s, A0, A1, t_start, time.perf_counter() - t_start, st))
+        qu.put((s, Rm, Cm, t_start, time.perf_counter() - t_start, st))
     qu.put(None)
     hth.join()
     return res, res[-1][3] - t_begin

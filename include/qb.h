@@ -189,6 +189,19 @@ int qb_residuals(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* lam, i
                  const qb_cheb_params* cheb, const double* phi, int32_t batch, double* res2,
                  double* resn1, double* resninf, double* opn1, double* opninf);
 
+/* qb_window_moments followed by a Householder QR of A0 on the device: the
+ * N x r blocks stay there, A0 = Q R, and only R (r x r, upper triangular) and
+ * C = (Q^H A1)[:r, :] (r x r) are returned, column-major.  svd(R) = U_R S W^H
+ * gives the singular values and W of svd(A0) (beyn.jl:286) and
+ * B = U_R[:, :k]^H C W[:, :k] S_k^{-1} (beyn.jl:310-321) without moving A0, A1
+ * to the host.  The reflectors are kept for qb_window_basis. */
+int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, double* R, double* C,
+                         double refine_tol, qb_stats* stats);
+
+/* U_k = Q [Ur; 0] (host N x k) from the last qb_window_moments_qr, Ur host
+ * r x k: the U_k that construct_B_matrix returns (beyn.jl:311). */
+int qb_window_basis(qb_window* w, const double* Ur, int32_t k, double* U);
+
 /* Device evaluation of H0^(1), H1^(1), J0, J1 at n complex points (the values
  * the reference takes from SpecialFunctions.besselh/besselj when sampling its
  * tables, chebyshev_bessels.jl:331-372).  Host arrays of n complex each. */

@@ -701,6 +701,10 @@ struct qb_window {
   int* fb_ipiv = nullptr;
   int* fb_perm = nullptr;
   size_t fb_cap = 0;
+  cd* qr = nullptr;      // [A0 | A1] after qb_window_moments_qr (N x 2r)
+  cd* qr_tau = nullptr;  // [r]
+  size_t qr_cap = 0;     // elements of qr
+  int qr_rcap = 0, qr_r = 0;
   qb_stats st{};
 };
 
@@ -716,8 +720,9 @@ extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision,
   size_t b = win_bytes_core(N, nq, precision);
   size_t nv = (size_t)N * r;
   // moment work buffers: X and R (complex128) per node, a complex64 copy
-  // per node in mixed mode, V, A0, A1
-  b += 2 * (size_t)nq * nv * sizeof(cd) + 3 * nv * sizeof(cd);
+  // per node in mixed mode, V, A0, A1, and the QR block [A0 | A1] that
+  // qb_window_moments_qr keeps
+  b += 2 * (size_t)nq * nv * sizeof(cd) + 5 * nv * sizeof(cd);
   if (precision == QB_MIXED) b += (size_t)nq * nv * sizeof(cf);
   // Chebyshev tables, bounded by 1.2e5 panels of degree 7 per node
   b += (size_t)nq * (32u << 20);
@@ -739,6 +744,8 @@ static void win_free(qb_window* w) {
   qb_dev_free(ctx, w->fb_lu, w->fb_cap * NN * sizeof(cd));
   qb_dev_free(ctx, w->fb_ipiv, w->fb_cap * w->N * sizeof(int));
   qb_dev_free(ctx, w->fb_perm, w->fb_cap * w->N * sizeof(int));
+  qb_dev_free(ctx, w->qr, w->qr_cap * sizeof(cd));
+  qb_dev_free(ctx, w->qr_tau, (size_t)w->qr_rcap * sizeof(cd));
 }
 
 extern "C" void qb_window_destroy(qb_window* w) {
@@ -919,11 +926,14 @@ static int win_fp64_fallback(qb_window* w, const std::vector<int>& nodes) {
   return QB_OK;
 }
 
-extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0, double* A1,
-                                 double refine_tol, qb_stats* stats) {
+// The moment loop.  With dev_out == nullptr, A0 and A1 go to the host
+// arrays; otherwise they stay on the device at dev_out (A0) and
+// dev_out + N r (A1) and the host arrays are not touched.
+static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, double* A1,
+                       double refine_tol, qb_stats* stats, cd* dev_out) {
   if (!w) return QB_E_ARG;
   qb_ctx* ctx = w->ctx;
-  if (!V || !A0 || !A1) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (!V || (!dev_out && (!A0 || !A1))) return qb_fail(ctx, QB_E_ARG, "null argument");
   if (r < 1 || r > w->N) return qb_fail(ctx, QB_E_ARG, "r must be in [1, N]");
   (void)hipSetDevice(ctx->device);
   const int64_t N = w->N;
@@ -940,8 +950,13 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
   std::vector<double> h3(3 * nq);
   QB_TRY(qb_dev_alloc(ctx, (void**)&Vd, nv * sizeof(cd)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&X, (size_t)nq * nv * sizeof(cd)));
-  QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, nv * sizeof(cd)));
-  QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, nv * sizeof(cd)));
+  if (dev_out) {
+    A0d = dev_out;
+    A1d = dev_out + nv;
+  } else {
+    QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, nv * sizeof(cd)));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, nv * sizeof(cd)));
+  }
   if (hipMemcpyAsync(Vd, V, nv * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "V upload failed");
     goto fail;
@@ -1088,10 +1103,15 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
     w->st.nodes_fp64_fallback = (int)w->fb_nodes.size();
   }
   QB_TRY(qb_moments(ctx, X, nv, nq, w->w_dev, w->w_dev + nq, A0d, A1d));
-  if (hipMemcpyAsync(A0, A0d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-      hipMemcpyAsync(A1, A1d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-      hipStreamSynchronize(ctx->stream) != hipSuccess) {
+  if (!dev_out &&
+      (hipMemcpyAsync(A0, A0d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+       hipMemcpyAsync(A1, A1d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) !=
+           hipSuccess)) {
     rc = qb_fail(ctx, QB_E_HIP, "moment download failed");
+    goto fail;
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    rc = qb_fail(ctx, QB_E_HIP, "moments failed");
     goto fail;
   }
   w->st.r = r;
@@ -1107,8 +1127,94 @@ fail:
   qb_dev_free(ctx, X, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, D32, (size_t)nq * nv * sizeof(cf));
-  qb_dev_free(ctx, A0d, nv * sizeof(cd));
-  qb_dev_free(ctx, A1d, nv * sizeof(cd));
+  if (!dev_out) {
+    qb_dev_free(ctx, A0d, nv * sizeof(cd));
+    qb_dev_free(ctx, A1d, nv * sizeof(cd));
+  }
+  return rc;
+}
+
+extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0, double* A1,
+                                 double refine_tol, qb_stats* stats) {
+  return win_moments(w, r, V, A0, A1, refine_tol, stats, nullptr);
+}
+
+// A0 and A1 stay on the device in w->qr ([A0 | A1], N x 2r); A0 = Q R by
+// Householder reflections, Q^H applied to A1; R (r x r upper triangular) and
+// C = (Q^H A1)[:r, :] are returned, the reflectors kept for qb_window_basis.
+extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, double* R,
+                                    double* Cm, double refine_tol, qb_stats* stats) {
+  if (!w) return QB_E_ARG;
+  qb_ctx* ctx = w->ctx;
+  if (!V || !R || !Cm) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (r < 1 || r > w->N) return qb_fail(ctx, QB_E_ARG, "r must be in [1, N]");
+  (void)hipSetDevice(ctx->device);
+  const int64_t N = w->N;
+  const size_t need = (size_t)2 * N * r;
+  if (w->qr_cap != need) {
+    qb_dev_free(ctx, w->qr, w->qr_cap * sizeof(cd));
+    qb_dev_free(ctx, w->qr_tau, (size_t)w->qr_rcap * sizeof(cd));
+    w->qr = nullptr;
+    w->qr_tau = nullptr;
+    w->qr_cap = 0;
+    w->qr_rcap = 0;
+    int rc = qb_dev_alloc(ctx, (void**)&w->qr, need * sizeof(cd));
+    if (!rc) rc = qb_dev_alloc(ctx, (void**)&w->qr_tau, (size_t)r * sizeof(cd));
+    if (rc) {
+      qb_dev_free(ctx, w->qr, need * sizeof(cd));
+      w->qr = nullptr;
+      return rc;
+    }
+    w->qr_cap = need;
+    w->qr_rcap = r;
+  }
+  w->qr_r = 0;
+  int rc = win_moments(w, r, V, nullptr, nullptr, refine_tol, nullptr, w->qr);
+  if (rc) return rc;
+  rc = qb_house_qr(ctx, w->qr, N, r, 2 * r, w->qr_tau);
+  if (rc) return rc;
+  const cd* Cd = w->qr + (size_t)N * r;
+  hipError_t e = hipMemcpy2DAsync(R, r * sizeof(cd), w->qr, N * sizeof(cd), r * sizeof(cd), r,
+                                  hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(Cm, r * sizeof(cd), Cd, N * sizeof(cd), r * sizeof(cd), r,
+                         hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  // R: zero the reflectors stored below the diagonal
+  for (int c = 0; c < r; ++c)
+    for (int i = c + 1; i < r; ++i) R[2 * ((size_t)c * r + i)] = R[2 * ((size_t)c * r + i) + 1] = 0;
+  w->qr_r = r;
+  if (stats) *stats = w->st;
+  return QB_OK;
+}
+
+// U = Q [Ur; 0] (N x k) with the reflectors of the last qb_window_moments_qr.
+extern "C" int qb_window_basis(qb_window* w, const double* Ur, int32_t k, double* U) {
+  if (!w) return QB_E_ARG;
+  qb_ctx* ctx = w->ctx;
+  if (!U || (k > 0 && !Ur) || k < 0) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (w->qr_r == 0) return qb_fail(ctx, QB_E_STATE, "no QR of the moments on this window");
+  if (k > w->qr_r) return qb_fail(ctx, QB_E_ARG, "k must be <= r");
+  if (k == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  const int64_t N = w->N;
+  const int r = w->qr_r;
+  const size_t bytes = (size_t)N * k * sizeof(cd);
+  cd* Y = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&Y, bytes);
+  if (rc) return rc;
+  hipError_t e = hipMemsetAsync(Y, 0, bytes, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy2DAsync(Y, N * sizeof(cd), Ur, r * sizeof(cd), r * sizeof(cd), k,
+                         hipMemcpyHostToDevice, ctx->stream);
+  if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  if (!rc) rc = qb_house_apply_q(ctx, w->qr, N, r, w->qr_tau, Y, k);
+  if (!rc && (hipMemcpyAsync(U, Y, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+              hipStreamSynchronize(ctx->stream) != hipSuccess))
+    rc = qb_fail(ctx, QB_E_HIP, "basis download failed");
+  (void)hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, Y, bytes);
   return rc;
 }
 

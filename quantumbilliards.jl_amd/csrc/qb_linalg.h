@@ -56,3 +56,11 @@ const char* qb_gemm_variant_name(int ty, int v);
 // colsum[b][j] of |T|.
 int qb_gemv_norms(qb_ctx* ctx, const cd* T, const cd* phi, int64_t N, int nb, int nsplit,
                   cd* part, double* rowabs, double* colsum);
+
+// Householder QR (qb_qr.hip): the first r columns of the N x ncols block A
+// (ld = N) are factored in place (R on and above the diagonal, reflectors
+// below, tau[r]) and H_j^H is applied to all columns right of each j.
+int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau);
+// Y <- Q Y for the N x ncols block Y (ld = N), Q = H_0 ... H_{r-1} from V.
+int qb_house_apply_q(qb_ctx* ctx, const cd* V, int64_t N, int r, const cd* tau, cd* Y,
+                     int ncols);

@@ -13,7 +13,7 @@ from .geometry import (Billiard, BoundaryPointsCFIE, CircularHoleBilliard, evalu
 from .solvers import (CFIE_kress, DLP_kress, Geometry, assemble, assemble_with_derivatives,
                       bessel_eval, boundary_matrix_size, chebyshev_params,
                       construct_boundary_matrices, construct_boundary_matrices_with_derivatives)
-from .beyn import (B_from_moments, Window, construct_B_matrix, footprint, probe_block, probe_count, solve,
+from .beyn import (B_from_moments, B_from_qr, Window, construct_B_matrix, footprint, probe_block, probe_count, solve,
                    solve_vect, weyl_count)
 from .spectrum import (beyn_disks_from_windows, compute_spectrum_beyn, delta_weyl,
                        imag_k_check_EXPERIMENTAL, plan_weyl_windows, residual_and_norm_select,

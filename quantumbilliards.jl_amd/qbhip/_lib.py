@@ -24,7 +24,7 @@ EXPORTS = (
     "qb_window_create", "qb_window_destroy", "qb_window_footprint", "qb_window_moments",
     "qb_bessel_eval", "qb_cheb_tune", "qb_ctx_set_profiling", "qb_ctx_kernel_stats",
     "qb_ctx_autotune", "qb_ctx_set_cache_limit", "qb_residuals", "qb_assemble_deriv",
-    "qb_ctx_mem_info", "qb_ctx_set_graphs",
+    "qb_ctx_mem_info", "qb_ctx_set_graphs", "qb_window_moments_qr", "qb_window_basis",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
@@ -35,6 +35,12 @@ class QBError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"libqbhip error {code}: {msg}")
         self.code = code
+
+
+class QBArgError(QBError, AssertionError):
+    """QB_E_ARG: an argument the library rejects (sizes, ranges, operator,
+    geometry kind).  Also an AssertionError, the exception the reference
+    raises for the same checks (cheb-CFIE:1433, cheb-DLP:1117)."""
 
 
 class ChebParams(C.Structure):
@@ -83,6 +89,8 @@ def lib():
         "qb_window_destroy": (None, [P]),
         "qb_window_footprint": (I64, [I64, I32, I32, I32]),
         "qb_window_moments": (C.c_int, [P, I32, P, P, P, D, C.POINTER(Stats)]),
+        "qb_window_moments_qr": (C.c_int, [P, I32, P, P, P, D, C.POINTER(Stats)]),
+        "qb_window_basis": (C.c_int, [P, P, I32, P]),
         "qb_bessel_eval": (C.c_int, [P, P, I64, P, P, P, P]),
         "qb_cheb_tune": (C.c_int, [P, P, I32, P, I64, D, I32, I32, C.POINTER(ChebParams), P]),
         "qb_ctx_set_profiling": (C.c_int, [P, I32]),
@@ -106,4 +114,5 @@ def lib():
 def check(ctx_ptr, rc: int):
     if rc != QB_OK:
         msg = lib().qb_last_error(ctx_ptr)
-        raise QBError(rc, msg.decode() if msg else "")
+        cls = QBArgError if rc == QB_E_ARG else QBError
+        raise cls(rc, msg.decode() if msg else "")

@@ -80,6 +80,32 @@ class Window:
         self.stats = st.as_dict()
         return A0, A1
 
+    def moments_qr(self, V: np.ndarray, refine_tol: float = 1e-13):
+        """Moments kept on the device and reduced there: A0 = Q R; returns
+        (R, C) with C = (Q^H A1)[:r, :], both r x r (qb_window_moments_qr)."""
+        V = np.asfortranarray(V, dtype=np.complex128)
+        assert V.shape[0] == self.N
+        r = V.shape[1]
+        R = np.empty((r, r), dtype=np.complex128, order="F")
+        Cm = np.empty((r, r), dtype=np.complex128, order="F")
+        st = _lib.Stats()
+        rc = _lib.lib().qb_window_moments_qr(self.ptr, int(r), V.ctypes.data, R.ctypes.data,
+                                             Cm.ctypes.data, float(refine_tol), C.byref(st))
+        _lib.check(self.ctx.ptr, rc)
+        self.stats = st.as_dict()
+        return R, Cm
+
+    def basis(self, Ur: np.ndarray) -> np.ndarray:
+        """U = Q [Ur; 0] (N x k) from the last moments_qr (qb_window_basis)."""
+        Ur = np.asfortranarray(Ur, dtype=np.complex128)
+        k = Ur.shape[1]
+        U = np.empty((self.N, k), dtype=np.complex128, order="F")
+        if k == 0:
+            return U
+        rc = _lib.lib().qb_window_basis(self.ptr, Ur.ctypes.data, int(k), U.ctypes.data)
+        _lib.check(self.ctx.ptr, rc)
+        return U
+
     def close(self):
         if getattr(self, "ptr", None):
             _lib.lib().qb_window_destroy(self.ptr)
@@ -134,6 +160,21 @@ def B_from_moments(A0: np.ndarray, A1: np.ndarray, svd_tol: float):
     return B, Uk, S, rk
 
 
+def B_from_qr(Rm: np.ndarray, Cm: np.ndarray, svd_tol: float):
+    """B from the device QR of the moments (A0 = Q Rm, Cm = (Q^H A1)[:r]):
+    svd(Rm) = U_R S W^H has the singular values and W of svd(A0), and
+    B = U_R[:, :k]^H Cm W[:, :k] S_k^{-1} equals U_k^H A1 W_k S_k^{-1}
+    (beyn.jl:286-321).  Returns (B, U_R[:, :k], S, rk); U_k = Q U_R[:, :k]
+    comes from Window.basis."""
+    Ur, S, Wh = np.linalg.svd(Rm)
+    rk = int(np.count_nonzero(S >= svd_tol))
+    if rk == 0:
+        return np.zeros((0, 0), np.complex128), Ur[:, :0], S, 0
+    Wk = Wh.conj().T[:, :rk]
+    B = Ur[:, :rk].conj().T @ ((Cm @ Wk) / S[:rk][None, :])
+    return B, Ur[:, :rk], S, rk
+
+
 def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 64, r: int = 48,
                        svd_tol: float = 1e-14, rng: Optional[np.random.Generator] = None,
                        use_chebyshev: bool = True, n_panels_h: int = 15000, M_h: int = 5,
@@ -141,9 +182,12 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
                        multithreaded: bool = True, precision: str = "fp64",
                        refine_tol: float = 1e-13, grow_r: bool = False,
                        V: Optional[np.ndarray] = None, window: Optional[Window] = None,
-                       return_info: bool = False):
+                       return_info: bool = False, device_qr: bool = True):
     """beyn.jl:257-322.  Returns (B, Uk) (and a diagnostics dict if
-    return_info)."""
+    return_info).  device_qr: reduce the moments on the device (A0 = Q R,
+    qb_window_moments_qr) and take the SVD of the r x r R on the host, instead
+    of moving the N x r A0, A1 to the host for their SVD; same singular
+    values, B and U_k up to rounding."""
     if not use_chebyshev:
         raise RuntimeError("only the Chebyshev path exists for complex contour nodes")
     rng = rng if rng is not None else np.random.default_rng(0)
@@ -153,13 +197,30 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
     win = window or Window(solver, pts, k0, R, nq, n_panels_h=n_panels_h, M_h=M_h,
                            n_panels_j=n_panels_j, M_j=M_j, precision=precision)
     diag = {"window": dict(win.stats)}
-    try:
-        Vb = V if V is not None else probe_block(rng, N, r)
-        r = Vb.shape[1]
+
+    def reduce(Vb):
+        # -> (rk, S, finish) with finish() -> (B, Uk)
+        if device_qr:
+            Rm, Cm = win.moments_qr(Vb, refine_tol)
+            th = time.perf_counter()
+            B, Ur, S, rk = B_from_qr(Rm, Cm, svd_tol)
+            dt = time.perf_counter() - th
+            return rk, S, dt, lambda: (B, win.basis(Ur))
         A0, A1 = win.moments(Vb, refine_tol)
         th = time.perf_counter()
         U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
-        t_svd = time.perf_counter() - th
+        dt = time.perf_counter() - th
+
+        def finish():
+            Uk = U[:, :rk]
+            Wk = Wh.conj().T[:, :rk]
+            return Uk.conj().T @ ((A1 @ Wk) / S[:rk][None, :]), Uk
+        return rk, S, dt, finish
+
+    try:
+        Vb = V if V is not None else probe_block(rng, N, r)
+        r = Vb.shape[1]
+        rk, S, t_svd, finish = reduce(Vb)
         r_used = r
         if rk == 0:
             diag.update(rk=0, r=r_used, sigma=S, seconds=time.perf_counter() - t0)
@@ -169,19 +230,14 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
             r_tmp = r + r
             while r_tmp < N:
                 Vb = probe_block(rng, N, r_tmp)
-                A0, A1 = win.moments(Vb, refine_tol)
-                th = time.perf_counter()
-                U, S, Wh, rk = _svd_B(A0, A1, svd_tol)
-                t_svd += time.perf_counter() - th
+                rk, S, dt, finish = reduce(Vb)
+                t_svd += dt
                 r_used = r_tmp
                 if rk < r_tmp or not grow_r:
                     break
                 r_tmp += r_tmp
         th = time.perf_counter()
-        Uk = U[:, :rk]
-        Wk = Wh.conj().T[:, :rk]
-        tmp = (A1 @ Wk) / S[:rk][None, :]
-        B = Uk.conj().T @ tmp
+        B, Uk = finish()
         t_svd += time.perf_counter() - th
         diag.update(rk=rk, r=r_used, sigma=S, moments=dict(win.stats), t_host_svd_s=t_svd,
                     seconds=time.perf_counter() - t0)

@@ -31,7 +31,7 @@ from typing import Dict, List, Optional
 import numpy as np
 
 from . import geometry as G
-from .beyn import B_from_moments, Window, footprint, probe_block, probe_count
+from .beyn import B_from_qr, Window, footprint, probe_block, probe_count
 from .context import Context
 from .solvers import CFIE_kress, Geometry, chebyshev_params
 
@@ -227,7 +227,7 @@ class Server:
             cheb = self.cheb_cache[ck]
             win = Window(solver, pts, req.k0, req.R, req.nq, precision=prec, geom=geom, **cheb)
             V = probe_block(np.random.default_rng(req.seed), N, r)
-            A0, A1 = win.moments(V, max(req.tol, 1e-13))
+            Rm, Cm = win.moments_qr(V, max(req.tol, 1e-13))
             st = dict(win.stats)
             st["precision_used"] = prec
             win.close()
@@ -240,7 +240,7 @@ class Server:
 
         def host():
             th = time.perf_counter()
-            B, Uk, S, rk = B_from_moments(A0, A1, req.svd_tol)
+            B, Ur, S, rk = B_from_qr(Rm, Cm, req.svd_tol)
             lam = np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128)
             t_done = time.perf_counter()
             inside = np.sort_complex(lam[np.abs(lam - req.k0) <= req.R])

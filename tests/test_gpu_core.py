@@ -246,3 +246,48 @@ def test_lu_graph_replay_matches_direct_launches():
     ctx.set_graphs(False)
     geom.close()
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_moments_qr_on_device(precision):
+    """qb_window_moments_qr: A0 = Q R with Q orthonormal and C = Q^H A1, against
+    the moments of qb_window_moments with the same probes; the device-QR route
+    of construct_B_matrix gives the eigenvalues and eigenvector span of the
+    host-SVD route."""
+    q = qb()
+    b, pts = _pts("star", [320])
+    solver = q.CFIE_kress(10.0, b)
+    cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
+    V = q.probe_block(np.random.default_rng(5), 320, 16)
+    win = q.Window(solver, pts, 30.0 + 0j, 0.4, 8, precision=precision, **cp)
+    A0, A1 = win.moments(V, 1e-13)
+    Rm, Cm = win.moments_qr(V, 1e-13)
+    Q = win.basis(np.eye(16, dtype=complex))
+    win.close()
+    s0 = np.max(np.abs(A0))
+    assert np.max(np.abs(Q.conj().T @ Q - np.eye(16))) < 1e-13
+    assert np.max(np.abs(Q @ Rm - A0)) < 1e-12 * s0
+    assert np.max(np.abs(np.tril(Rm, -1))) == 0.0
+    assert np.max(np.abs(Q.conj().T @ A1 - Cm)) < 1e-12 * np.max(np.abs(A1))
+    # the two routes of construct_B_matrix on the unit-circle window
+    bc = q.make_circle(1.0)
+    sc = q.CFIE_kress(10.0, bc)
+    pc = q.evaluate_points(sc, bc, 20.0, N=[256])
+    out = []
+    for dq in (True, False):
+        B, Uk, info = q.construct_B_matrix(sc, pc, 256, 20.0 + 0j, 0.5, nq=32, r=24,
+                                           svd_tol=1e-10, rng=np.random.default_rng(1),
+                                           precision=precision, device_qr=dq, return_info=True)
+        lam = np.linalg.eigvals(B)
+        out.append((np.sort_complex(lam[np.abs(lam - 20.0) <= 0.5]), Uk, info["sigma"]))
+    (l1, U1, S1), (l2, U2, S2) = out
+    # inside the disk the eigenvalues agree (outside, leaked poles are set by
+    # singular values near the threshold and are not compared)
+    assert len(l1) == len(l2) == 10 and np.max(np.abs(l1 - l2) / np.abs(l2)) < 1e-12
+    assert np.max(np.abs(S1 - S2)) <= 1e-13 * S2[0]
+    # the span of the well-separated singular directions agrees
+    lead = int(np.count_nonzero(S2 >= 1e-6 * S2[0]))
+    P1 = U1[:, :lead] @ U1[:, :lead].conj().T
+    P2 = U2[:, :lead] @ U2[:, :lead].conj().T
+    assert np.max(np.abs(U1.conj().T @ U1 - np.eye(U1.shape[1]))) < 1e-12
+    assert np.max(np.abs(P1 - P2)) < 1e-8
