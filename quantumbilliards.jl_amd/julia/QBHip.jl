@@ -171,27 +171,33 @@ function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float6
                     (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64, Float64, Int32,
                      Ref{ChebParams}, Int32, Ref{Ptr{Cvoid}}, Ref{Stats}),
                     ctx(), g, opcode(solver), real(k0), imag(k0), R, Int32(nq), cp, prec, w, st))
-        function moments(V)
-            A0 = similar(V); A1 = similar(V)
-            check(ccall((:qb_window_moments, LIB), Cint,
+        # moments reduced on the device: A0 = Q R, C = (Q^H A1)[1:r, :]
+        function moments_qr(V)
+            r = size(V, 2)
+            Rm = Matrix{ComplexF64}(undef, r, r); Cm = Matrix{ComplexF64}(undef, r, r)
+            check(ccall((:qb_window_moments_qr, LIB), Cint,
                         (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64},
                          Float64, Ref{Stats}),
-                        w[], Int32(size(V, 2)), V, A0, A1, refine_tol, st))
-            return A0, A1
+                        w[], Int32(r), V, Rm, Cm, refine_tol, st))
+            return Rm, Cm
         end
         V = randn(rng, ComplexF64, N, r)
-        A0, A1 = moments(V)
-        U, Σ, W = svd!(copy(A0); full=false)
+        Rm, Cm = moments_qr(V)
+        U, Σ, W = svd(Rm)                          # same Σ, W as svd(A0)
         rk = count(>=(svd_tol), Σ)
         rk == 0 && return Matrix{ComplexF64}(undef, N, 0), Matrix{ComplexF64}(undef, N, 0)
         if rk == r && 2r < N                       # the single retry of beyn.jl:291-309
             V = randn(rng, ComplexF64, N, 2r)
-            A0, A1 = moments(V)
-            U, Σ, W = svd!(copy(A0); full=false)
+            Rm, Cm = moments_qr(V)
+            U, Σ, W = svd(Rm)
             rk = count(>=(svd_tol), Σ)
         end
-        Uk = U[:, 1:rk]; Wk = W[:, 1:rk]
-        B = adjoint(Uk) * ((A1 * Wk) ./ transpose(Σ[1:rk]))
+        Ur = U[:, 1:rk]; Wk = W[:, 1:rk]
+        B = adjoint(Ur) * ((Cm * Wk) ./ transpose(Σ[1:rk]))
+        Uk = Matrix{ComplexF64}(undef, N, rk)      # U_k = Q [Ur; 0] on the device
+        check(ccall((:qb_window_basis, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{ComplexF64}, Int32, Ptr{ComplexF64}),
+                    w[], Matrix(Ur), Int32(rk), Uk))
         return B, Uk
     finally
         w[] != C_NULL && ccall((:qb_window_destroy, LIB), Cvoid, (Ptr{Cvoid},), w[])
