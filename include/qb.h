@@ -127,6 +127,26 @@ int qb_geom_create(qb_ctx* ctx, int32_t ncomp, const int64_t* n_per_comp,
 void qb_geom_destroy(qb_geom* g);
 int64_t qb_geom_size(const qb_geom* g);
 
+/* Symmetry reduction of a geometry: the reduced workspaces of the reference
+ * (build_CFIE_kress_reduced_workspace, cheb-CFIE:475-505, with the orbits of
+ * build_symmetry_maps / apply_symmetry_orbit_from_maps, symmetry.jl:746-833;
+ * build_dlp_kress_reduced_workspace, dlp_kress.jl:702-708, with the index
+ * orbits of periodic_symmetry_index_orbits, symmetry.jl:214-323).  m orbits of
+ * G nodes each: img[b*G + l] (0-based node index) is image l of orbit b, the
+ * first one its representative (Ifund[b]), and scale[b*G + l] (complex) its
+ * weight (fund_to_scale[b][l]; 1 for l = 0).  The orbits must partition the
+ * nodes.  After this call every assembly on g (qb_assemble,
+ * qb_assemble_deriv, qb_residuals, qb_window_*) produces the m x m reduced
+ * matrices of _all_k_symm_CFIE_chebyshev! (cheb-CFIE:698-824, its derivative
+ * twin :1087-1243) and of the reduced DLP paths (cheb-DLP:771-880, :955-1050):
+ *   A_red(a, b) = sum_l scale[b*G+l] A(img[a*G], img[b*G+l])
+ * with A the full operator (identity included).  m = 0 removes the reduction. */
+int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G, const int64_t* img,
+                       const double* scale);
+/* Matrix dimension of assemblies on g: m after qb_geom_set_orbits, else N
+ * (boundary_matrix_size_for_solver, ebim.jl:26-40). */
+int64_t qb_geom_dim(const qb_geom* g);
+
 /* construct_boundary_matrices!(Tbufs, solver, pts, zj; n_panels_h, M_h,
  * n_panels_j, M_j) for CFIE_kress (cheb-CFIE:1431-1450) and DLP_kress
  * (cheb-DLP:1110-1124).  Writes nz dense N x N complex128 matrices

@@ -735,6 +735,90 @@ def dlp_assemble_deriv(comp: Comp, zs, cp: ChebParams = ChebParams()):
     return tuple(out)
 
 
+# --------------------------------------------------------------------------
+# Symmetry reduction (the reduced Kress workspaces)
+# --------------------------------------------------------------------------
+def sym_scales(kind: str, n: int = 0, m: int = 0, parity=1) -> np.ndarray:
+    """Orbit weights: [1, p] for one reflection, [1, px, py, px py] for both
+    (apply_symmetry_orbit_from_maps, symmetry.jl:776-787), and the rotation
+    characters cis(2 pi (m mod n) l / n) (_rotation_tables, symmetry.jl:64-82)."""
+    if kind == "rot":
+        return np.array([np.exp(2j * np.pi * ((m % n) * l) / n) for l in range(n)])
+    if kind == "origin":
+        px, py = parity
+        return np.array([1, px, py, px * py], dtype=np.complex128)
+    return np.array([1, parity], dtype=np.complex128)
+
+
+def orbits_index(N: int, kind: str, n: int = 0, m: int = 0, parity=1):
+    """periodic_symmetry_index_orbits (symmetry.jl:214-298), the DLP_kress
+    orbits, as (img [m x G] 0-based with img[:, 0] = Ifund, scale [m x G])."""
+    rows = []
+    if kind == "rot":
+        for q1 in range(1, N // n + 1):
+            rows.append([((q1 + l * (N // n)) - 1) % N for l in range(n)])
+    elif kind in ("x_axis", "y_axis"):
+        for q1 in range(1, N // 2 + 1):
+            rows.append([q1 - 1, ((N - q1 + 1) - 1) % N])
+    else:
+        for q1 in range(1, N // 4 + 1):
+            rows.append([q1 - 1, ((N // 2 - q1 + 1) - 1) % N, ((N - q1 + 1) - 1) % N,
+                         ((q1 + N // 2) - 1) % N])
+    img = np.array(rows, dtype=np.int64)
+    return img, np.tile(sym_scales(kind, n, m, parity), (len(rows), 1))
+
+
+def orbits_coords(comps, kind: str, n: int = 0, m: int = 0, parity=1, center=(0.0, 0.0),
+                  tol: float = 1e-10):
+    """The CFIE_kress orbits (build_CFIE_kress_reduced_workspace,
+    cheb-CFIE:475-505): node maps by coordinates (build_symmetry_maps,
+    symmetry.jl:746-766: match_index for reflections, cyclic shifts per
+    component for rotations, :648-730), nodes visited in order.  Brute force
+    over all node pairs."""
+    x = np.concatenate([c.x for c in comps])
+    y = np.concatenate([c.y for c in comps])
+    N = len(x)
+
+    def match(tx, ty, tl):
+        d = (x[None, :] - tx[:, None]) ** 2 + (y[None, :] - ty[:, None]) ** 2
+        hit = d < tl * tl
+        if not np.all(hit.sum(axis=1) == 1):
+            raise ValueError("no unique match")
+        return np.argmax(hit, axis=1)
+
+    if kind == "rot":
+        cx, cy = center
+        maps = [np.arange(N)]
+        for l in range(1, n):
+            th = 2 * np.pi * l / n
+            c, s = math.cos(th), math.sin(th)
+            maps.append(match(cx + c * (x - cx) - s * (y - cy), cy + s * (x - cx) + c * (y - cy),
+                              max(1e-8, tol)))
+    elif kind == "y_axis":
+        maps = [np.arange(N), match(-x, y, tol)]
+    elif kind == "x_axis":
+        maps = [np.arange(N), match(x, -y, tol)]
+    else:
+        maps = [np.arange(N), match(-x, y, tol), match(x, -y, tol), match(-x, -y, tol)]
+    M = np.stack(maps, axis=1)
+    seen = np.zeros(N, bool)
+    rows = []
+    for g in range(N):
+        if not seen[g]:
+            rows.append(M[g])
+            seen[M[g]] = True
+    img = np.array(rows, dtype=np.int64)
+    return img, np.tile(sym_scales(kind, n, m, parity), (len(rows), 1))
+
+
+def reduce_matrix(A: np.ndarray, img: np.ndarray, scale: np.ndarray) -> np.ndarray:
+    """Reduced matrix from the full one, entry by entry as the reduced
+    assembly sums it (cheb-CFIE:745-820, cheb-DLP:806-841):
+    A_red(a, b) = sum_l scale[b, l] A(img[a, 0], img[b, l])."""
+    rep = img[:, 0]
+    return np.einsum("bl,alb->ab", scale, A[rep][:, img.T])
+
+
 def cauchy_derivatives(assemble, z: complex, rho: float = 0.25, n: int = 24):
     """dT/dz and d2T/dz2 of an analytic matrix function from its values on
     the circle z + rho e^{i theta_m} (trapezoid rule of Cauchy's integral

@@ -440,11 +440,80 @@ extern "C" int qb_geom_create(qb_ctx* ctx, int32_t ncomp, const int64_t* n_per_c
 
 extern "C" void qb_geom_destroy(qb_geom* g) {
   if (!g) return;
+  qb_dev_free(g->ctx, g->sym_buf, (size_t)g->sym_bytes);
   qb_dev_free(g->ctx, g->buf, (size_t)g->bytes);
   delete g;
 }
 
 extern "C" int64_t qb_geom_size(const qb_geom* g) { return g ? g->d.N : 0; }
+
+extern "C" int64_t qb_geom_dim(const qb_geom* g) { return g ? qb_dim(g) : 0; }
+
+// Orbit tables of a symmetry reduction.  The checks are those the reference
+// makes while building its orbit maps (_build_periodic_orbit_maps,
+// symmetry.jl:300-323: no collision, every node covered) plus the ones its
+// assembly relies on: the representative is image 0 with weight 1, and no
+// orbit meets a node twice (a node on a reflection axis,
+// _has_node_on_reflection_axis, symmetry.jl:326-345).
+extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
+                                  const int64_t* img, const double* scale) {
+  if (!ctx) return QB_E_ARG;
+  if (!g) return qb_fail(ctx, QB_E_ARG, "null geometry");
+  if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
+  (void)hipSetDevice(ctx->device);
+  if (m == 0) {
+    qb_dev_free(ctx, g->sym_buf, (size_t)g->sym_bytes);
+    g->sym_buf = nullptr;
+    g->sym_bytes = 0;
+    g->sym = DevSym{};
+    return QB_OK;
+  }
+  if (!img || !scale) return qb_fail(ctx, QB_E_ARG, "null orbit table");
+  const int64_t N = g->d.N;
+  if (G < 2 || G > 64) return qb_fail(ctx, QB_E_ARG, "orbit size G must be in [2, 64]");
+  if (m < 1 || m * (int64_t)G != N)
+    return qb_fail(ctx, QB_E_ARG, "orbits must partition the nodes: m * G != N");
+  std::vector<int32_t> im((size_t)m * G);
+  std::vector<char> seen((size_t)N, 0);
+  for (int64_t b = 0; b < m; ++b) {
+    if (scale[2 * b * G] != 1.0 || scale[2 * b * G + 1] != 0.0)
+      return qb_fail(ctx, QB_E_ARG, "orbit " + std::to_string(b) +
+                                        ": the representative (image 0) must have weight 1");
+    for (int l = 0; l < G; ++l) {
+      const int64_t q = img[b * G + l];
+      if (q < 0 || q >= N)
+        return qb_fail(ctx, QB_E_ARG, "orbit " + std::to_string(b) + ": node index out of range");
+      if (seen[q])
+        return qb_fail(ctx, QB_E_ARG,
+                       "node " + std::to_string(q) +
+                           " appears twice in the orbits (collision, or a node on a symmetry axis)");
+      seen[q] = 1;
+      im[b * G + l] = (int32_t)q;
+    }
+  }
+  const size_t bi = im.size() * sizeof(int32_t), bs = im.size() * sizeof(cd);
+  const size_t bytes = bs + ((bi + 255) & ~(size_t)255);
+  void* buf = nullptr;
+  int rc = qb_dev_alloc(ctx, &buf, bytes);
+  if (rc) return rc;
+  cd* ds = (cd*)buf;
+  int32_t* di = (int32_t*)((char*)buf + bs);
+  hipError_t e = hipMemcpyAsync(ds, scale, bs, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(di, im.data(), bi, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    qb_dev_free(ctx, buf, bytes);
+    return qb_fail(ctx, QB_E_HIP, std::string("orbit upload: ") + hipGetErrorString(e));
+  }
+  qb_dev_free(ctx, g->sym_buf, (size_t)g->sym_bytes);
+  g->sym_buf = buf;
+  g->sym_bytes = (int64_t)bytes;
+  g->sym.m = m;
+  g->sym.G = G;
+  g->sym.img = di;
+  g->sym.scale = ds;
+  return QB_OK;
+}
 
 // ---------------------------------------------------------------------------
 // assembly
@@ -486,7 +555,7 @@ extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const doub
     rminmax[0] = rmin;
     rminmax[1] = rmax;
   }
-  int64_t N = g->d.N;
+  int64_t N = qb_dim(g);
   cd* zd = nullptr;
   rc = qb_dev_alloc(ctx, (void**)&zd, nz * sizeof(cd));
   if (rc) return rc;
@@ -539,7 +608,7 @@ extern "C" int qb_assemble_deriv(qb_ctx* ctx, const qb_geom* g, int32_t op, cons
     rminmax[0] = rmin;
     rminmax[1] = rmax;
   }
-  const int64_t N = g->d.N;
+  const int64_t N = qb_dim(g);
   const size_t obytes = (size_t)nz * N * N * sizeof(cd);
   const bool on_dev = (flags & QB_OUT_DEVICE) != 0;
   cd* zd = nullptr;
@@ -592,7 +661,7 @@ extern "C" int qb_residuals(qb_ctx* ctx, const qb_geom* g, int32_t op, const dou
   (void)hipSetDevice(ctx->device);
   double rmin, rmax;
   op_range(g, op, &rmin, &rmax);
-  const int64_t N = g->d.N;
+  const int64_t N = qb_dim(g);
   const size_t per = (size_t)N * N * sizeof(cd);
   int64_t G = batch > 0 ? batch : (int64_t)std::max<size_t>(1, ((size_t)8 << 30) / per);
   G = std::min<int64_t>(G, nl);
@@ -789,7 +858,7 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   w->op = op;
   w->nq = nq;
   w->precision = precision;
-  w->N = g->d.N;
+  w->N = qb_dim(g);
   w->k0re = k0_re;
   w->k0im = k0_im;
   w->R = R;

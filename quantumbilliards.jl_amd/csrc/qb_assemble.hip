@@ -246,6 +246,7 @@ int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int6
     return qb_fail(ctx, QB_E_ARG, "unknown operator");
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  if (g->sym.m > 0) return qb_assemble_red_dev(ctx, g, op, z_dev, nz, t, out64, out32);
   int64_t N = g->d.N;
   // grid.x = number of 64 x 64 tile pairs (2^31 - 1 limit: N < 2.9e6)
   // grid.z is limited to 65535; chunk the wavenumbers
@@ -524,6 +525,8 @@ int qb_assemble_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   if (!t.J) return qb_fail(ctx, QB_E_ARG, "derivative assembly needs the J tables");
+  if (g->sym.m > 0)
+    return qb_assemble_red_deriv_dev(ctx, g, op, z_dev, nz, t, out, dout, ddout, off_sign);
   int64_t N = g->d.N;
   for (int64_t q0 = 0; q0 < nz; q0 += 1024) {
     int64_t nzc = nz - q0 < 1024 ? nz - q0 : 1024;
@@ -542,6 +545,236 @@ int qb_assemble_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev
                                              ddout + o, off_sign);
       else
         launch_asm_deriv<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                             ddout + o, off_sign);
+    }
+    QB_HIP(ctx, hipGetLastError());
+  }
+  return QB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Symmetry-reduced assembly (geometry with orbits, qb_geom_set_orbits):
+//   CFIE: _all_k_symm_CFIE_chebyshev! (cheb-CFIE:698-824) and its derivative
+//         twin _all_k_symm_CFIE_chebyshev_deriv! (cheb-CFIE:1087-1243);
+//   DLP:  the reduced _construct_dlp_kress_matrices_chebyshev! (cheb-DLP:771-880)
+//         and its derivative twin (cheb-DLP:955-1050), then F = I - D.
+// Entry (a, b) of the m x m reduced matrix is
+//   sum_l scale[b, l] * A(rep_a, img[b, l]),
+// A the full operator entry computed exactly as k_assemble / k_assemble_deriv
+// compute it (same specials, same Kress terms, identity on the diagonal).
+// One thread per reduced entry and wavenumber: a wave holds 64 consecutive
+// rows a of one column b, so the images of b are wave-uniform and the stores
+// are coalesced.  The workgroup covers a 64 x 64 tile; grid.z walks the
+// wavenumbers.
+// ---------------------------------------------------------------------------
+struct PairGeo {
+  double dx, dy, r, invr, Rij, lt;
+  bool same;
+};
+
+__device__ __forceinline__ PairGeo pair_geo(const DevGeom& g, int64_t i, int64_t j, double xi,
+                                            double yi, int ci) {
+  PairGeo p;
+  p.dx = xi - g.x[j];
+  p.dy = yi - g.y[j];
+  p.r = hypot(p.dx, p.dy);
+  p.invr = 1.0 / p.r;
+  p.same = (ci == g.comp[j]);
+  p.Rij = 0;
+  p.lt = 0;
+  if (p.same) {
+    const int64_t o = g.off[ci];
+    const int64_t nc = g.off[ci + 1] - o;
+    int64_t dd = (i - j) % nc;
+    if (dd < 0) dd += nc;
+    p.Rij = g.rk[o + dd];
+    p.lt = g.lt[o + dd];
+  }
+  return p;
+}
+
+__device__ __forceinline__ TabView tab_view(const cd* H, const cd* J, int64_t q, int np_h,
+                                            int M_h, int np_j, int M_j, double rmin_h,
+                                            double rmax) {
+  TabView tv;
+  tv.H = H + q * (int64_t)np_h * (2 * (M_h + 1));
+  tv.J = J ? J + q * (int64_t)np_j * (2 * (M_j + 1)) : nullptr;
+  tv.np_h = np_h;
+  tv.M_h = M_h;
+  tv.np_j = np_j;
+  tv.M_j = M_j;
+  tv.rmin_h = rmin_h;
+  tv.rmax = rmax;
+  return tv;
+}
+
+template <int OP, int MH, int MJ>
+__global__ void __launch_bounds__(256) k_assemble_red(DevGeom g, DevSym s, const cd* __restrict__ zk,
+                                                      const cd* __restrict__ H,
+                                                      const cd* __restrict__ J, int np_h, int M_h,
+                                                      int np_j, int M_j, double rmin_h,
+                                                      double rmax, int64_t q0,
+                                                      cd* __restrict__ out64,
+                                                      cf* __restrict__ out32) {
+  const int64_t m = s.m;
+  const int G = s.G;
+  const int64_t q = q0 + blockIdx.z;
+  const TabView tv = tab_view(H, J, q, np_h, M_h, np_j, M_j, rmin_h, rmax);
+  const cd k = zk[q];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t a = (int64_t)blockIdx.x * 64 + il;
+  if (a >= m) return;  // no barrier in this kernel
+  const int64_t i = s.img[a * G];
+  const double xi = g.x[i], yi = g.y[i];
+  const int ci = g.comp[i];
+  const int64_t base = (q - q0) * m * m;
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int64_t b = (int64_t)blockIdx.y * 64 + jl0 + 4 * c;
+    if (b >= m) break;
+    cd acc = mk(0, 0);
+#pragma unroll 1
+    for (int l = 0; l < G; ++l) {
+      const int64_t j = s.img[b * G + l];
+      const cd sc = s.scale[b * G + l];
+      cd v;
+      if (i == j) {
+        v = diag_entry<OP>(g, j, k);
+      } else {
+        const PairGeo p = pair_geo(g, i, j, xi, yi, ci);
+        const Spec sp = specials<OP, MH, MJ>(tv, p.r, k, p.same || OP == QB_DLP_KRESS);
+        v = offdiag_entry<OP>(g, i, j, p.dx, p.dy, p.invr, p.same, p.Rij, p.lt, k, sp);
+      }
+      acc = acc + sc * v;
+    }
+    const int64_t idx = base + a + b * m;
+    if (out64) out64[idx] = acc;
+    if (out32) out32[idx] = to_cf(acc);
+  }
+}
+
+template <int OP, int MH, int MJ>
+__global__ void __launch_bounds__(256) k_assemble_red_deriv(
+    DevGeom g, DevSym s, const cd* __restrict__ zk, const cd* __restrict__ H,
+    const cd* __restrict__ J, int np_h, int M_h, int np_j, int M_j, double rmin_h, double rmax,
+    int64_t q0, cd* __restrict__ o0, cd* __restrict__ o1, cd* __restrict__ o2, double off_sign) {
+  const int64_t m = s.m;
+  const int G = s.G;
+  const int64_t q = q0 + blockIdx.z;
+  const TabView tv = tab_view(H, J, q, np_h, M_h, np_j, M_j, rmin_h, rmax);
+  const cd k = zk[q];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t a = (int64_t)blockIdx.x * 64 + il;
+  if (a >= m) return;
+  const int64_t i = s.img[a * G];
+  const double xi = g.x[i], yi = g.y[i];
+  const int ci = g.comp[i];
+  const int64_t base = (q - q0) * m * m;
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int64_t b = (int64_t)blockIdx.y * 64 + jl0 + 4 * c;
+    if (b >= m) break;
+    cd a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0);
+#pragma unroll 1
+    for (int l = 0; l < G; ++l) {
+      const int64_t j = s.img[b * G + l];
+      const cd sc = s.scale[b * G + l];
+      D3 v;
+      if (i == j) {
+        v = diag_deriv<OP>(g, j, k);
+      } else {
+        const PairGeo p = pair_geo(g, i, j, xi, yi, ci);
+        const Spec sp = specials_all<MH, MJ>(tv, p.r, k, p.same);
+        v = offdiag_deriv<OP>(g, j, p.dx, p.dy, p.r, p.invr, p.same, p.Rij, p.lt, k, sp,
+                              off_sign);
+      }
+      a0 = a0 + sc * v.a;
+      a1 = a1 + sc * v.a1;
+      a2 = a2 + sc * v.a2;
+    }
+    const int64_t idx = base + a + b * m;
+    o0[idx] = a0;
+    o1[idx] = a1;
+    o2[idx] = a2;
+  }
+}
+
+template <int OP, int MH, int MJ>
+static void launch_red(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, int64_t nzc,
+                       const DevTables& t, cd* o64, cf* o32) {
+  const int64_t m = g->sym.m;
+  const unsigned Tm = (unsigned)((m + 63) / 64);
+  const double ent = (double)m * m * nzc;
+  int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
+                           ent * ((o64 ? sizeof(cd) : 0) + (o32 ? sizeof(cf) : 0)));
+  hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ>), dim3(Tm, Tm, (unsigned)nzc), dim3(256), 0,
+                     ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j,
+                     t.rmin_h, t.rmax, q0, o64, o32);
+  qb_prof_end(ctx, slot);
+}
+
+template <int OP, int MH, int MJ>
+static void launch_red_deriv(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0,
+                             int64_t nzc, const DevTables& t, cd* o0, cd* o1, cd* o2,
+                             double off_sign) {
+  const int64_t m = g->sym.m;
+  const unsigned Tm = (unsigned)((m + 63) / 64);
+  int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0, 3.0 * (double)m * m * nzc * sizeof(cd));
+  hipLaunchKernelGGL((k_assemble_red_deriv<OP, MH, MJ>), dim3(Tm, Tm, (unsigned)nzc), dim3(256),
+                     0, ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j,
+                     t.rmin_h, t.rmax, q0, o0, o1, o2, off_sign);
+  qb_prof_end(ctx, slot);
+}
+
+int qb_assemble_red_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
+                        const DevTables& t, cd* out64, cf* out32) {
+  const int64_t m = g->sym.m;
+  for (int64_t q0 = 0; q0 < nz; q0 += 1024) {
+    const int64_t nzc = nz - q0 < 1024 ? nz - q0 : 1024;
+    cd* o64 = out64 ? out64 + q0 * m * m : nullptr;
+    cf* o32 = out32 ? out32 + q0 * m * m : nullptr;
+    const int mh = t.M_h, mj = t.M_j;
+    if (op == QB_CFIE_KRESS) {
+      if (mh == 7 && mj == 7)
+        launch_red<QB_CFIE_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+      else if (mh == 5 && mj == 5)
+        launch_red<QB_CFIE_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+      else
+        launch_red<QB_CFIE_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+    } else {
+      if (mh == 7 && mj == 7)
+        launch_red<QB_DLP_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+      else if (mh == 5 && mj == 5)
+        launch_red<QB_DLP_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+      else
+        launch_red<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+    }
+    QB_HIP(ctx, hipGetLastError());
+  }
+  return QB_OK;
+}
+
+int qb_assemble_red_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
+                              int64_t nz, const DevTables& t, cd* out, cd* dout, cd* ddout,
+                              double off_sign) {
+  const int64_t m = g->sym.m;
+  for (int64_t q0 = 0; q0 < nz; q0 += 1024) {
+    const int64_t nzc = nz - q0 < 1024 ? nz - q0 : 1024;
+    const int64_t o = q0 * m * m;
+    const int mh = t.M_h, mj = t.M_j;
+    if (op == QB_CFIE_KRESS) {
+      if (mh == 7 && mj == 7)
+        launch_red_deriv<QB_CFIE_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                              ddout + o, off_sign);
+      else
+        launch_red_deriv<QB_CFIE_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                              ddout + o, off_sign);
+    } else {
+      if (mh == 7 && mj == 7)
+        launch_red_deriv<QB_DLP_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
+                                             ddout + o, off_sign);
+      else
+        launch_red_deriv<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, out + o, dout + o,
                                              ddout + o, off_sign);
     }
     QB_HIP(ctx, hipGetLastError());

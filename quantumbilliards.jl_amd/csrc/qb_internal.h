@@ -32,6 +32,17 @@ struct DevGeom {
   const double* lt;     // log(4 sin^2((t_i - t_j)/2)) circulant columns (length N)
 };
 
+// Symmetry orbits of a geometry (symmetry-reduced assembly).  Reduced index
+// b stands for the orbit img[b*G + 0 .. b*G + G-1] of full node indices, with
+// img[b*G] its representative; entry (a, b) of a reduced matrix is
+//   sum_l scale[b*G + l] * A_full(img[a*G], img[b*G + l]).
+struct DevSym {
+  int64_t m = 0;  // reduced dimension (number of orbits)
+  int32_t G = 0;  // orbit size (group order)
+  const int32_t* img = nullptr;
+  const cd* scale = nullptr;
+};
+
 struct QbProfRec {
   hipEvent_t a, b;
   int kind;
@@ -94,7 +105,14 @@ struct qb_geom {
   int64_t bytes = 0;
   double rmin_geom = 0, rmax_geom = 0;  // padded (x0.95, x1.05) min / max distance
   double rmin_dlp = 0, rmax_dlp = 0;    // same over the first component only
+  DevSym sym{};          // set by qb_geom_set_orbits (m = 0: no reduction)
+  void* sym_buf = nullptr;
+  int64_t sym_bytes = 0;
 };
+
+// Dimension of the matrices assembled on a geometry: the number of orbits
+// when symmetry orbits are set, else the number of nodes.
+static inline int64_t qb_dim(const qb_geom* g) { return g->sym.m > 0 ? g->sym.m : g->d.N; }
 
 // Chebyshev tables of one or more wavenumbers, resident on the device.
 // H: [nz][np_h][M_h+1] of {H0 coeff, H1 coeff}; J: [nz][np_j][M_j+1] of
@@ -138,6 +156,14 @@ int qb_assemble_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev
                           const DevTables& t, cd* out, cd* dout, cd* ddout, bool ref_sign);
 int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
                     int64_t nz, const DevTables& t, cd* out64, cf* out32);
+// The same two for a geometry with symmetry orbits: m x m reduced matrices
+// (ld = m, stride m*m).  qb_assemble_dev / qb_assemble_deriv_dev call these
+// when g->sym.m > 0.
+int qb_assemble_red_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
+                        const DevTables& t, cd* out64, cf* out32);
+int qb_assemble_red_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
+                              int64_t nz, const DevTables& t, cd* out, cd* dout, cd* ddout,
+                              double off_sign);
 
 // bessel (qb_tables.hip)
 int qb_bessel_eval_dev(qb_ctx* ctx, const cd* z, int64_t n, cd* out4);

@@ -32,7 +32,7 @@ import numpy as np
 
 from . import _lib
 from .context import Context, default_context
-from .solvers import Geometry, boundary_matrix_size
+from .solvers import Geometry, boundary_matrix_size, boundary_matrix_size_for_solver
 
 _PREC = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED}
 
@@ -52,8 +52,8 @@ class Window:
                  n_panels_j=3000, M_j=5, precision="fp64", geom: Optional[Geometry] = None,
                  ctx: Optional[Context] = None):
         self.ctx = ctx or (geom.ctx if geom is not None else default_context())
-        self.geom = geom or Geometry(pts, self.ctx)
-        self.N = self.geom.N
+        self.geom = geom or Geometry(pts, self.ctx, solver)
+        self.N = self.geom.dim  # reduced dimension with a solver symmetry
         self.nq = nq
         self.k0, self.R = complex(k0), float(R)
         self.precision = precision
@@ -191,7 +191,8 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
     if not use_chebyshev:
         raise RuntimeError("only the Chebyshev path exists for complex contour nodes")
     rng = rng if rng is not None else np.random.default_rng(0)
-    assert boundary_matrix_size(pts) == N, "N must equal the boundary matrix size"
+    assert boundary_matrix_size_for_solver(solver, pts) == N, \
+        "N must equal the boundary matrix size"
     t0 = time.perf_counter()
     own = window is None
     win = window or Window(solver, pts, k0, R, nq, n_panels_h=n_panels_h, M_h=M_h,
@@ -250,7 +251,7 @@ def construct_B_matrix(solver, pts, N: int, k0: complex, R: float, *, nq: int = 
 def solve_vect(solver, basis, pts, k: complex, dk: float, *, nq: int = 32, r: int = 48,
                svd_tol: float = 1e-14, res_tol: float = 1e-8, rng=None, **kw):
     """beyn.jl:364-375 -> (lam, Uk, Y, k, dk, pts)."""
-    N = boundary_matrix_size(pts)
+    N = boundary_matrix_size_for_solver(solver, pts)
     B, Uk = construct_B_matrix(solver, pts, N, k, dk, nq=nq, r=r, svd_tol=svd_tol, rng=rng, **kw)
     if B.size == 0:
         return np.zeros(0, np.complex128), Uk, np.zeros((0, 0), np.complex128), k, dk, pts
@@ -261,7 +262,7 @@ def solve_vect(solver, basis, pts, k: complex, dk: float, *, nq: int = 32, r: in
 def solve(solver, basis, pts, k: complex, dk: float, *, nq: int = 32, r: int = 48,
           svd_tol: float = 1e-14, res_tol: float = 1e-8, rng=None, **kw):
     """beyn.jl:386-397 -> lam (no filtering, like the reference)."""
-    N = boundary_matrix_size(pts)
+    N = boundary_matrix_size_for_solver(solver, pts)
     B, Uk = construct_B_matrix(solver, pts, N, k, dk, nq=nq, r=r, svd_tol=svd_tol, rng=rng, **kw)
     if B.size == 0:
         return np.zeros(0, np.complex128)

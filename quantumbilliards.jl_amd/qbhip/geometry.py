@@ -52,6 +52,7 @@ class Billiard:
     area: float
     length: float
     name: str = "billiard"
+    area_fundamental: Optional[float] = None  # Weyl area of a symmetry sector
 
 
 def _circle_curve(R, cx=0.0, cy=0.0):
@@ -198,10 +199,13 @@ def reverse_orientation(p: BoundaryPointsCFIE) -> BoundaryPointsCFIE:
                               p.ts[::-1].copy(), p.ws[::-1].copy(), p.compid)
 
 
-def n_kress(k: float, L: float, b: float, min_pts: int = 20) -> int:
-    """max(min_pts, round(k L b / 2pi)), bumped to even (cfie.jl:537-549)."""
+def n_kress(k: float, L: float, b: float, min_pts: int = 20, divisor: int = 2) -> int:
+    """max(min_pts, round(k L b / 2pi)), bumped up to a multiple of `divisor`:
+    2 without symmetry, 4 with a reflection, lcm(2, n) with a C_n rotation
+    (cfie.jl:537-549, dlp_kress.jl:361-372)."""
     N = max(min_pts, int(round(k * L * b / TWO_PI)))
-    return N + (N % 2)
+    rem = N % divisor
+    return N + (divisor - rem if rem else 0)
 
 
 def evaluate_points(solver, billiard: Billiard, k: float,
@@ -209,11 +213,14 @@ def evaluate_points(solver, billiard: Billiard, k: float,
     """evaluate_points for CFIE_kress / DLP_kress (cfie.jl:822-832,
     dlp_kress.jl:563-606).  `N` (one count per component) overrides the
     points-per-wavelength rule, for workloads that fix N."""
+    from .symmetry import node_divisor
     comps = billiard.full_boundary
+    div = node_divisor(getattr(solver, "symmetry", None))
     out = []
     for idx, comp in enumerate(comps):
         n = int(N[idx]) if N is not None else n_kress(k, _component_length(comp),
-                                                     solver.pts_scaling_factor, solver.min_pts)
+                                                     solver.pts_scaling_factor, solver.min_pts,
+                                                     div)
         p = _nodes(comp, n, idx + 1)
         out.append(p if idx == 0 else reverse_orientation(p))
     return out

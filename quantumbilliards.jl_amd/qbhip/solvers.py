@@ -23,6 +23,7 @@ import numpy as np
 from . import _lib
 from .context import Context, default_context
 from .geometry import Billiard, BoundaryPointsCFIE
+from .symmetry import orbits_for
 
 
 @dataclass
@@ -54,14 +55,26 @@ def boundary_matrix_size(pts) -> int:
     return int(sum(len(p) for p in pts))
 
 
+def boundary_matrix_size_for_solver(solver, pts) -> int:
+    """boundary_matrix_size_for_solver (ebim.jl:26-40): the number of
+    symmetry orbits when the solver carries a symmetry, else the node count."""
+    sym = getattr(solver, "symmetry", None)
+    if sym is None:
+        return boundary_matrix_size(pts)
+    return boundary_matrix_size(pts) // sym.order
+
+
 def _as_list(pts) -> List[BoundaryPointsCFIE]:
     return [pts] if isinstance(pts, BoundaryPointsCFIE) else list(pts)
 
 
 class Geometry:
-    """A boundary discretisation resident on the device (qb_geom)."""
+    """A boundary discretisation resident on the device (qb_geom).  With a
+    solver that carries a symmetry, the geometry also holds the solver's
+    orbit table (qb_geom_set_orbits) and every assembly on it is the
+    symmetry-reduced one; `N` is the node count, `dim` the matrix dimension."""
 
-    def __init__(self, pts, ctx: Optional[Context] = None):
+    def __init__(self, pts, ctx: Optional[Context] = None, solver=None):
         self.ctx = ctx or default_context()
         comps = _as_list(pts)
         cat = lambda f: np.ascontiguousarray(np.concatenate([f(p) for p in comps]), dtype=np.float64)
@@ -78,6 +91,18 @@ class Geometry:
                               *[a.ctypes.data for a in self._arrs], C.byref(h))
         _lib.check(self.ctx.ptr, rc)
         self.ptr = h
+        self.dim = self.N
+        self.symmetry = getattr(solver, "symmetry", None)
+        orb = orbits_for(solver, pts) if solver is not None else None
+        if orb is not None:
+            img, sc = orb
+            img = np.ascontiguousarray(img, dtype=np.int64)
+            sc = np.ascontiguousarray(sc, dtype=np.complex128)
+            rc = L.qb_geom_set_orbits(self.ctx.ptr, self.ptr, img.shape[0], img.shape[1],
+                                      img.ctypes.data, sc.ctypes.data)
+            _lib.check(self.ctx.ptr, rc)
+            self.dim = int(img.shape[0])
+            self.orbits = (img, sc)
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -100,11 +125,12 @@ def assemble(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_panels_j=3000, M_j=5
              out: Optional[np.ndarray] = None) -> np.ndarray:
     """All T(z_q) in one launch.  Returns an array `T` of shape (nz, N, N)
     with T[q] the matrix T(z_q) in the usual (row, column) indexing; its
-    storage is column-major per matrix (T[q].flags.f_contiguous)."""
+    storage is column-major per matrix (T[q].flags.f_contiguous).  With a
+    solver symmetry, N is the number of orbits (the reduced matrices)."""
     ctx = ctx or (geom.ctx if geom is not None else default_context())
-    g = geom or Geometry(pts, ctx)
+    g = geom or Geometry(pts, ctx, solver)
     zj = np.ascontiguousarray(np.asarray(zj, dtype=np.complex128).ravel())
-    nz, N = len(zj), g.N
+    nz, N = len(zj), g.dim
     if out is None:
         raw = np.empty((nz, N, N), dtype=np.complex128)
     else:
@@ -134,7 +160,7 @@ def construct_boundary_matrices(Tbufs, solver, pts, zj, *, multithreaded: bool =
     if isinstance(solver, DLP_kress) and not isinstance(pts, BoundaryPointsCFIE):
         comps = _as_list(pts)
         assert len(comps) == 1, "DLP_kress expects one BoundaryPointsCFIE component"
-    N = boundary_matrix_size(pts)
+    N = boundary_matrix_size_for_solver(solver, pts)
     for q, T in enumerate(Tbufs):
         assert T.shape == (N, N), f"Tbufs[{q}] has size {T.shape}, but the workspace requires ({N},{N})."
     T = assemble(solver, pts, zj, n_panels_h=n_panels_h, M_h=M_h, n_panels_j=n_panels_j,
@@ -153,9 +179,9 @@ def assemble_with_derivatives(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_pan
     reference_offblock_sign=True gives the between-component double-layer
     derivative terms the reference's (opposite) sign, cheb-CFIE:1031-1032."""
     ctx = ctx or (geom.ctx if geom is not None else default_context())
-    g = geom or Geometry(pts, ctx)
+    g = geom or Geometry(pts, ctx, solver)
     zj = np.ascontiguousarray(np.asarray(zj, dtype=np.complex128).ravel())
-    nz, N = len(zj), g.N
+    nz, N = len(zj), g.dim
     raws = [np.empty((nz, N, N), dtype=np.complex128) for _ in range(3)]
     cp = _cheb(n_panels_h, M_h, n_panels_j, M_j)
     rr = np.zeros(2)
@@ -186,7 +212,7 @@ def construct_boundary_matrices_with_derivatives(Tbufs, dTbufs, ddTbufs, solver,
     assert len(Tbufs) == Mk and len(dTbufs) == Mk and len(ddTbufs) == Mk
     if isinstance(solver, DLP_kress) and not isinstance(pts, BoundaryPointsCFIE):
         assert len(_as_list(pts)) == 1, "DLP_kress expects one BoundaryPointsCFIE component"
-    N = boundary_matrix_size(pts)
+    N = boundary_matrix_size_for_solver(solver, pts)
     for bufs in (Tbufs, dTbufs, ddTbufs):
         for q, T in enumerate(bufs):
             assert T.shape == (N, N), f"buffer {q} has size {T.shape}, but ({N},{N}) is required"

@@ -24,6 +24,7 @@ come back.  Indices are 0-based where the reference's are 1-based.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import math
 from typing import List, Optional, Sequence
 
@@ -33,7 +34,8 @@ from . import _lib
 from .beyn import construct_B_matrix
 from .context import Context, default_context
 from .geometry import Billiard, evaluate_points
-from .solvers import Geometry, _cheb, boundary_matrix_size, chebyshev_params
+from .solvers import (Geometry, _cheb, boundary_matrix_size, boundary_matrix_size_for_solver,
+                      chebyshev_params)
 
 EPS = np.finfo(np.float64).eps
 
@@ -134,11 +136,11 @@ def residual_norms(solver, pts, lam, Phi, *, n_panels_h=15000, M_h=5, n_panels_j
     """(||y_q||_2, ||y_q||_1, ||y_q||_inf, ||T(lam_q)||_1, ||T(lam_q)||_inf)
     with y_q = T(lam_q) Phi[:, q], for every q (qb_residuals)."""
     ctx = ctx or (geom.ctx if geom is not None else default_context())
-    g = geom or Geometry(pts, ctx)
+    g = geom or Geometry(pts, ctx, solver)
     lam = np.ascontiguousarray(np.asarray(lam, dtype=np.complex128).ravel())
     nl = len(lam)
     Phi = np.asfortranarray(Phi, dtype=np.complex128)
-    assert Phi.shape == (g.N, nl), f"Phi must be {g.N} x {nl}"
+    assert Phi.shape == (g.dim, nl), f"Phi must be {g.dim} x {nl}"
     outs = [np.zeros(nl) for _ in range(5)]
     if nl == 0:
         return tuple(outs)
@@ -289,6 +291,10 @@ def compute_spectrum_beyn(solver, billiard: Billiard, k1: float, k2: float, *, m
     r = m + 15 if r is None else r
     ctx = ctx or default_context()
     fundamental = solver.symmetry is not None
+    if fundamental and billiard.area_fundamental is None:
+        # the leading Weyl term of one symmetry sector
+        billiard = dataclasses.replace(billiard,
+                                       area_fundamental=billiard.area / solver.symmetry.order)
     intervals = plan_weyl_windows(billiard, k1, k2, m=m, fundamental=fundamental, Rmax=Rmax)
     if len(intervals) >= 2:
         kL2, kR2 = intervals[-2]
@@ -303,7 +309,7 @@ def compute_spectrum_beyn(solver, billiard: Billiard, k1: float, k2: float, *, m
     if nq <= 15:
         raise ValueError("Do not use less than 15 contour nodes")
     all_pts = [evaluate_points(solver, billiard, float(k.real)) for k in k0]
-    geoms = [Geometry(p, ctx) for p in all_pts]
+    geoms = [Geometry(p, ctx, solver) for p in all_pts]
     try:
         if use_chebyshev:
             imax = int(np.argmax(k0.real + R))
@@ -339,14 +345,14 @@ def compute_spectrum_beyn(solver, billiard: Billiard, k1: float, k2: float, *, m
                 tens_l.append(np.abs(li[idx].imag))
                 tensN_l.append(resid[i])
                 phi_l.append(Uks[i] @ Ys[i][:, idx] if len(idx) else
-                             np.zeros((boundary_matrix_size(all_pts[i]), 0), np.complex128))
+                             np.zeros((boundary_matrix_size_for_solver(solver, all_pts[i]), 0), np.complex128))
         else:
             for i in range(len(k0)):
                 if len(lams[i]) == 0:
                     ks_l.append(np.zeros(0, np.complex128 if return_imag_part else np.float64))
                     tens_l.append(np.zeros(0))
                     tensN_l.append(np.zeros(0))
-                    phi_l.append(np.zeros((boundary_matrix_size(all_pts[i]), 0), np.complex128))
+                    phi_l.append(np.zeros((boundary_matrix_size_for_solver(solver, all_pts[i]), 0), np.complex128))
                     continue
                 idx, Phi, traw, tnorm, _ = residual_and_norm_select(
                     solver, lams[i], Uks[i], Ys[i], k0[i], R[i], all_pts[i], res_tol=res_tol,
@@ -377,7 +383,7 @@ def _window_B(solver, pts, geom, k0, R, nq, r, svd_tol, precision, refine_tol, g
     from .beyn import Window
     win = Window(solver, pts, k0, R, nq, precision=precision, geom=geom, **cheb)
     try:
-        return construct_B_matrix(solver, pts, geom.N, k0, R, nq=nq, r=r, svd_tol=svd_tol,
+        return construct_B_matrix(solver, pts, geom.dim, k0, R, nq=nq, r=r, svd_tol=svd_tol,
                                   rng=np.random.default_rng(0), precision=precision,
                                   refine_tol=refine_tol, grow_r=grow_r, window=win, **cheb)
     finally:
