@@ -55,6 +55,11 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second")
     ap.add_argument("--serve-kmax", type=float, default=1000.0)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--symmetry", action="store_true",
+                    help="the same disk on the C_5-reduced star: one window per irrep "
+                         "(Rotation(5, m), m = 0..4), eigenvalues checked against the "
+                         "unreduced FP64 window")
+    ap.add_argument("--sym-precision", default="mixed")
     return ap.parse_args()
 
 
@@ -86,6 +91,83 @@ def run_serve(a):
                    "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns)}}), flush=True)
 
 
+def run_symmetry(a):
+    """Symmetry-reduced windows (_all_k_symm_CFIE_chebyshev!, cheb-CFIE:698-824,
+    as the reference's examples run the star, example_beyn_kress.jl:51): the
+    disk |k - k0| <= R of star(1, 0.2, 5) with N nodes (a multiple of 10) as
+    five reduced windows of N/5 unknowns, one per C_5 irrep.  Prints one JSON
+    line: eigenvalues/s over all irreps, per-irrep window times, and the match
+    of the union of the irrep spectra against the unreduced FP64 window."""
+    import qbhip as q
+    ctx = q.default_context()
+    b = q.make_star_polar_boundary(1.0, 0.2, 5)
+    N = a.N + (-a.N) % 10
+    full = q.CFIE_kress(10.0, b)
+    pts = q.evaluate_points(full, b, a.k0, N=[N])
+    gfull = q.Geometry(pts, ctx)
+    zj = a.k0 + a.R * np.exp(2j * np.pi * np.arange(a.nq) / a.nq)
+    nh, mh, nj, mj, errs = q.chebyshev_params(full, pts, zj, tol=a.cheb_tol, geom=gfull)
+    cheb = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
+    solvers = [q.CFIE_kress(10.0, b, symmetry=q.Rotation(5, m)) for m in range(5)]
+    geoms = [q.Geometry(pts, ctx, s) for s in solvers]
+    m_dim = geoms[0].dim
+    bf = q.make_star_polar_boundary(1.0, 0.2, 5)
+    bf.area = b.area / 5  # probes from the Weyl count of one sector
+    bf.length = b.length / 5
+    r_irrep = a.r or q.probe_count(bf, a.k0, a.R, a.nq, a.svd_tol, 32)
+
+    def one_pass(rs, seed):
+        lams, infos, t = [], [], []
+        for m in range(5):
+            lam, dt, info = window(q, solvers[m], pts, geoms[m], a, cheb, a.sym_precision, rs[m],
+                                   seed, a.r == 0)
+            lams.append(lam)
+            infos.append(info)
+            t.append(dt)
+        return lams, infos, t
+
+    rs = [r_irrep] * 5
+    for w in range(max(1, a.warmup)):
+        lams, infos, t = one_pass(rs, 1)
+        rs = [i["r"] for i in infos]
+        print(f"# warmup {w}: {sum(t):.2f}s r={rs} rk={[i['rk'] for i in infos]}",
+              file=sys.stderr, flush=True)
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    passes = []
+    for s in range(a.steps):
+        passes.append(one_pass(rs, 1))
+    span = time.perf_counter() - t0
+    ks = ctx.kernel_stats()
+    ctx.set_profiling(False)
+    # the unreduced FP64 window of the same disk as the reference spectrum
+    lam_ref, t_full, _ = window(q, full, pts, gfull, a, cheb, "fp64",
+                                a.r or q.probe_count(b, a.k0, a.R, a.nq, a.svd_tol, a.r0), 1,
+                                a.r == 0)
+    lams = np.concatenate(passes[-1][0])
+    n, worst, nin = match_count(lams, lam_ref, a.k0, a.R)
+    nref = int(np.count_nonzero(np.abs(lam_ref - a.k0) <= a.R))
+    per_pass = span / a.steps
+    out = {
+        "metric": "eigenvalues/sec, C5-reduced Beyn CFIE star N=%d (5 irrep windows of %d)"
+                  % (N, m_dim),
+        "value": n / per_pass, "unit": "eigenvalues/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": per_pass * 1e3, "higher_is_better": True,
+        "dtype": "c64+c128-refine" if a.sym_precision == "mixed" else "c128",
+        "config": {"workload": "beyn-cfie-kress star(1,0.2,5) Rotation(5,m) m=0..4 k0=%g R=%g"
+                               % (a.k0, a.R), "N": N, "m": m_dim, "nq": a.nq, "r": rs,
+                   "cheb": cheb, "precision": a.sym_precision},
+        "irrep_window_s": [round(x, 4) for x in passes[-1][2]],
+        "eigenvalues_matched": n, "eigenvalues_in_disk_reduced": nin,
+        "eigenvalues_in_disk_full_fp64": nref, "max_rel_eig_err_vs_full_fp64": worst,
+        "full_fp64_window_s": t_full,
+        "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]} for k, v in ks.items()},
+    }
+    for g in geoms + [gfull]:
+        g.close()
+    print(json.dumps(out), flush=True)
+
+
 def match_count(lam, ref, k0, R, tol=1e-10):
     """Eigenvalues inside the disk that match a reference eigenvalue."""
     lam = lam[np.abs(lam - k0) <= R]
@@ -108,7 +190,7 @@ def match_count(lam, ref, k0, R, tol=1e-10):
 def window(q, solver, pts, geom, a, cheb, precision, r, rng_seed, grow):
     t0 = time.perf_counter()
     win = q.Window(solver, pts, complex(a.k0), a.R, a.nq, precision=precision, geom=geom, **cheb)
-    B, Uk, info = q.construct_B_matrix(solver, pts, geom.N, complex(a.k0), a.R, nq=a.nq, r=r,
+    B, Uk, info = q.construct_B_matrix(solver, pts, geom.dim, complex(a.k0), a.R, nq=a.nq, r=r,
                                        svd_tol=a.svd_tol, rng=np.random.default_rng(rng_seed),
                                        precision=precision, refine_tol=a.refine_tol,
                                        grow_r=grow, window=win, return_info=True)
@@ -224,6 +306,9 @@ def main():
         sys.exit(2)
     if a.serve:
         run_serve(a)
+        return
+    if a.symmetry:
+        run_symmetry(a)
         return
     import qbhip as q
     ctx = q.default_context()

@@ -505,13 +505,66 @@ extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
     qb_dev_free(ctx, buf, bytes);
     return qb_fail(ctx, QB_E_HIP, std::string("orbit upload: ") + hipGetErrorString(e));
   }
+  DevSym sym;
+  sym.m = m;
+  sym.G = G;
+  sym.img = di;
+  sym.scale = ds;
+  // Pair sharing (k_assemble_red_pair): find, for each image l, the image
+  // inv[l] whose distances mirror those of l on a few sample pairs, then
+  // verify the identity on every (a, b, l) on the device.
+  {
+    std::vector<double> hx(N), hy(N);
+    e = hipMemcpyAsync(hx.data(), g->d.x, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hy.data(), g->d.y, N * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+      qb_dev_free(ctx, buf, bytes);
+      return qb_fail(ctx, QB_E_HIP, std::string("orbit check: ") + hipGetErrorString(e));
+    }
+    auto dist = [&](int64_t p, int64_t q) { return hypot(hx[p] - hx[q], hy[p] - hy[q]); };
+    double scale = 0.0;
+    for (int64_t q = 0; q < N; ++q) scale = std::max(scale, std::max(fabs(hx[q]), fabs(hy[q])));
+    // a few ulps of the coordinate scale: the rounding of any node distance
+    const double floor_abs = 16.0 * 2.220446049250313e-16 * scale;
+    const int64_t samp[4] = {0, m / 3, (2 * m) / 3, m - 1};
+    bool found_all = true;
+    for (int l = 0; l < G && found_all; ++l) {
+      int pick = -1;
+      for (int c = 0; c < G && pick < 0; ++c) {
+        bool okc = true;
+        for (int sa = 0; sa < 4 && okc; ++sa)
+          for (int sb = 0; sb < 4 && okc; ++sb) {
+            const int64_t a = samp[sa], b = samp[sb];
+            const double r1 = dist(im[a * G], im[b * G + l]);
+            const double r2 = dist(im[b * G], im[a * G + c]);
+            okc = fabs(r1 - r2) <= 1e-13 * r1 + floor_abs;
+          }
+        if (okc) pick = c;
+      }
+      if (pick < 0) found_all = false;
+      else sym.inv[l] = (int8_t)pick;
+    }
+    int ok = 0;
+    if (found_all) {
+      rc = qb_orbit_pairs_ok(ctx, g->d, sym, floor_abs, &ok);
+      if (rc) {
+        qb_dev_free(ctx, buf, bytes);
+        return rc;
+      }
+    }
+    sym.pair = ok != 0 && getenv("QB_SYM_NO_PAIR") == nullptr;
+    if (getenv("QB_SYM_TRACE")) {
+      fprintf(stderr, "[qb] orbits m=%lld G=%d pair=%d inv=", (long long)m, G, (int)sym.pair);
+      for (int l = 0; l < G; ++l) fprintf(stderr, "%d%s", found_all ? sym.inv[l] : -1, l + 1 < G ? "," : "\n");
+    }
+  }
   qb_dev_free(ctx, g->sym_buf, (size_t)g->sym_bytes);
   g->sym_buf = buf;
   g->sym_bytes = (int64_t)bytes;
-  g->sym.m = m;
-  g->sym.G = G;
-  g->sym.img = di;
-  g->sym.scale = ds;
+  g->sym = sym;
   return QB_OK;
 }
 

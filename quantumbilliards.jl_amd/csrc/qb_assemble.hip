@@ -653,6 +653,170 @@ __global__ void __launch_bounds__(256) k_assemble_red(DevGeom g, DevSym s, const
   }
 }
 
+// Pair-shared reduced assembly (geometry with verified isometric orbits,
+// DevSym::pair).  Workgroups walk 64 x 64 tile pairs (I, J), I >= J, as
+// k_assemble does.  Off the diagonal a thread evaluates the specials of
+// r = |x_{rep_a} - x_{img[b,l]}| once per l and uses them for the image term
+// of entry (a, b) and for the image term inv[l] of entry (b, a), whose
+// distance |x_{rep_b} - x_{img[a,inv[l]]}| is the same; every other factor
+// (direction, tangent, component, Kress R and log term) is computed for each
+// entry from its own nodes.  Entry (b, a) is staged in LDS and written as
+// columns of the mirror tile.  Diagonal tiles are assembled entry by entry.
+template <int OP, int MH, int MJ>
+__global__ void __launch_bounds__(256) k_assemble_red_pair(DevGeom g, DevSym s,
+                                                           const cd* __restrict__ zk,
+                                                           const cd* __restrict__ H,
+                                                           const cd* __restrict__ J, int np_h,
+                                                           int M_h, int np_j, int M_j,
+                                                           double rmin_h, double rmax, int64_t q0,
+                                                           int T, cd* __restrict__ out64,
+                                                           cf* __restrict__ out32) {
+  __shared__ cd mir[64][65];  // mir[a_local][b_local] = A_red[b, a]
+  const int64_t m = s.m;
+  const int G = s.G;
+  const int64_t q = q0 + blockIdx.z;
+  int64_t p = blockIdx.x;
+  int d = 0;
+  while (p >= T - d) {
+    p -= T - d;
+    ++d;
+  }
+  const int Jt = (int)p, It = Jt + d;
+  const TabView tv = tab_view(H, J, q, np_h, M_h, np_j, M_j, rmin_h, rmax);
+  const cd k = zk[q];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t a = (int64_t)It * 64 + il;
+  const bool aok = a < m;
+  const bool diag_tile = (It == Jt);
+  const int64_t base = (q - q0) * m * m;
+  int64_t i = 0;
+  double xi = 0, yi = 0;
+  int ci = 0;
+  if (aok) {
+    i = s.img[a * G];
+    xi = g.x[i];
+    yi = g.y[i];
+    ci = g.comp[i];
+  }
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int jl = jl0 + 4 * c;
+    const int64_t b = (int64_t)Jt * 64 + jl;
+    if (!aok || b >= m) continue;
+    cd acc = mk(0, 0), accm = mk(0, 0);
+    if (diag_tile) {
+#pragma unroll 1
+      for (int l = 0; l < G; ++l) {
+        const int64_t j = s.img[b * G + l];
+        cd v;
+        if (i == j) {
+          v = diag_entry<OP>(g, j, k);
+        } else {
+          const PairGeo pg = pair_geo(g, i, j, xi, yi, ci);
+          const Spec sp = specials<OP, MH, MJ>(tv, pg.r, k, pg.same || OP == QB_DLP_KRESS);
+          v = offdiag_entry<OP>(g, i, j, pg.dx, pg.dy, pg.invr, pg.same, pg.Rij, pg.lt, k, sp);
+        }
+        acc = acc + s.scale[b * G + l] * v;
+      }
+    } else {
+      const int64_t ib = s.img[b * G];
+      const double xb = g.x[ib], yb = g.y[ib];
+      const int cb = g.comp[ib];
+#pragma unroll 1
+      for (int l = 0; l < G; ++l) {
+        const int64_t j = s.img[b * G + l];
+        const int ls = s.inv[l];
+        const int64_t jm = s.img[a * G + ls];
+        const PairGeo p1 = pair_geo(g, i, j, xi, yi, ci);
+        PairGeo p2 = pair_geo(g, ib, jm, xb, yb, cb);
+        const Spec sp = specials<OP, MH, MJ>(tv, p1.r, k,
+                                             p1.same || p2.same || OP == QB_DLP_KRESS);
+        const cd v1 = offdiag_entry<OP>(g, i, j, p1.dx, p1.dy, p1.invr, p1.same, p1.Rij, p1.lt,
+                                        k, sp);
+        const cd v2 = offdiag_entry<OP>(g, ib, jm, p2.dx, p2.dy, p1.invr, p2.same, p2.Rij,
+                                        p2.lt, k, sp);
+        acc = acc + s.scale[b * G + l] * v1;
+        accm = accm + s.scale[a * G + ls] * v2;
+      }
+      mir[il][jl] = accm;
+    }
+    const int64_t idx = base + a + b * m;
+    if (out64) out64[idx] = acc;
+    if (out32) out32[idx] = to_cf(acc);
+  }
+  if (diag_tile) return;  // uniform over the workgroup
+  __syncthreads();
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int il2 = jl0 + 4 * c;       // column a of the mirror tile
+    const int jl2 = threadIdx.x & 63;  // row b inside it
+    const int64_t aa = (int64_t)It * 64 + il2, bb = (int64_t)Jt * 64 + jl2;
+    if (aa >= m || bb >= m) continue;
+    const cd v = mir[il2][jl2];
+    const int64_t idx = base + bb + aa * m;
+    if (out64) out64[idx] = v;
+    if (out32) out32[idx] = to_cf(v);
+  }
+}
+
+// Largest mismatch | |x_{rep_a} - x_{img[b,l]}| - |x_{rep_b} - x_{img[a,inv[l]]}| |
+// / (1e-13 r + floor_abs) over one 64 x 64 tile of (a, b) and all l, one value
+// per workgroup (<= 1: the distances mirror).  floor_abs is a few ulps of the
+// coordinate scale, the rounding every distance carries from its nodes.
+__global__ void __launch_bounds__(256) k_orbit_check(DevGeom g, DevSym s, double floor_abs,
+                                                     double* out) {
+  __shared__ double red[256];
+  const int64_t m = s.m;
+  const int G = s.G;
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t a = (int64_t)blockIdx.x * 64 + il;
+  double worst = 0.0;
+  if (a < m) {
+    const int64_t i = s.img[a * G];
+    for (int c = 0; c < 16; ++c) {
+      const int64_t b = (int64_t)blockIdx.y * 64 + jl0 + 4 * c;
+      if (b >= m) break;
+      const int64_t ib = s.img[b * G];
+      for (int l = 0; l < G; ++l) {
+        const int64_t j = s.img[b * G + l];
+        const int64_t jm = s.img[a * G + s.inv[l]];
+        const double r1 = hypot(g.x[i] - g.x[j], g.y[i] - g.y[j]);
+        const double r2 = hypot(g.x[ib] - g.x[jm], g.y[ib] - g.y[jm]);
+        const double dd = fabs(r1 - r2) / (1e-13 * r1 + floor_abs);
+        worst = fmax(worst, (i == j) != (ib == jm) ? 1e300 : dd);
+      }
+    }
+  }
+  red[threadIdx.x] = worst;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+int qb_orbit_pairs_ok(qb_ctx* ctx, const DevGeom& g, const DevSym& s, double floor_abs, int* ok) {
+  *ok = 0;
+  const unsigned Tm = (unsigned)((s.m + 63) / 64);
+  const size_t nb = (size_t)Tm * Tm;
+  double* d = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&d, nb * sizeof(double));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_orbit_check, dim3(Tm, Tm), dim3(256), 0, ctx->stream, g, s, floor_abs, d);
+  std::vector<double> h(nb);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), d, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, d, nb * sizeof(double));
+  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, std::string("orbit check: ") + hipGetErrorString(e));
+  double worst = 0.0;
+  for (double v : h) worst = v > worst ? v : worst;
+  *ok = worst <= 1.0 ? 1 : 0;
+  return QB_OK;
+}
+
 template <int OP, int MH, int MJ>
 __global__ void __launch_bounds__(256) k_assemble_red_deriv(
     DevGeom g, DevSym s, const cd* __restrict__ zk, const cd* __restrict__ H,
@@ -707,9 +871,16 @@ static void launch_red(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, i
   const double ent = (double)m * m * nzc;
   int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
                            ent * ((o64 ? sizeof(cd) : 0) + (o32 ? sizeof(cf) : 0)));
-  hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ>), dim3(Tm, Tm, (unsigned)nzc), dim3(256), 0,
-                     ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j,
-                     t.rmin_h, t.rmax, q0, o64, o32);
+  if (g->sym.pair) {
+    const int64_t pairs = (int64_t)Tm * (Tm + 1) / 2;
+    hipLaunchKernelGGL((k_assemble_red_pair<OP, MH, MJ>), dim3((unsigned)pairs, 1, (unsigned)nzc),
+                       dim3(256), 0, ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h,
+                       t.np_j, t.M_j, t.rmin_h, t.rmax, q0, (int)Tm, o64, o32);
+  } else {
+    hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ>), dim3(Tm, Tm, (unsigned)nzc), dim3(256), 0,
+                       ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j,
+                       t.rmin_h, t.rmax, q0, o64, o32);
+  }
   qb_prof_end(ctx, slot);
 }
 

@@ -41,6 +41,13 @@ struct DevSym {
   int32_t G = 0;  // orbit size (group order)
   const int32_t* img = nullptr;
   const cd* scale = nullptr;
+  // Pair sharing: when the images come from isometries of the node set
+  // (img[b*G + l] = g_l(rep_b) for one g_l per l), |x_{rep_a} - x_{img[b,l]}|
+  // = |x_{rep_b} - x_{img[a, inv[l]]}| with g_{inv[l]} = g_l^{-1}, so the
+  // specials of one distance serve entry (a, b) and entry (b, a).  pair is set
+  // only after that identity was verified for every (a, b, l).
+  bool pair = false;
+  int8_t inv[64] = {0};
 };
 
 struct QbProfRec {
@@ -161,6 +168,9 @@ int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
 // when g->sym.m > 0.
 int qb_assemble_red_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
                         const DevTables& t, cd* out64, cf* out32);
+// Checks |x_{rep_a} - x_{img[b,l]}| = |x_{rep_b} - x_{img[a,inv[l]]}| to
+// 1e-13 r + floor_abs for every (a, b, l) on the device; ok = 1 if it holds.
+int qb_orbit_pairs_ok(qb_ctx* ctx, const DevGeom& g, const DevSym& s, double floor_abs, int* ok);
 int qb_assemble_red_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
                               int64_t nz, const DevTables& t, cd* out, cd* dout, cd* ddout,
                               double off_sign);

@@ -85,6 +85,51 @@ end
 geom(p::QB.BoundaryPointsCFIE{Float64}) = geom([p])
 geom_destroy(g) = ccall((:qb_geom_destroy, LIB), Cvoid, (Ptr{Cvoid},), g)
 
+# Orbit table of the solver's reduced workspace, from the reference's own
+# orbit builders: CFIE_kress visits the nodes in order with the coordinate maps
+# of build_symmetry_maps (the loop of build_CFIE_kress_reduced_workspace,
+# cheb-CFIE:482-502); DLP_kress takes symmetry_index_orbits (dlp_kress.jl:703).
+function orbit_table(solver, pts)
+    sym = solver.symmetry
+    if opcode(solver) == QB_DLP_KRESS
+        p = pts isa Vector ? only(pts) : pts
+        _, _, _, f2full, f2scale = QB.symmetry_index_orbits(Float64, p, sym, solver.billiard)
+    else
+        pv = pts isa Vector ? pts : [pts]
+        maps = QB.build_symmetry_maps(pv, sym)
+        Nfull = sum(length(p.xy) for p in pv)
+        visited = falses(Nfull)
+        f2full = Vector{Vector{Int}}()
+        f2scale = Vector{Vector{ComplexF64}}()
+        for g in 1:Nfull
+            visited[g] && continue
+            orb = QB.apply_symmetry_orbit_from_maps(maps, sym, g)
+            push!(f2full, orb.indices)
+            push!(f2scale, ComplexF64.(orb.scales))
+            visited[orb.indices] .= true
+        end
+    end
+    G = length(f2full[1])
+    img = Int64[f2full[b][l] - 1 for l in 1:G, b in eachindex(f2full)]   # img[l, b], 0-based
+    sc = ComplexF64[f2scale[b][l] for l in 1:G, b in eachindex(f2full)]
+    return img, sc, length(f2full), G
+end
+
+# geometry of a solver: with a symmetry, the orbits are attached and every
+# assembly on it is the reduced one (qb_geom_set_orbits)
+function geom(solver, pts)
+    g = geom(pts)
+    if !isnothing(solver.symmetry)
+        img, sc, m, G = orbit_table(solver, pts)
+        rc = ccall((:qb_geom_set_orbits, LIB), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Int32, Ptr{Int64}, Ptr{ComplexF64}),
+                   ctx(), g, Int64(m), Int32(G), img, sc)
+        rc == 0 || (geom_destroy(g); check(rc))
+    end
+    return g
+end
+geom_dim(g) = Int(ccall((:qb_geom_dim, LIB), Int64, (Ptr{Cvoid},), g))
+
 # construct_boundary_matrices! (cheb-CFIE:1431-1450, cheb-DLP:1110-1124)
 function construct_boundary_matrices!(Tbufs::Vector{Matrix{ComplexF64}}, solver, pts,
                                       zj::AbstractVector{ComplexF64}; multithreaded::Bool=true,
@@ -94,9 +139,9 @@ function construct_boundary_matrices!(Tbufs::Vector{Matrix{ComplexF64}}, solver,
     use_chebyshev || error("Direct complex-k construction is not implemented.")
     Mk = length(zj)
     @assert length(Tbufs) == Mk
-    g = geom(pts)
+    g = geom(solver, pts)
     try
-        N = Int(ccall((:qb_geom_size, LIB), Int64, (Ptr{Cvoid},), g))
+        N = geom_dim(g)
         for T in Tbufs
             @assert size(T) == (N, N)
         end
@@ -133,9 +178,9 @@ function construct_boundary_matrices_with_derivatives!(Tbufs::Vector{Matrix{Comp
     use_chebyshev || error("Direct derivative matrix construction is only for real k currently")
     Mk = length(zj)
     @assert length(Tbufs) == Mk && length(dTbufs) == Mk && length(ddTbufs) == Mk
-    g = geom(pts)
+    g = geom(solver, pts)
     try
-        N = Int(ccall((:qb_geom_size, LIB), Int64, (Ptr{Cvoid},), g))
+        N = geom_dim(g)
         outs = [Array{ComplexF64}(undef, N, N, Mk) for _ in 1:3]
         cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
         check(ccall((:qb_assemble_deriv, LIB), Cint,
@@ -161,7 +206,7 @@ function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float6
                             n_panels_j::Int=3000, M_j::Int=5, info::Bool=false,
                             multithreaded::Bool=true, precision::Symbol=:fp64,
                             refine_tol::Float64=1e-12)
-    g = geom(pts)
+    g = geom(solver, pts)                          # N = geom_dim(g) with a symmetry
     w = Ref{Ptr{Cvoid}}(C_NULL)
     st = Ref(Stats())
     cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
@@ -223,7 +268,7 @@ function residual_and_norm_select(solver, λ::AbstractVector{ComplexF64},
     if !isempty(inside)
         nl = length(inside)
         r2 = zeros(nl); r1 = zeros(nl); rinf = zeros(nl); n1 = zeros(nl); ninf = zeros(nl)
-        g = geom(pts)
+        g = geom(solver, pts)
         try
             cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
             check(ccall((:qb_residuals, LIB), Cint,

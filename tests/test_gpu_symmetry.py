@@ -82,15 +82,24 @@ CASES = [
     ("circle_hole", [320, 96], "cfie", "x_axis", 0, 0, -1, HOLE),
     ("ellipse", [480], "dlp", "origin", 0, 0, (1, -1), None),
     ("star", [600], "dlp", "rot", 5, 4, 1, None),
+    # index orbits of a rotation the ellipse does not have: the reference sums
+    # them all the same; the distances do not mirror, so no pair sharing
+    ("ellipse", [480], "dlp", "rot", 4, 1, 1, None),
 ]
+# whether the orbit distances mirror (pair-shared kernel) for each case
+PAIRED = [True, True, True, True, True, True, True, False]
 
 
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_reduced_assembly_matches_oracle(case):
+def test_reduced_assembly_matches_oracle(case, capfd, monkeypatch):
     kind, Ns, op, sk, n, m, p, hole = CASES[case]
     print("case", CASES[case], flush=True)
     q = qb()
+    monkeypatch.setenv("QB_SYM_TRACE", "1")
     solver, pts, comps, img, sc = _setup(kind, Ns, op, sk, n, m, p, hole)
+    q.Geometry(pts, None, solver).close()
+    err = capfd.readouterr().err
+    assert ("pair=1" if PAIRED[case] else "pair=0") in err, err
     zj, _ = O.contour(30.0, 0.5, 4)
     cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
     T = q.assemble(solver, pts, zj, **cp)
