@@ -141,7 +141,7 @@ int qb_ctx_create(int device, qb_ctx** out) {
   qb_ctx* c = new qb_ctx();
   c->device = device;
   for (int t = 0; t < 2; ++t)
-    for (int k = 0; k < QB_GEMM_CLASSES; ++k) c->gemm_pick[t][k] = -1;
+    for (int k = 0; k < QB_GEMM_CLASSES; ++k) c->gemm_pick[t][k] = c->gemm_pick_conc[t][k] = -1;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream_aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming) != hipSuccess ||
@@ -1478,9 +1478,12 @@ extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, in
     int v = sh.ty == 0 ? qb_gemm_tune<cf>(ctx, sh.M, sh.N, sh.K, nq, &tf)
                        : qb_gemm_tune<cd>(ctx, sh.M, sh.N, sh.K, nq, &tf);
     if (v < 0) return v;
-    char line[160];
-    snprintf(line, sizeof(line), "%s %dx%dx%d -> %s (%.1f TF/s)\n", sh.ty == 0 ? "c64" : "c128",
-             sh.M, sh.N, sh.K, qb_gemm_variant_name(sh.ty, v), tf);
+    char line[200];
+    const int vc = ctx->gemm_pick_conc[sh.ty][cls];
+    snprintf(line, sizeof(line), "%s %dx%dx%d -> %s (%.1f TF/s)%s%s\n", sh.ty == 0 ? "c64" : "c128",
+             sh.M, sh.N, sh.K, qb_gemm_variant_name(sh.ty, v), tf,
+             vc != v ? "; beside the lookahead panel: " : "",
+             vc != v ? qb_gemm_variant_name(sh.ty, vc) : "");
     rep += line;
   }
   if (report && report_len > 0) {

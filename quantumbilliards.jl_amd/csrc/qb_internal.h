@@ -9,7 +9,7 @@
 #include "../../include/qb.h"
 
 #define QB_MAX_COMP 8
-#define QB_GEMM_VARIANTS 9
+#define QB_GEMM_VARIANTS 11
 #define QB_GEMM_CLASSES 45
 
 // Boundary geometry resident on the device.  Node arrays are concatenated
@@ -64,6 +64,11 @@ struct qb_ctx {
   std::string err;
   int64_t bytes = 0;  // device bytes held by objects of this context
   int gemm_pick[2][QB_GEMM_CLASSES];  // autotuned tile variant per shape class (-1: default)
+  // The same among the tiles that leave room on each CU for other work: used
+  // while gemm_conc is set, i.e. for the LU trailing update that runs beside
+  // the lookahead panel factorisation on the second stream.
+  int gemm_pick_conc[2][QB_GEMM_CLASSES];
+  bool gemm_conc = false;
   // Node list (device array) for batched launches: when set, batch entry z
   // of a GEMM / TRSM / gather / node-wise helper works on matrix bix[z]
   // (strides unchanged), so a subset of a window's nodes runs in one launch.

@@ -20,6 +20,7 @@
 // complex128) with accumulation in the operand precision.
 #include <stdlib.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "qb_internal.h"
@@ -68,8 +69,8 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
 // both the coalesced global loads and the fragment reads are conflict-free.  A is
 // negated when it is written to LDS, so the accumulators hold -A*B and the
 // epilogue adds them to C.
-template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD, bool G3>
-__global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD, bool G3, int PF>
+__global__ void __launch_bounds__(256, PF == 2 ? 2 : 1) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
                                                   int64_t lda, int64_t sA,
                                                   const T* __restrict__ B, int64_t ldb,
                                                   int64_t sB, T* __restrict__ C, int64_t ldc,
@@ -133,23 +134,27 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
       if constexpr (G3) acc_s[i][j] = V4{0, 0, 0, 0};
     }
 
-  T ra[NA], rb[NB];
+  // PF register slots stage k-tiles on their way from global memory to LDS:
+  // with PF = 2 the loads of k-tile t+2 are issued while k-tile t is
+  // multiplied, so they have two k-tiles of MFMA work to arrive in.
+  T ra[PF][NA], rb[PF][NB];
   // tiles fully inside M x N (all but the edge tiles) load k-tiles that are
   // fully inside K without per-element bounds checks
   const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N);
-  auto gload = [&](int k0) {
+  auto gload = [&](auto slot, int k0) {
+    constexpr int sl = decltype(slot)::value;
     if (full_mn && k0 + BK <= K) {
 #pragma unroll
       for (int p = 0; p < NA; ++p) {
         int e = tid + 256 * p;
         int m = e % BM, kk = e / BM;
-        ra[p] = A[(m0 + m) + (int64_t)(k0 + kk) * lda];
+        ra[sl][p] = A[(m0 + m) + (int64_t)(k0 + kk) * lda];
       }
 #pragma unroll
       for (int p = 0; p < NB; ++p) {
         int e = tid + 256 * p;
         int kk = e % BK, n = e / BK;
-        rb[p] = B[(k0 + kk) + (int64_t)(n0 + n) * ldb];
+        rb[sl][p] = B[(k0 + kk) + (int64_t)(n0 + n) * ldb];
       }
       return;
     }
@@ -158,40 +163,37 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
       int e = tid + 256 * p;
       int m = e % BM, kk = e / BM;
       int gm = m0 + m, gk = k0 + kk;
-      ra[p] = (gm < M && gk < K) ? A[gm + (int64_t)gk * lda] : T{0, 0};
+      ra[sl][p] = (gm < M && gk < K) ? A[gm + (int64_t)gk * lda] : T{0, 0};
     }
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
       int e = tid + 256 * p;
       int kk = e % BK, n = e / BK;
       int gn = n0 + n, gk = k0 + kk;
-      rb[p] = (gn < N && gk < K) ? B[gk + (int64_t)gn * ldb] : T{0, 0};
+      rb[sl][p] = (gn < N && gk < K) ? B[gk + (int64_t)gn * ldb] : T{0, 0};
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](auto slot, int buf) {
+    constexpr int sl = decltype(slot)::value;
 #pragma unroll
     for (int p = 0; p < NA; ++p) {
       int e = tid + 256 * p;
       int m = e % BM, kk = e / BM;
-      As[buf][0][kk * AST + m] = -ra[p].re;
-      As[buf][1][kk * AST + m] = -ra[p].im;
+      As[buf][0][kk * AST + m] = -ra[sl][p].re;
+      As[buf][1][kk * AST + m] = -ra[sl][p].im;
     }
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
       int e = tid + 256 * p;
       int kk = e % BK, n = e / BK;
-      Bs[buf][0][n * BST + kk] = rb[p].re;
-      Bs[buf][1][n * BST + kk] = rb[p].im;
+      Bs[buf][0][n * BST + kk] = rb[sl][p].re;
+      Bs[buf][1][n * BST + kk] = rb[sl][p].im;
     }
   };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, PF - 1>;
 
-  const int nt = (K + BK - 1) / BK;
-  gload(0);
-  sstore(0);
-  __syncthreads();
-  for (int t = 0; t < nt; ++t) {
-    const int buf = t & 1;
-    if (t + 1 < nt) gload((t + 1) * BK);
+  auto compute = [&](int buf) {
     const S* Ar = As[buf][0];
     const S* Ai = As[buf][1];
     const S* Br = Bs[buf][0];
@@ -238,8 +240,39 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
           }
       }
     }
-    if (t + 1 < nt) sstore(buf ^ 1);
+  };
+
+  const int nt = (K + BK - 1) / BK;
+  if constexpr (PF == 1) {
+    gload(S0{}, 0);
+    sstore(S0{}, 0);
     __syncthreads();
+    for (int t = 0; t < nt; ++t) {
+      const int buf = t & 1;
+      if (t + 1 < nt) gload(S0{}, (t + 1) * BK);
+      compute(buf);
+      if (t + 1 < nt) sstore(S0{}, buf ^ 1);
+      __syncthreads();
+    }
+  } else {
+    // k-tile t sits in LDS buffer t & 1 and came through slot t & 1; when it
+    // is multiplied, slot t & 1 is free for k-tile t+2 and slot (t+1) & 1
+    // holds k-tile t+1, stored to the other buffer at the end of the step.
+    gload(S0{}, 0);
+    if (nt > 1) gload(S1{}, BK);
+    sstore(S0{}, 0);
+    __syncthreads();
+    auto step = [&](auto P, int t) {
+      constexpr int p = decltype(P)::value;
+      if (t + 2 < nt) gload(std::integral_constant<int, p>{}, (t + 2) * BK);
+      compute(p);
+      if (t + 1 < nt) sstore(std::integral_constant<int, 1 - p>{}, 1 - p);
+      __syncthreads();
+    };
+    for (int t = 0; t < nt; t += 2) {
+      step(S0{}, t);
+      if (t + 1 < nt) step(S1{}, t + 1);
+    }
   }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -268,13 +301,13 @@ __global__ void __launch_bounds__(256) k_gemm_sub(int M, int N, int K, const T* 
 }
 
 template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD = false,
-          bool G3 = false>
+          bool G3 = false, int PF = 1>
 static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
                         const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                         int batch) {
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD, G3>), g, dim3(256), 0, ctx->stream, M, N, K,
-                     A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
+  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD, G3, PF>), g, dim3(256), 0,
+                     ctx->stream, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
 // Tile variants and the per-context choice.  Variant 0/1 are the defaults
@@ -289,18 +322,27 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 64, 256, 16, 1, 4>,       launch_gemm<cf, 128, 64, 16, 2, 2>,
     launch_gemm<cf, 128, 128, 32, 2, 2>,      launch_gemm<cf, 128, 64, 16, 2, 2, true>,
     launch_gemm<cf, 128, 128, 16, 2, 2, true>, launch_gemm<cf, 128, 64, 16, 2, 2, false, true>,
-    launch_gemm<cf, 128, 128, 16, 2, 2, false, true>};
+    launch_gemm<cf, 128, 128, 16, 2, 2, false, true>, launch_gemm<cf, 128, 64, 16, 2, 2, true, true>,
+    launch_gemm<cf, 64, 64, 16, 2, 2, false, true>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
     launch_gemm<cd, 256, 32, 16, 4, 1>,       launch_gemm<cd, 64, 64, 16, 2, 2, true>,
     launch_gemm<cd, 128, 64, 16, 2, 2, true>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true>,
-    launch_gemm<cd, 64, 32, 16, 2, 2, false, true>};
+    launch_gemm<cd, 64, 32, 16, 2, 2, false, true>, launch_gemm<cd, 64, 64, 16, 2, 2, true, false, 2>,
+    launch_gemm<cd, 64, 32, 16, 2, 2, false, true, 2>};
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
-     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m"},
+     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
-     "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m"}};
+     "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2"}};
+
+// Variants that pack 4 waves per SIMD (small tiles, low register and LDS
+// use): fastest alone, but they leave a concurrently running kernel (the
+// lookahead panel) few issue slots, so gemm_pick_conc excludes them.
+static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
+    {false, false, false, false, false, false, false, false, false, false, true},
+    {false, false, false, false, false, false, false, false, false, false, false}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);
@@ -332,7 +374,7 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
              int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
   const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
-  int v = ctx->gemm_pick[ty][qb_gemm_class(M, N, K)];
+  int v = (ctx->gemm_conc ? ctx->gemm_pick_conc : ctx->gemm_pick)[ty][qb_gemm_class(M, N, K)];
   if (v < 0) v = gemm_default<T>(N);
   int slot = qb_prof_begin(ctx, ty == 0 ? QB_K_GEMM_C64 : QB_K_GEMM_C128,
                            8.0 * M * N * K * (double)batch,
@@ -361,8 +403,8 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
   int rc = qb_dev_alloc(ctx, (void**)&A, (size_t)M * K * b * sizeof(T));
   if (!rc) rc = qb_dev_alloc(ctx, (void**)&B, (size_t)K * N * b * sizeof(T));
   if (!rc) rc = qb_dev_alloc(ctx, (void**)&C, (size_t)M * N * b * sizeof(T));
-  int best = gemm_default<T>(N);
-  double best_ms = 1e300;
+  int best = gemm_default<T>(N), best_conc = best;
+  double best_ms = 1e300, best_conc_ms = 1e300;
   if (!rc) {
     int64_t na = (int64_t)M * K * b * sizeof(T) / 8, nb = (int64_t)K * N * b * sizeof(T) / 8,
             nc = (int64_t)M * N * b * sizeof(T) / 8;
@@ -372,12 +414,23 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
     hipEvent_t e0 = nullptr, e1 = nullptr;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    // QB_TUNE_SKIP: comma-separated variant indices the tuner leaves out
-    // (a diagnostic for comparing tile families in whole windows)
+    // QB_TUNE_SKIP (both operand types), QB_TUNE_SKIP_C64, QB_TUNE_SKIP_C128:
+    // comma-separated variant indices the tuner leaves out (a diagnostic for
+    // comparing tile families in whole windows)
     std::vector<char> skip(QB_GEMM_VARIANTS, 0);
-    if (const char* sk = getenv("QB_TUNE_SKIP"))
-      for (const char* c = sk; *c; ++c)
-        if (*c >= '0' && *c <= '9' && *c - '0' < QB_GEMM_VARIANTS) skip[*c - '0'] = 1;
+    for (const char* env : {"QB_TUNE_SKIP", ty == 0 ? "QB_TUNE_SKIP_C64" : "QB_TUNE_SKIP_C128"}) {
+      const char* sk = getenv(env);
+      for (const char* c = sk; c && *c;) {
+        char* end = nullptr;
+        const long v = strtol(c, &end, 10);
+        if (end == c) {
+          ++c;
+          continue;
+        }
+        if (v >= 0 && v < QB_GEMM_VARIANTS) skip[v] = 1;
+        c = end;
+      }
+    }
     for (int v = 0; v < QB_GEMM_VARIANTS; ++v) {
       if (skip[v]) continue;
       gemm_launch_variant<T>(ctx, v, M, N, K, A, M, (int64_t)M * K, B, K, (int64_t)K * N, C, M,
@@ -390,15 +443,24 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
       (void)hipEventSynchronize(e1);
       float ms = 0.f;
       if (hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
-          ms > 0.f && ms < best_ms) {
-        best_ms = ms;
-        best = v;
+          ms > 0.f) {
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = v;
+        }
+        if (!kGemmPacks[ty][v] && ms < best_conc_ms) {
+          best_conc_ms = ms;
+          best_conc = v;
+        }
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (ctx->gemm_pick[ty][qb_gemm_class(M, N, K)] != best) ++ctx->gemm_epoch;
-    ctx->gemm_pick[ty][qb_gemm_class(M, N, K)] = best;
+    const int cls = qb_gemm_class(M, N, K);
+    if (ctx->gemm_pick[ty][cls] != best || ctx->gemm_pick_conc[ty][cls] != best_conc)
+      ++ctx->gemm_epoch;
+    ctx->gemm_pick[ty][cls] = best;
+    ctx->gemm_pick_conc[ty][cls] = best_conc;
   }
   (void)hipStreamSynchronize(ctx->stream);
   qb_dev_free(ctx, A, (size_t)M * K * b * sizeof(T));
@@ -831,8 +893,10 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
     if (rc) return rc;
     QB_HIP(ctx, hipEventRecord(ctx->ev_aux, aux));
     if (m > kb1) {
+      ctx->gemm_conc = true;  // beside the panel on the second stream
       rc = gemm_sub<T>(ctx, m, m - kb1, kb, A21, lda, sA, A12 + (int64_t)kb1 * lda, lda, sA,
                        A + k1 + (int64_t)(k1 + kb1) * lda, lda, sA, batch);
+      ctx->gemm_conc = false;
       if (rc) return rc;
     }
     QB_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_aux, 0));

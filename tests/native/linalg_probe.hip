@@ -142,6 +142,6 @@ extern "C" double qbt_copy_bench(qb_ctx* ctx, int64_t bytes, int iters) {
 // Force every shape class of one operand type to tile variant v (-1 restores
 // the default choice), so tests can exercise each variant.
 extern "C" void qbt_force_variant(qb_ctx* ctx, int ty, int v) {
-  for (int k = 0; k < QB_GEMM_CLASSES; ++k) ctx->gemm_pick[ty][k] = v;
+  for (int k = 0; k < QB_GEMM_CLASSES; ++k) ctx->gemm_pick[ty][k] = ctx->gemm_pick_conc[ty][k] = v;
   ++ctx->gemm_epoch;
 }
