@@ -69,8 +69,9 @@ __device__ __forceinline__ doublex4 mfma16(double a, double b, doublex4 c) {
 // both the coalesced global loads and the fragment reads are conflict-free.  A is
 // negated when it is written to LDS, so the accumulators hold -A*B and the
 // epilogue adds them to C.
-template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD, bool G3, int PF>
-__global__ void __launch_bounds__(256, PF == 2 ? 2 : 1) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
+template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD, bool G3, int PF,
+          int OCC>
+__global__ void __launch_bounds__(256, OCC) k_gemm_sub(int M, int N, int K, const T* __restrict__ A,
                                                   int64_t lda, int64_t sA,
                                                   const T* __restrict__ B, int64_t ldb,
                                                   int64_t sB, T* __restrict__ C, int64_t ldc,
@@ -300,13 +301,14 @@ __global__ void __launch_bounds__(256, PF == 2 ? 2 : 1) k_gemm_sub(int M, int N,
     }
 }
 
+// OCC: waves per SIMD the register allocation must allow (__launch_bounds__).
 template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD = false,
-          bool G3 = false, int PF = 1>
+          bool G3 = false, int PF = 1, int OCC = (PF == 2 ? 2 : 1)>
 static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
                         const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC,
                         int batch) {
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD, G3, PF>), g, dim3(256), 0,
+  hipLaunchKernelGGL((k_gemm_sub<T, BM, BN, BK, WGM, WGN, XCD, G3, PF, OCC>), g, dim3(256), 0,
                      ctx->stream, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
@@ -323,26 +325,29 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 128, 128, 32, 2, 2>,      launch_gemm<cf, 128, 64, 16, 2, 2, true>,
     launch_gemm<cf, 128, 128, 16, 2, 2, true>, launch_gemm<cf, 128, 64, 16, 2, 2, false, true>,
     launch_gemm<cf, 128, 128, 16, 2, 2, false, true>, launch_gemm<cf, 128, 64, 16, 2, 2, true, true>,
-    launch_gemm<cf, 64, 64, 16, 2, 2, false, true>};
+    launch_gemm<cf, 64, 64, 16, 2, 2, false, true>, launch_gemm<cf, 64, 64, 16, 2, 2, true, true>,
+    launch_gemm<cf, 64, 64, 16, 2, 2>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
     launch_gemm<cd, 256, 32, 16, 4, 1>,       launch_gemm<cd, 64, 64, 16, 2, 2, true>,
     launch_gemm<cd, 128, 64, 16, 2, 2, true>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true>,
     launch_gemm<cd, 64, 32, 16, 2, 2, false, true>, launch_gemm<cd, 64, 64, 16, 2, 2, true, false, 2>,
-    launch_gemm<cd, 64, 32, 16, 2, 2, false, true, 2>};
+    launch_gemm<cd, 64, 32, 16, 2, 2, false, true, 2>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true, 1, 2>,
+    launch_gemm<cd, 64, 64, 16, 2, 2, true, true, 2, 2>};
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
-     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m"},
+     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
-     "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2"}};
+     "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
+     "64x64x16/3m/xcd/pf2"}};
 
 // Variants that pack 4 waves per SIMD (small tiles, low register and LDS
 // use): fastest alone, but they leave a concurrently running kernel (the
 // lookahead panel) few issue slots, so gemm_pick_conc excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
-    {false, false, false, false, false, false, false, false, false, false, true},
-    {false, false, false, false, false, false, false, false, false, false, false}};
+    {false, false, false, false, false, false, false, false, false, false, true, true, true},
+    {false, false, false, false, false, false, false, false, false, false, false, false, false}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);

@@ -92,15 +92,16 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(11))
+@pytest.mark.parametrize("variant", range(13))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
     LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
     are multiples of 8 with a partial group of m-tiles, the case in which the
-    XCD-aware variants (5, 6, 9) remap tiles; variants 7 and 8 (and complex64
-    9, 10, complex128 10) form complex products from three real ones (Gauss);
-    complex128 9 and 10 stage k-tiles two ahead (K = 33, 40, 128 cover one,
-    two and an odd number of k-tiles)."""
+    XCD-aware variants (5, 6, 9, complex64 11, complex128 12) remap tiles;
+    variants 7 and 8 (and complex64 9-11, complex128 10-12) form complex
+    products from three real ones (Gauss); complex128 9, 10 and 12 stage
+    k-tiles two ahead (K = 33, 40, 128 cover one, two and an odd number of
+    k-tiles)."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
