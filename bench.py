@@ -28,13 +28,13 @@ PEAK = {"c64": 157.3, "c128": 78.6}  # dense MFMA TFLOP/s (fp32, fp64)
 # HBM bytes per complex64 GEMM launch over one window of the default workload,
 # from FETCH_SIZE / WRITE_SIZE counter runs scaled by a copy kernel of known
 # size (profiles/beyn_window_star_N8192_mixed_pmc_traffic.txt)
-TRAFFIC_C64_PER_LAUNCH = 687.6e6
+TRAFFIC_C64_PER_LAUNCH = 557.2e6
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=8192)
     ap.add_argument("--nq", type=int, default=48)
@@ -289,9 +289,16 @@ def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
         t_start = time.perf_counter()
         win = q.Window(solver, pts, complex(a.k0), a.R, a.nq, precision=precision, geom=geom,
                        **cheb)
+        t_c = time.perf_counter()
         Rm, Cm = win.moments_qr(V, a.refine_tol)
+        t_m = time.perf_counter()
         st = dict(win.stats)
         win.close()
+        t_e = time.perf_counter()
+        st["host_phases_s"] = {"create": round(t_c - t_start, 4), "moments": round(t_m - t_c, 4),
+                               "close": round(t_e - t_m, 4),
+                               "since_prev": round(t_start - (t_prev if s else t_begin), 4)}
+        t_prev = t_e
         qu.put((s, Rm, Cm, t_start, time.perf_counter() - t_start, st))
     qu.put(None)
     hth.join()
@@ -335,14 +342,14 @@ def main():
     ref = None
     fp64_s = None
     if not a.no_fp64:
-        Vd = q.probe_block(np.random.default_rng(1), a.N, r_used)
+        Vd = q.probe_block(np.random.default_rng(1), a.N, r_used, pinned=True)
         res_d, span_d = pipelined(q, ctx, solver, pts, geom, a, cheb, "fp64", Vd, a.fp64_steps)
         ref = res_d[0][0]
         fp64_s = span_d / a.fp64_steps
         print(f"# fp64 path: {fp64_s:.3f}s per window (pipelined, {a.fp64_steps} windows), "
               f"latency {res_d[0][3] - res_d[0][2]:.3f}s rk={res_d[0][1]} stats={res_d[0][6]}",
               file=sys.stderr, flush=True)
-    V = q.probe_block(np.random.default_rng(1), a.N, r_used)
+    V = q.probe_block(np.random.default_rng(1), a.N, r_used, pinned=True)
     ctx.set_profiling(True)
     res, t_span = pipelined(q, ctx, solver, pts, geom, a, cheb, "mixed", V, a.steps)
     lat, counts, worst = [], [], 0.0

@@ -80,6 +80,12 @@ int qb_ctx_set_graphs(qb_ctx* ctx, int32_t on);
  * serving loop sizes its admission budget from it.  No reference
  * counterpart. */
 int qb_ctx_mem_info(qb_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+/* Page-locked host memory (hipHostMalloc) for probe blocks and results: the
+ * library's host<->device copies from and to it run as direct DMA instead of
+ * being staged through a bounce buffer.  Free with qb_host_free.  No reference
+ * counterpart. */
+int qb_host_alloc(qb_ctx* ctx, int64_t bytes, void** out);
+void qb_host_free(void* p);
 /* Set the cache limit in bytes (0 releases the cache and disables it). */
 int qb_ctx_set_cache_limit(qb_ctx* ctx, int64_t bytes);
 

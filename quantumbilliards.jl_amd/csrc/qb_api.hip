@@ -197,6 +197,24 @@ int qb_ctx_mem_info(qb_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
   return QB_OK;
 }
 
+int qb_host_alloc(qb_ctx* ctx, int64_t bytes, void** out) {
+  if (!ctx) return QB_E_ARG;
+  if (!out || bytes < 0) return qb_fail(ctx, QB_E_ARG, "null output or negative size");
+  *out = nullptr;
+  if (bytes == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  hipError_t e = hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    return qb_fail(ctx, QB_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+  }
+  return QB_OK;
+}
+
+void qb_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int qb_ctx_set_cache_limit(qb_ctx* ctx, int64_t bytes) {
   if (!ctx || bytes < 0) return QB_E_ARG;
   (void)hipSetDevice(ctx->device);

@@ -6,7 +6,7 @@ reference's Julia API does (see qbhip.solvers and qbhip.beyn).
 """
 from ._lib import (QB_CFIE_KRESS, QB_DLP_KRESS, QB_FP64, QB_MIXED, QBError, LIB_PATH,
                    EXPORTS, lib)
-from .context import Context, default_context
+from .context import Context, default_context, pinned_array
 from .geometry import (Billiard, BoundaryPointsCFIE, CircularHoleBilliard, evaluate_points,
                        make_circle, make_full_ellipse, make_full_stadium,
                        make_star_polar_boundary, n_kress)

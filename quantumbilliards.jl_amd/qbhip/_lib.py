@@ -25,7 +25,7 @@ EXPORTS = (
     "qb_bessel_eval", "qb_cheb_tune", "qb_ctx_set_profiling", "qb_ctx_kernel_stats",
     "qb_ctx_autotune", "qb_ctx_set_cache_limit", "qb_residuals", "qb_assemble_deriv",
     "qb_ctx_mem_info", "qb_ctx_set_graphs", "qb_window_moments_qr", "qb_window_basis",
-    "qb_geom_set_orbits", "qb_geom_dim",
+    "qb_geom_set_orbits", "qb_geom_dim", "qb_host_alloc", "qb_host_free",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
@@ -83,6 +83,8 @@ def lib():
         "qb_geom_destroy": (None, [P]),
         "qb_geom_size": (I64, [P]),
         "qb_geom_dim": (I64, [P]),
+        "qb_host_alloc": (C.c_int, [P, I64, PP]),
+        "qb_host_free": (None, [P]),
         "qb_geom_set_orbits": (C.c_int, [P, P, I64, I32, P, P]),
         "qb_assemble": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P]),
         "qb_assemble_deriv": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, P, P, I32,

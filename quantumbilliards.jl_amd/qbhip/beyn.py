@@ -37,11 +37,17 @@ from .solvers import Geometry, boundary_matrix_size, boundary_matrix_size_for_so
 _PREC = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED}
 
 
-def probe_block(rng: np.random.Generator, N: int, r: int) -> np.ndarray:
+def probe_block(rng: np.random.Generator, N: int, r: int, pinned: bool = False) -> np.ndarray:
     """randn(rng, ComplexF64, N, r) (beyn.jl:193): real and imaginary parts
-    N(0, 1/2), column-major."""
+    N(0, 1/2), column-major; in page-locked host memory when pinned (its
+    upload to the device is then a direct DMA)."""
     V = (rng.standard_normal((N, r)) + 1j * rng.standard_normal((N, r))) * np.sqrt(0.5)
-    return np.asfortranarray(V)
+    if not pinned:
+        return np.asfortranarray(V)
+    from .context import pinned_array
+    P = pinned_array((N, r), np.complex128, "F")
+    P[...] = V
+    return P
 
 
 class Window:

@@ -3,6 +3,9 @@ from __future__ import annotations
 
 import ctypes as C
 import threading
+import weakref
+
+import numpy as np
 
 from . import _lib
 
@@ -74,3 +77,17 @@ def default_context(device: int = 0) -> Context:
             c = Context(device)
             _default[device] = c
         return c
+
+
+def pinned_array(shape, dtype=np.complex128, order: str = "F", ctx: "Context" = None) -> np.ndarray:
+    """An uninitialised NumPy array in page-locked host memory (qb_host_alloc):
+    the library moves it to and from the device by DMA.  The memory is
+    released when the last view of the array is gone."""
+    ctx = ctx or default_context()
+    dtype = np.dtype(dtype)
+    n = int(np.prod(shape)) * dtype.itemsize
+    p = C.c_void_p()
+    _lib.check(ctx.ptr, _lib.lib().qb_host_alloc(ctx.ptr, max(n, 1), C.byref(p)))
+    buf = (C.c_char * max(n, 1)).from_address(p.value)
+    weakref.finalize(buf, _lib.lib().qb_host_free, C.c_void_p(p.value))
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape, order=order)
