@@ -1476,11 +1476,11 @@ extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, in
   const int n = (int)N;
   const int lu_ty = precision == QB_MIXED ? 0 : 1;
   // LU: trailing updates (K = 256), block-row TRSM updates, panel updates
-  shapes.push_back({n - 256, n - 256, 256, lu_ty});
-  shapes.push_back({n / 2, n / 2, 256, lu_ty});
-  shapes.push_back({n / 4, n / 4, 256, lu_ty});
-  shapes.push_back({128, n / 2, 128, lu_ty});
-  for (int w = 128; w >= 32; w /= 2) shapes.push_back({n / 2, w, w, lu_ty});
+  // over the whole factorisation (the remaining size shrinks from n to 256)
+  for (int d : {1, 2, 4, 8, 16}) shapes.push_back({n / d - (d == 1 ? 256 : 0), n / d - (d == 1 ? 256 : 0), 256, lu_ty});
+  for (int d : {2, 8}) shapes.push_back({128, n / d, 128, lu_ty});
+  for (int d : {2, 8, 32})
+    for (int w = 128; w >= 32; w /= 2) shapes.push_back({n / d, w, w, lu_ty});
   // triangular solves with r right-hand sides (recursive halves)
   for (int h = n / 2; h >= 32; h /= 2) shapes.push_back({h, r, h, lu_ty});
   // FP64 refinement residual

@@ -81,3 +81,22 @@ def test_empty_batches_are_no_ops():
     assert len(idx) == 0 and Phi.shape == (64, 0)
     ks, tens, us, pk, tn = Q.compute_spectrum_beyn(s, b, 10.0, 10.0, m=5)
     assert len(ks) == 0 and len(us) == 0
+
+
+def test_pinned_probe_block_gives_the_same_moments():
+    """qb_host_alloc memory (qbhip.pinned_array) as the probe block: the
+    moments equal those from ordinary host memory bit for bit."""
+    Q = q()
+    b, s, pts = _circle(96)
+    rng = np.random.default_rng(4)
+    V = Q.probe_block(rng, 96, 8)
+    P = Q.pinned_array((96, 8))
+    P[...] = V
+    assert P.flags.f_contiguous
+    w = Q.Window(s, pts, 10.0 + 0j, 0.3, 8, **CP)
+    A0, A1 = w.moments(V)
+    B0, B1 = w.moments(P)
+    w.close()
+    assert np.array_equal(A0, B0) and np.array_equal(A1, B1)
+    Vp = Q.probe_block(np.random.default_rng(4), 96, 8, pinned=True)
+    assert np.array_equal(Vp, V)
