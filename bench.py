@@ -59,7 +59,9 @@ def parse():
                     help="the same disk on the C_5-reduced star: one window per irrep "
                          "(Rotation(5, m), m = 0..4), eigenvalues checked against the "
                          "unreduced FP64 window")
-    ap.add_argument("--sym-precision", default="mixed")
+    ap.add_argument("--sym-precision", default="fp64")
+    ap.add_argument("--sym-slots", type=int, default=2,
+                    help="contexts that take irrep windows concurrently")
     return ap.parse_args()
 
 
@@ -95,59 +97,88 @@ def run_symmetry(a):
     """Symmetry-reduced windows (_all_k_symm_CFIE_chebyshev!, cheb-CFIE:698-824,
     as the reference's examples run the star, example_beyn_kress.jl:51): the
     disk |k - k0| <= R of star(1, 0.2, 5) with N nodes (a multiple of 10) as
-    five reduced windows of N/5 unknowns, one per C_5 irrep.  Prints one JSON
-    line: eigenvalues/s over all irreps, per-irrep window times, and the match
+    five reduced windows of N/5 unknowns, one per C_5 irrep.  The irreps are
+    independent, so `--sym-slots` contexts (one HIP stream pair each, one
+    host thread each) take irrep windows from a queue; each window reduces its
+    moments on the device (qb_window_moments_qr) and its thread then forms B
+    and eig(B) while the other context keeps the device busy.  Prints one JSON
+    line: eigenvalues/s over all irreps, per-window latencies, and the match
     of the union of the irrep spectra against the unreduced FP64 window."""
+    import queue
+    import threading
     import qbhip as q
-    ctx = q.default_context()
     b = q.make_star_polar_boundary(1.0, 0.2, 5)
     N = a.N + (-a.N) % 10
     full = q.CFIE_kress(10.0, b)
     pts = q.evaluate_points(full, b, a.k0, N=[N])
-    gfull = q.Geometry(pts, ctx)
+    ctxs = [q.default_context()] + [q.Context(0) for _ in range(max(1, a.sym_slots) - 1)]
+    gfull = q.Geometry(pts, ctxs[0])
     zj = a.k0 + a.R * np.exp(2j * np.pi * np.arange(a.nq) / a.nq)
     nh, mh, nj, mj, errs = q.chebyshev_params(full, pts, zj, tol=a.cheb_tol, geom=gfull)
     cheb = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
     solvers = [q.CFIE_kress(10.0, b, symmetry=q.Rotation(5, m)) for m in range(5)]
-    geoms = [q.Geometry(pts, ctx, s) for s in solvers]
-    m_dim = geoms[0].dim
+    geoms = [[q.Geometry(pts, c, s) for s in solvers] for c in ctxs]
+    m_dim = geoms[0][0].dim
     bf = q.make_star_polar_boundary(1.0, 0.2, 5)
     bf.area = b.area / 5  # probes from the Weyl count of one sector
     bf.length = b.length / 5
     r_irrep = a.r or q.probe_count(bf, a.k0, a.R, a.nq, a.svd_tol, 32)
+    for c in ctxs:
+        c.autotune(m_dim, a.nq, r_irrep, a.sym_precision)
+    V = q.probe_block(np.random.default_rng(1), m_dim, r_irrep, pinned=True)
 
-    def one_pass(rs, seed):
-        lams, infos, t = [], [], []
+    def one_pass():
+        """All five irrep windows; returns per-irrep (lam, rk, latency) and the
+        span from the first window's start to the last result."""
+        work = queue.Queue()
         for m in range(5):
-            lam, dt, info = window(q, solvers[m], pts, geoms[m], a, cheb, a.sym_precision, rs[m],
-                                   seed, a.r == 0)
-            lams.append(lam)
-            infos.append(info)
-            t.append(dt)
-        return lams, infos, t
+            work.put(m)
+        res = [None] * 5
+        t0 = time.perf_counter()
 
-    rs = [r_irrep] * 5
+        def worker(ci):
+            c = ctxs[ci]
+            while True:
+                try:
+                    m = work.get_nowait()
+                except queue.Empty:
+                    return
+                ts = time.perf_counter()
+                win = q.Window(solvers[m], pts, complex(a.k0), a.R, a.nq,
+                               precision=a.sym_precision, geom=geoms[ci][m], ctx=c, **cheb)
+                Rm, Cm = win.moments_qr(V, a.refine_tol)
+                win.close()
+                B, Ur, S, rk = q.B_from_qr(Rm, Cm, a.svd_tol)
+                lam = np.linalg.eigvals(B) if B.size else np.zeros(0, complex)
+                res[m] = (lam, rk, time.perf_counter() - ts)
+
+        th = [threading.Thread(target=worker, args=(i,)) for i in range(len(ctxs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return res, time.perf_counter() - t0
+
     for w in range(max(1, a.warmup)):
-        lams, infos, t = one_pass(rs, 1)
-        rs = [i["r"] for i in infos]
-        print(f"# warmup {w}: {sum(t):.2f}s r={rs} rk={[i['rk'] for i in infos]}",
-              file=sys.stderr, flush=True)
-    ctx.set_profiling(True)
-    t0 = time.perf_counter()
-    passes = []
+        res, sp = one_pass()
+        print(f"# warmup {w}: {sp:.3f}s rk={[x[1] for x in res]}", file=sys.stderr, flush=True)
+    for c in ctxs:
+        c.set_profiling(True)
+    spans, last = [], None
     for s in range(a.steps):
-        passes.append(one_pass(rs, 1))
-    span = time.perf_counter() - t0
-    ks = ctx.kernel_stats()
-    ctx.set_profiling(False)
+        last, sp = one_pass()
+        spans.append(sp)
+    ks = ctxs[0].kernel_stats()
+    for c in ctxs:
+        c.set_profiling(False)
     # the unreduced FP64 window of the same disk as the reference spectrum
     lam_ref, t_full, _ = window(q, full, pts, gfull, a, cheb, "fp64",
                                 a.r or q.probe_count(b, a.k0, a.R, a.nq, a.svd_tol, a.r0), 1,
                                 a.r == 0)
-    lams = np.concatenate(passes[-1][0])
+    lams = np.concatenate([x[0] for x in last])
     n, worst, nin = match_count(lams, lam_ref, a.k0, a.R)
     nref = int(np.count_nonzero(np.abs(lam_ref - a.k0) <= a.R))
-    per_pass = span / a.steps
+    per_pass = float(np.sum(spans)) / a.steps
     out = {
         "metric": "eigenvalues/sec, C5-reduced Beyn CFIE star N=%d (5 irrep windows of %d)"
                   % (N, m_dim),
@@ -155,16 +186,21 @@ def run_symmetry(a):
         "warmup": a.warmup, "ms_per_step": per_pass * 1e3, "higher_is_better": True,
         "dtype": "c64+c128-refine" if a.sym_precision == "mixed" else "c128",
         "config": {"workload": "beyn-cfie-kress star(1,0.2,5) Rotation(5,m) m=0..4 k0=%g R=%g"
-                               % (a.k0, a.R), "N": N, "m": m_dim, "nq": a.nq, "r": rs,
-                   "cheb": cheb, "precision": a.sym_precision},
-        "irrep_window_s": [round(x, 4) for x in passes[-1][2]],
+                               % (a.k0, a.R), "N": N, "m": m_dim, "nq": a.nq, "r": r_irrep,
+                   "cheb": cheb, "precision": a.sym_precision, "slots": len(ctxs)},
+        "irrep_window_latency_s": [round(x[2], 4) for x in last],
         "eigenvalues_matched": n, "eigenvalues_in_disk_reduced": nin,
         "eigenvalues_in_disk_full_fp64": nref, "max_rel_eig_err_vs_full_fp64": worst,
         "full_fp64_window_s": t_full,
-        "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]} for k, v in ks.items()},
+        "kernels_first_context": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]}
+                                  for k, v in ks.items()},
     }
-    for g in geoms + [gfull]:
-        g.close()
+    for gl in geoms:
+        for g in gl:
+            g.close()
+    gfull.close()
+    for c in ctxs[1:]:
+        c.close()
     print(json.dumps(out), flush=True)
 
 
