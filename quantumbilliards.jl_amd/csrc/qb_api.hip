@@ -565,6 +565,13 @@ extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
       if (pick < 0) found_all = false;
       else sym.inv[l] = (int8_t)pick;
     }
+    // the mirror entry sums over l' = inv[l] for every l: inv must be a
+    // permutation of the images
+    if (found_all) {
+      std::vector<char> hit(G, 0);
+      for (int l = 0; l < G; ++l) hit[sym.inv[l]] = 1;
+      for (int l = 0; l < G; ++l) found_all = found_all && hit[l];
+    }
     int ok = 0;
     if (found_all) {
       rc = qb_orbit_pairs_ok(ctx, g->d, sym, floor_abs, &ok);
@@ -1114,10 +1121,10 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       goto fail;
     }
     QB_TRY(qb_bcast(ctx, Vd, Rr, nv, nv, nq));
-    // host copies of ||T_j||_inf and ||V||_max for the backward error
+    // host copies of ||T_j||_inf and ||V||_max for the backward error;
+    // ||V||_max is formed on the host while the first residual pass runs
     std::vector<double> tn(nq);
-    double vmax = 0.0;
-    for (int64_t i = 0; i < nv; ++i) vmax = std::max(vmax, hypot(V[2 * i], V[2 * i + 1]));
+    double vmax = -1.0;
     if (hipMemcpyAsync(tn.data(), w->tnorm, nq * sizeof(double), hipMemcpyDeviceToHost,
                        ctx->stream) != hipSuccess) {
       rc = qb_fail(ctx, QB_E_HIP, "norm download failed");
@@ -1162,8 +1169,14 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       ctx->bix = nullptr;
       if (e) return e;
       if (hipMemcpyAsync(h3.data(), w->scratch_d, 3 * nq * sizeof(double),
-                         hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-          hipStreamSynchronize(ctx->stream) != hipSuccess)
+                         hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return qb_fail(ctx, QB_E_HIP, "refinement norms failed");
+      if (vmax < 0.0) {
+        double m2 = 0.0;
+        for (int64_t i = 0; i < nv; ++i) m2 = std::max(m2, V[2 * i] * V[2 * i] + V[2 * i + 1] * V[2 * i + 1]);
+        vmax = sqrt(m2);
+      }
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess)
         return qb_fail(ctx, QB_E_HIP, "refinement norms failed");
       for (int j : list) bn[j] = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
       return QB_OK;

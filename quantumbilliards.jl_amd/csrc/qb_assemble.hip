@@ -728,7 +728,7 @@ __global__ void __launch_bounds__(256) k_assemble_red_pair(DevGeom g, DevSym s,
         const int ls = s.inv[l];
         const int64_t jm = s.img[a * G + ls];
         const PairGeo p1 = pair_geo(g, i, j, xi, yi, ci);
-        PairGeo p2 = pair_geo(g, ib, jm, xb, yb, cb);
+        const PairGeo p2 = pair_geo(g, ib, jm, xb, yb, cb);  // its r equals p1.r
         const Spec sp = specials<OP, MH, MJ>(tv, p1.r, k,
                                              p1.same || p2.same || OP == QB_DLP_KRESS);
         const cd v1 = offdiag_entry<OP>(g, i, j, p1.dx, p1.dy, p1.invr, p1.same, p1.Rij, p1.lt,
