@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second")
     ap.add_argument("--serve-kmax", type=float, default=1000.0)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-symmetry", action="store_true",
+                    help="serve each request as the symmetry-reduced windows of its shape")
     ap.add_argument("--symmetry", action="store_true",
                     help="the same disk on the C_5-reduced star: one window per irrep "
                          "(Rotation(5, m), m = 0..4), eigenvalues checked against the "
@@ -73,9 +75,11 @@ def run_serve(a):
     HBM footprint into `slots` device contexts.  One untimed warmup window
     first.  Prints one JSON line with eigenvalues/s and p50/p99 latency."""
     from qbhip import serve as S
-    reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax)
+    reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax,
+                            symmetry=a.serve_symmetry)
     srv = S.Server(slots=a.serve_slots)
-    srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32)).result()
+    srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32,
+                               symmetry=a.serve_symmetry)).result()
     t0 = time.perf_counter()
     res = srv.run_stream(reqs)
     span = time.perf_counter() - t0
@@ -90,7 +94,8 @@ def run_serve(a):
         "higher_is_better": True, "dtype": "c64+c128-refine", "data": "synthetic (seeded)",
         "config": {"workload": "serving stream ellipse/stadium/star", "requests": len(reqs),
                    "rate_per_s": a.serve_rate, "k0": [50.0, a.serve_kmax], "nq": 32,
-                   "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns)}}), flush=True)
+                   "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns),
+                   "symmetry_reduced": a.serve_symmetry}}), flush=True)
 
 
 def run_symmetry(a):

@@ -15,6 +15,13 @@ at start-up less 12 GiB.  The host part of
 each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
 windows' device work.  Latency is measured from arrival to result.
 
+A request with symmetry=True is served as the independent symmetry-reduced
+windows of its shape's symmetry group (both reflections of the ellipse,
+stadium and disk, the C_5 rotations of the star; _all_k_symm_CFIE_chebyshev!,
+cheb-CFIE:698-824): every irrep is a job of its own, admitted by its own
+(G^2 times smaller) footprint, and the request completes with the union of
+the irrep spectra when its last irrep does.
+
 No counterpart in the reference: its driver (compute_spectrum_beyn,
 beyn.jl:778-921) loops over disks serially on the CPU.  The per-window
 computation is the same construct_B_matrix + eigen!(B) as qbhip.beyn.
@@ -30,10 +37,13 @@ from typing import Dict, List, Optional
 
 import numpy as np
 
+import dataclasses
+
 from . import geometry as G
 from .beyn import B_from_qr, Window, footprint, probe_block, probe_count
 from .context import Context
 from .solvers import CFIE_kress, Geometry, chebyshev_params
+from .symmetry import Rotation, XYReflection, YReflection, node_divisor
 
 SHAPES = {
     "circle": lambda: G.make_circle(1.0),
@@ -41,6 +51,16 @@ SHAPES = {
     "stadium": lambda: G.make_full_stadium(1.0, 1.0),
     "star": lambda: G.make_star_polar_boundary(1.0, 0.2, 5),
     "circle_hole": lambda: G.CircularHoleBilliard(1.0, holes=[(0.3, 0.35, 0.0)]),
+}
+
+
+# the irreps a symmetric request is split into, per shape
+SYM_GROUPS = {
+    "circle": lambda: [XYReflection(px, py) for px in (1, -1) for py in (1, -1)],
+    "ellipse": lambda: [XYReflection(px, py) for px in (1, -1) for py in (1, -1)],
+    "stadium": lambda: [XYReflection(px, py) for px in (1, -1) for py in (1, -1)],
+    "star": lambda: [Rotation(5, m) for m in range(5)],
+    "circle_hole": lambda: [YReflection(p) for p in (1, -1)],
 }
 
 
@@ -58,6 +78,7 @@ class WindowRequest:
     seed: int = 0
     arrival: float = 0.0    # seconds after the stream starts
     N: Optional[List[int]] = None  # explicit node counts per component
+    symmetry: bool = False  # serve as the symmetry-reduced windows of SYM_GROUPS[shape]
 
 
 @dataclass
@@ -74,6 +95,7 @@ class WindowResult:
     device_s: float
     host_s: float
     stats: dict = field(default_factory=dict)
+    irreps: int = 1             # reduced windows merged into this result
 
     @property
     def latency(self) -> float:
@@ -85,7 +107,7 @@ def node_counts(billiard: G.Billiard, k: float, ppw: float, mult: int = 2) -> Li
     for comp in billiard.full_boundary:
         L = sum(c.length for c in comp)
         n = int(math.ceil(ppw * k * L / (2 * math.pi)))
-        out.append(max(16, n + (-n) % mult))
+        out.append(max(16 + (-16) % mult, n + (-n) % mult))
     return out
 
 
@@ -117,10 +139,32 @@ class Server:
     # -- public --------------------------------------------------------------
     def submit(self, req: WindowRequest, t_arrival: Optional[float] = None) -> Future:
         fut: Future = Future()
+        ta = time.perf_counter() if t_arrival is None else t_arrival
+        jobs = [(req, None)] if not req.symmetry else [(req, g) for g in SYM_GROUPS[req.shape]()]
+        if req.symmetry:
+            parts: List = [None] * len(jobs)
+            left = [len(jobs)]
+            guard = threading.Lock()
+
+            def collect(i, f):
+                with guard:
+                    if fut.done():
+                        return
+                    if f.exception() is not None:
+                        fut.set_exception(f.exception())
+                        return
+                    parts[i] = f.result()
+                    left[0] -= 1
+                    if left[0] == 0:
+                        fut.set_result(_merge(parts))
         with self.lock:
             if self.closed:
                 raise RuntimeError("server closed")
-            self.pending.append((req, fut, time.perf_counter() if t_arrival is None else t_arrival))
+            for i, (r, g) in enumerate(jobs):
+                f = fut if g is None else Future()
+                if g is not None:
+                    f.add_done_callback(lambda ff, i=i: collect(i, ff))
+                self.pending.append(((r, g), f, ta))
             self.lock.notify_all()
         return fut
 
@@ -157,15 +201,21 @@ class Server:
             c.close()
 
     # -- internals -----------------------------------------------------------
-    def _prepare(self, req: WindowRequest):
+    def _prepare(self, job):
+        req, sym = job
         b = SHAPES[req.shape]()
-        Ns = tuple(req.N) if req.N is not None else tuple(node_counts(b, req.k0 + req.R, req.ppw))
+        mult = node_divisor(sym)
+        Ns = (tuple(req.N) if req.N is not None
+              else tuple(node_counts(b, req.k0 + req.R, req.ppw, mult)))
         key = (req.shape, Ns)
         if key not in self.pts_cache:
             self.pts_cache[key] = G.evaluate_points(CFIE_kress(req.ppw, b), b, req.k0, N=list(Ns))
-        r = probe_count(b, req.k0, req.R, req.nq, req.svd_tol)
-        N = int(sum(Ns))
-        r = min(r, N)
+        order = 1 if sym is None else sym.order
+        # probes from the Weyl count of one symmetry sector
+        bs = b if order == 1 else dataclasses.replace(b, area=b.area / order,
+                                                      length=b.length / order)
+        N = int(sum(Ns)) // order
+        r = min(probe_count(bs, req.k0, req.R, req.nq, req.svd_tol), N)
         prec = req.precision
         fp = footprint(N, req.nq, prec, r)
         if fp > self.budget and prec == "mixed":
@@ -182,13 +232,13 @@ class Server:
                         return
                     pick = None
                     if self.free_slots:
-                        for idx, (req, fut, ta) in enumerate(self.pending):
-                            prep = self._prepare(req)
+                        for idx, (job, fut, ta) in enumerate(self.pending):
+                            prep = self._prepare(job)
                             fp = prep[4]
                             if fp > self.budget:
                                 self.pending.pop(idx)
                                 fut.set_exception(MemoryError(
-                                    f"request {req.rid}: footprint {fp} B exceeds the budget {self.budget} B"))
+                                    f"request {job[0].rid}: footprint {fp} B exceeds the budget {self.budget} B"))
                                 pick = "rejected"
                                 break
                             if self.live_bytes + fp <= self.budget:
@@ -207,22 +257,23 @@ class Server:
             self.workers.append(t)
             t.start()
 
-    def _run(self, slot, req, fut, t_arrival, prep):
+    def _run(self, slot, job, fut, t_arrival, prep):
+        req, sym = job
         b, Ns, N, r, fp, prec = prep
         t_admit = time.perf_counter()
         try:
             ctx = self.slots[slot]
             pts = self.pts_cache[(req.shape, Ns)]
-            gkey = (slot, req.shape, Ns)
+            solver = CFIE_kress(req.ppw, b, symmetry=sym)
+            gkey = (slot, req.shape, Ns, sym)
             if gkey not in self.geoms:
-                self.geoms[gkey] = Geometry(pts, ctx)
+                self.geoms[gkey] = Geometry(pts, ctx, solver)
             geom = self.geoms[gkey]
-            solver = CFIE_kress(req.ppw, b)
             zj = req.k0 + req.R * np.exp(2j * np.pi * np.arange(req.nq) / req.nq)
             ck = (req.shape, Ns, round(req.k0, 3), round(req.R, 3), req.nq, req.tol)
             if ck not in self.cheb_cache:
-                nh, mh, nj, mj, _ = chebyshev_params(solver, pts, zj, tol=max(req.tol, 1e-13),
-                                                     geom=geom)
+                nh, mh, nj, mj, _ = chebyshev_params(CFIE_kress(req.ppw, b), pts, zj,
+                                                     tol=max(req.tol, 1e-13), geom=geom)
                 self.cheb_cache[ck] = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
             cheb = self.cheb_cache[ck]
             win = Window(solver, pts, req.k0, req.R, req.nq, precision=prec, geom=geom, **cheb)
@@ -244,6 +295,7 @@ class Server:
             lam = np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128)
             t_done = time.perf_counter()
             inside = np.sort_complex(lam[np.abs(lam - req.k0) <= req.R])
+            st["symmetry"] = None if sym is None else repr(sym)
             return WindowResult(req.rid, lam, inside, rk, r, N, t_arrival, t_admit, t_done, t_dev,
                                 t_done - th, st)
 
@@ -257,6 +309,19 @@ class Server:
             self.free_slots.append(slot)
             self.free_slots.sort()
             self.lock.notify_all()
+
+
+def _merge(parts: List[WindowResult]) -> WindowResult:
+    """One result from the reduced windows of a symmetric request: the union of
+    their spectra, completed when the last one completed."""
+    p0 = parts[0]
+    lam = np.concatenate([p.lam for p in parts])
+    inside = np.sort_complex(np.concatenate([p.inside for p in parts]))
+    return WindowResult(p0.rid, lam, inside, sum(p.rk for p in parts), sum(p.r for p in parts),
+                        sum(p.N for p in parts), p0.t_arrival, min(p.t_admit for p in parts),
+                        max(p.t_done for p in parts), sum(p.device_s for p in parts),
+                        sum(p.host_s for p in parts), {"irreps": [p.stats for p in parts]},
+                        irreps=len(parts))
 
 
 def summarize(results: List, t_span: Optional[float] = None) -> dict:
@@ -277,7 +342,7 @@ def summarize(results: List, t_span: Optional[float] = None) -> dict:
 
 def poisson_stream(n: int, rate: float, seed: int = 0, shapes=("ellipse", "stadium", "star"),
                    kmin: float = 50.0, kmax: float = 1000.0, nq: int = 32,
-                   precision: str = "mixed") -> List[WindowRequest]:
+                   precision: str = "mixed", symmetry: bool = False) -> List[WindowRequest]:
     """Seeded request stream of the serving workload: Poisson arrivals, shapes
     drawn from `shapes`, k0 log-uniform in [kmin, kmax], R uniform in
     [0.25, 0.5], tol 1e-8 or 1e-12."""
@@ -290,5 +355,6 @@ def poisson_stream(n: int, rate: float, seed: int = 0, shapes=("ellipse", "stadi
                                  k0=float(np.exp(rng.uniform(np.log(kmin), np.log(kmax)))),
                                  R=float(rng.uniform(0.25, 0.5)),
                                  tol=float(rng.choice([1e-8, 1e-12])), nq=nq,
-                                 precision=precision, seed=1000 + i, arrival=t))
+                                 precision=precision, seed=1000 + i, arrival=t,
+                                 symmetry=symmetry))
     return out

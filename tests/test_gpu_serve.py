@@ -50,7 +50,7 @@ def test_admission_by_footprint():
     from qbhip.serve import Server, WindowRequest
     reqs = _reqs()[:3]
     srv0 = Server(slots=1)
-    fps = [srv0._prepare(q)[4] for q in reqs]
+    fps = [srv0._prepare((q, None))[4] for q in reqs]
     srv0.close()
     # budget for the largest one only: windows must run one at a time
     srv = Server(slots=3, hbm_budget=max(fps))
@@ -75,7 +75,7 @@ def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
     from qbhip.serve import Server, WindowRequest, summarize
     req = _reqs()[0]
     srv0 = Server(slots=1)
-    b, Ns, N, r, fp_mixed, prec = srv0._prepare(req)
+    b, Ns, N, r, fp_mixed, prec = srv0._prepare((req, None))
     ref = srv0.submit(req).result()
     srv0.close()
     assert prec == "mixed"
@@ -92,3 +92,36 @@ def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
     assert isinstance(out[1], MemoryError)
     s = summarize(out)
     assert s["windows"] == 1 and s["rejected"] == 1
+
+
+def test_symmetric_request_equals_the_unreduced_one():
+    """A request served as the reduced windows of its shape's symmetry group
+    (both reflections of the ellipse and stadium, the C_5 rotations of the
+    star) returns the eigenvalues the unreduced window finds in the disk; on
+    the disk they are the zeros of J_m."""
+    from qbhip.serve import Server, WindowRequest
+    reqs = [WindowRequest(0, "ellipse", 24.0, 0.4, tol=1e-12, nq=32, ppw=12, seed=21),
+            WindowRequest(1, "star", 30.0, 0.3, tol=1e-12, nq=32, ppw=12, seed=22),
+            WindowRequest(2, "stadium", 15.0, 0.3, tol=1e-12, nq=32, ppw=12, seed=23),
+            WindowRequest(3, "circle", 20.0, 0.5, tol=1e-12, nq=32, ppw=13, seed=24)]
+    srv = Server(slots=2)
+    full, red = {}, {}
+    for q in reqs:
+        # the same node counts (a multiple of 4 / 10) for both
+        sym = WindowRequest(**{**q.__dict__, "symmetry": True})
+        f = srv.submit(sym)
+        rs = f.result()
+        Ns = srv._prepare((sym, __import__("qbhip").serve.SYM_GROUPS[q.shape]()[0]))[1]
+        full[q.rid] = srv.submit(WindowRequest(**{**q.__dict__, "N": list(Ns)})).result()
+        red[q.rid] = rs
+    srv.close()
+    for rid in full:
+        a, m = full[rid].inside, red[rid].inside
+        print(rid, len(a), len(m), red[rid].irreps)
+        assert len(a) == len(m) and len(a) > 0, rid
+        a, m = np.sort_complex(a), np.sort_complex(m)
+        assert np.max(np.abs(a - m) / np.abs(a)) <= 1e-10, rid
+        assert red[rid].irreps == (5 if reqs[rid].shape == "star" else 4)
+    ex = O.jm_zeros_in_disk(20.0, 0.5)
+    got = np.sort(red[3].inside.real)
+    assert len(got) == len(ex) and np.max(np.abs(got - ex) / ex) < 1e-10
