@@ -63,7 +63,9 @@ def parse():
                          "unreduced FP64 window")
     ap.add_argument("--sym-precision", default="fp64")
     ap.add_argument("--sym-slots", type=int, default=2,
-                    help="contexts that take irrep windows concurrently")
+                    help="contexts that take irrep windows concurrently (--no-sym-batched)")
+    ap.add_argument("--no-sym-batched", dest="sym_batched", action="store_false",
+                    help="one window per irrep instead of one window over all irreps")
     return ap.parse_args()
 
 
@@ -128,11 +130,35 @@ def run_symmetry(a):
     bf.area = b.area / 5  # probes from the Weyl count of one sector
     bf.length = b.length / 5
     r_irrep = a.r or q.probe_count(bf, a.k0, a.R, a.nq, a.svd_tol, 32)
-    for c in ctxs:
-        c.autotune(m_dim, a.nq, r_irrep, a.sym_precision)
     V = q.probe_block(np.random.default_rng(1), m_dim, r_irrep, pinned=True)
+    if a.sym_batched:
+        # one window over all five irreps: a single assembly evaluates every
+        # image term once for all characters, one LU factors 5 x nq matrices
+        ctxs = ctxs[:1]
+        gmulti = q.Geometry(pts, ctxs[0], irreps=solvers)
+        ctxs[0].autotune(m_dim, a.nq * 5, r_irrep, a.sym_precision)
+    else:
+        for c in ctxs:
+            c.autotune(m_dim, a.nq, r_irrep, a.sym_precision)
+
+    def batched_pass():
+        t0 = time.perf_counter()
+        ts = time.perf_counter()
+        win = q.Window(solvers[0], pts, complex(a.k0), a.R, a.nq, precision=a.sym_precision,
+                       geom=gmulti, ctx=ctxs[0], **cheb)
+        Rm, Cm = win.moments_qr(V, a.refine_tol)
+        win.close()
+        t_dev = time.perf_counter() - ts
+        res = []
+        for m in range(5):
+            B, Ur, S, rk = q.B_from_qr(Rm[m], Cm[m], a.svd_tol)
+            lam = np.linalg.eigvals(B) if B.size else np.zeros(0, complex)
+            res.append((lam, rk, t_dev))
+        return res, time.perf_counter() - t0
 
     def one_pass():
+        if a.sym_batched:
+            return batched_pass()
         """All five irrep windows; returns per-irrep (lam, rk, latency) and the
         span from the first window's start to the last result."""
         work = queue.Queue()
@@ -192,7 +218,8 @@ def run_symmetry(a):
         "dtype": "c64+c128-refine" if a.sym_precision == "mixed" else "c128",
         "config": {"workload": "beyn-cfie-kress star(1,0.2,5) Rotation(5,m) m=0..4 k0=%g R=%g"
                                % (a.k0, a.R), "N": N, "m": m_dim, "nq": a.nq, "r": r_irrep,
-                   "cheb": cheb, "precision": a.sym_precision, "slots": len(ctxs)},
+                   "cheb": cheb, "precision": a.sym_precision, "slots": len(ctxs),
+                   "irreps_batched": bool(a.sym_batched)},
         "irrep_window_latency_s": [round(x[2], 4) for x in last],
         "eigenvalues_matched": n, "eigenvalues_in_disk_reduced": nin,
         "eigenvalues_in_disk_full_fp64": nref, "max_rel_eig_err_vs_full_fp64": worst,
@@ -203,6 +230,8 @@ def run_symmetry(a):
     for gl in geoms:
         for g in gl:
             g.close()
+    if a.sym_batched:
+        gmulti.close()
     gfull.close()
     for c in ctxs[1:]:
         c.close()

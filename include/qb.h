@@ -149,6 +149,22 @@ int64_t qb_geom_size(const qb_geom* g);
  * with A the full operator (identity included).  m = 0 removes the reduction. */
 int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G, const int64_t* img,
                        const double* scale);
+/* The same orbits with nirr weight tables (scale: nirr x m x G complex, one
+ * table per irrep of the group, e.g. the n characters of a C_n rotation): the
+ * image terms of every entry are evaluated once and summed into all nirr
+ * reduced matrices.  qb_assemble then writes nirr * nz matrices, matrix
+ * e * nz + q for irrep e and wavenumber q.  A window on such a geometry holds
+ * all irreps of its contour nodes and factors and solves them as one batch;
+ * qb_window_moments then returns nirr blocks of A0 and of A1 (N x r each, one
+ * after the other), qb_window_moments_qr nirr blocks of R and of C (r x r
+ * each), and qb_window_basis_irrep forms U_k of one irrep; the admission
+ * footprint is qb_window_footprint(N, nq * nirr, ...).  No reference
+ * counterpart (the reference assembles one irrep per solver); derivative
+ * assembly and qb_residuals take one irrep.  1 <= nirr <= 16. */
+int qb_geom_set_orbits_multi(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G, int32_t nirr,
+                             const int64_t* img, const double* scales);
+/* Irreps assembled together on g (1 without a multi-irrep orbit table). */
+int32_t qb_geom_irreps(const qb_geom* g);
 /* Matrix dimension of assemblies on g: m after qb_geom_set_orbits, else N
  * (boundary_matrix_size_for_solver, ebim.jl:26-40). */
 int64_t qb_geom_dim(const qb_geom* g);
@@ -227,6 +243,9 @@ int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, double* R, do
 /* U_k = Q [Ur; 0] (host N x k) from the last qb_window_moments_qr, Ur host
  * r x k: the U_k that construct_B_matrix returns (beyn.jl:311). */
 int qb_window_basis(qb_window* w, const double* Ur, int32_t k, double* U);
+/* The same for irrep e of a window on a multi-irrep geometry
+ * (qb_geom_set_orbits_multi); qb_window_basis is irrep 0. */
+int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* Ur, int32_t k, double* U);
 
 /* Device evaluation of H0^(1), H1^(1), J0, J1 at n complex points (the values
  * the reference takes from SpecialFunctions.besselh/besselj when sampling its

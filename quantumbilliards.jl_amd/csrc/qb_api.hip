@@ -473,8 +473,15 @@ extern "C" int64_t qb_geom_dim(const qb_geom* g) { return g ? qb_dim(g) : 0; }
 // assembly relies on: the representative is image 0 with weight 1, and no
 // orbit meets a node twice (a node on a reflection axis,
 // _has_node_on_reflection_axis, symmetry.jl:326-345).
+extern "C" int32_t qb_geom_irreps(const qb_geom* g) { return g ? qb_nirr(g) : 0; }
+
 extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
                                   const int64_t* img, const double* scale) {
+  return qb_geom_set_orbits_multi(ctx, g, m, G, 1, img, scale);
+}
+
+extern "C" int qb_geom_set_orbits_multi(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
+                                        int32_t nirr, const int64_t* img, const double* scale) {
   if (!ctx) return QB_E_ARG;
   if (!g) return qb_fail(ctx, QB_E_ARG, "null geometry");
   if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
@@ -487,16 +494,22 @@ extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
     return QB_OK;
   }
   if (!img || !scale) return qb_fail(ctx, QB_E_ARG, "null orbit table");
+  if (nirr < 1 || nirr > QB_MAX_IRREPS)
+    return qb_fail(ctx, QB_E_ARG, "nirr must be in [1, 16]");
   const int64_t N = g->d.N;
   if (G < 2 || G > 64) return qb_fail(ctx, QB_E_ARG, "orbit size G must be in [2, 64]");
   if (m < 1 || m * (int64_t)G != N)
     return qb_fail(ctx, QB_E_ARG, "orbits must partition the nodes: m * G != N");
   std::vector<int32_t> im((size_t)m * G);
   std::vector<char> seen((size_t)N, 0);
+  for (int64_t e = 0; e < nirr; ++e)
+    for (int64_t b = 0; b < m; ++b) {
+      const double* sc = scale + 2 * (e * m + b) * G;
+      if (sc[0] != 1.0 || sc[1] != 0.0)
+        return qb_fail(ctx, QB_E_ARG, "orbit " + std::to_string(b) +
+                                          ": the representative (image 0) must have weight 1");
+    }
   for (int64_t b = 0; b < m; ++b) {
-    if (scale[2 * b * G] != 1.0 || scale[2 * b * G + 1] != 0.0)
-      return qb_fail(ctx, QB_E_ARG, "orbit " + std::to_string(b) +
-                                        ": the representative (image 0) must have weight 1");
     for (int l = 0; l < G; ++l) {
       const int64_t q = img[b * G + l];
       if (q < 0 || q >= N)
@@ -509,7 +522,7 @@ extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
       im[b * G + l] = (int32_t)q;
     }
   }
-  const size_t bi = im.size() * sizeof(int32_t), bs = im.size() * sizeof(cd);
+  const size_t bi = im.size() * sizeof(int32_t), bs = (size_t)nirr * im.size() * sizeof(cd);
   const size_t bytes = bs + ((bi + 255) & ~(size_t)255);
   void* buf = nullptr;
   int rc = qb_dev_alloc(ctx, &buf, bytes);
@@ -526,6 +539,7 @@ extern "C" int qb_geom_set_orbits(qb_ctx* ctx, qb_geom* g, int64_t m, int32_t G,
   DevSym sym;
   sym.m = m;
   sym.G = G;
+  sym.nirr = nirr;
   sym.img = di;
   sym.scale = ds;
   // Pair sharing (k_assemble_red_pair): find, for each image l, the image
@@ -639,7 +653,7 @@ extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const doub
   if (rc) return rc;
   DevTables t;
   cd* od = nullptr;
-  size_t obytes = (size_t)nz * N * N * sizeof(cd);
+  size_t obytes = (size_t)nz * qb_nirr(g) * N * N * sizeof(cd);
   bool on_dev = (flags & QB_OUT_DEVICE) != 0;
   hipError_t e = hipMemcpyAsync(zd, z, nz * sizeof(cd), hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
@@ -733,6 +747,7 @@ extern "C" int qb_residuals(qb_ctx* ctx, const qb_geom* g, int32_t op, const dou
   if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  if (qb_nirr(g) != 1) return qb_fail(ctx, QB_E_ARG, "the residual check takes one irrep");
   int rc = check_cheb(ctx, cheb);
   if (rc) return rc;
   if (nl == 0) return QB_OK;
@@ -830,7 +845,10 @@ extern "C" int qb_residuals(qb_ctx* ctx, const qb_geom* g, int32_t op, const dou
 struct qb_window {
   qb_ctx* ctx = nullptr;
   const qb_geom* g = nullptr;
+  // nq: matrices of the window (the batch of every launch) = nc contour
+  // nodes x nirr irreps assembled together (irrep-major: e * nc + j)
   int op = 0, nq = 0, precision = 0;
+  int nc = 0, nirr = 1;
   int64_t N = 0;
   double k0re = 0, k0im = 0, R = 0;
   std::vector<cd> zj, wj, wzj;  // host copies
@@ -879,8 +897,8 @@ extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision,
 static void win_free(qb_window* w) {
   qb_ctx* ctx = w->ctx;
   size_t NN = (size_t)w->N * w->N;
-  qb_dev_free(ctx, w->z_dev, w->nq * sizeof(cd));
-  qb_dev_free(ctx, w->w_dev, 2 * w->nq * sizeof(cd));
+  qb_dev_free(ctx, w->z_dev, w->nc * sizeof(cd));
+  qb_dev_free(ctx, w->w_dev, 2 * w->nc * sizeof(cd));
   qb_dev_free(ctx, w->T64, (size_t)w->nq * NN * sizeof(cd));
   qb_dev_free(ctx, w->LU32, (size_t)w->nq * NN * sizeof(cf));
   qb_dev_free(ctx, w->ipiv, (size_t)w->nq * w->N * sizeof(int));
@@ -934,7 +952,9 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   w->ctx = ctx;
   w->g = g;
   w->op = op;
-  w->nq = nq;
+  w->nc = nq;
+  w->nirr = qb_nirr(g);
+  w->nq = nq * w->nirr;
   w->precision = precision;
   w->N = qb_dim(g);
   w->k0re = k0_re;
@@ -942,27 +962,29 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   w->R = R;
   const int64_t N = w->N;
   const size_t NN = (size_t)N * N;
+  const int nc = w->nc;  // contour nodes; nq from here on is the batch nc x nirr
+  nq = w->nq;
   double t0 = now_ms(), t1, t2;
   DevTables tab;
-  std::vector<cd> wbuf(2 * nq);
+  std::vector<cd> wbuf(2 * nc);
   double rmin, rmax;
   op_range(g, op, &rmin, &rmax);
   // contour nodes and weights (beyn.jl:258-261)
-  w->zj.resize(nq);
-  w->wj.resize(nq);
-  w->wzj.resize(nq);
-  for (int j = 0; j < nq; ++j) {
-    long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)nq;
+  w->zj.resize(nc);
+  w->wj.resize(nc);
+  w->wzj.resize(nc);
+  for (int j = 0; j < nc; ++j) {
+    long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)nc;
     double c = (double)cosl(th), s = (double)sinl(th);
     cd e = mk(c, s);
     w->zj[j] = mk(k0_re + R * c, k0_im + R * s);
-    w->wj[j] = (R / (double)nq) * e;
+    w->wj[j] = (R / (double)nc) * e;
     w->wzj[j] = w->wj[j] * w->zj[j];
     wbuf[j] = w->wj[j];
-    wbuf[nq + j] = w->wzj[j];
+    wbuf[nc + j] = w->wzj[j];
   }
-  QB_TRY(qb_dev_alloc(ctx, (void**)&w->z_dev, nq * sizeof(cd)));
-  QB_TRY(qb_dev_alloc(ctx, (void**)&w->w_dev, 2 * nq * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->z_dev, nc * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->w_dev, 2 * nc * sizeof(cd)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->T64, (size_t)nq * NN * sizeof(cd)));
   if (precision == QB_MIXED) {
     QB_TRY(qb_dev_alloc(ctx, (void**)&w->LU32, (size_t)nq * NN * sizeof(cf)));
@@ -972,17 +994,17 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->perm, (size_t)nq * N * sizeof(int)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->info, nq * sizeof(int)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->scratch_d, 3 * nq * sizeof(double)));
-  if (hipMemcpyAsync(w->z_dev, w->zj.data(), nq * sizeof(cd), hipMemcpyHostToDevice,
+  if (hipMemcpyAsync(w->z_dev, w->zj.data(), nc * sizeof(cd), hipMemcpyHostToDevice,
                      ctx->stream) != hipSuccess ||
-      hipMemcpyAsync(w->w_dev, wbuf.data(), 2 * nq * sizeof(cd), hipMemcpyHostToDevice,
+      hipMemcpyAsync(w->w_dev, wbuf.data(), 2 * nc * sizeof(cd), hipMemcpyHostToDevice,
                      ctx->stream) != hipSuccess ||
       hipMemsetAsync(w->info, 0, nq * sizeof(int), ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "window upload failed");
     goto fail;
   }
   // tables + assembly (construct_boundary_matrices!, beyn.jl:264)
-  QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nq, rmin, rmax, *cheb, true, &tab));
-  QB_TRY(qb_assemble_dev(ctx, g, op, w->z_dev, nq, tab, w->T64,
+  QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nc, rmin, rmax, *cheb, true, &tab));
+  QB_TRY(qb_assemble_dev(ctx, g, op, w->z_dev, nc, tab, w->T64,
                          precision == QB_MIXED ? w->LU32 : nullptr));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "assembly failed");
@@ -1004,7 +1026,7 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   }
   t2 = now_ms();
   w->st.N = N;
-  w->st.nq = nq;
+  w->st.nq = nc;
   w->st.precision = precision;
   w->st.t_assemble_ms = t1 - t0;
   w->st.t_factor_ms = t2 - t1;
@@ -1087,6 +1109,7 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
   const int nq = w->nq;
   const size_t NN = (size_t)N * N;
   const int64_t nv = N * (int64_t)r;
+  const int nc = w->nc, nirr = w->nirr;  // nq = nc x nirr matrices
   double t0 = now_ms();
   cd *Vd = nullptr, *X = nullptr, *Rr = nullptr, *A0d = nullptr, *A1d = nullptr;
   cf* D32 = nullptr;
@@ -1097,12 +1120,11 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
   std::vector<double> h3(3 * nq);
   QB_TRY(qb_dev_alloc(ctx, (void**)&Vd, nv * sizeof(cd)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&X, (size_t)nq * nv * sizeof(cd)));
-  if (dev_out) {
-    A0d = dev_out;
-    A1d = dev_out + nv;
-  } else {
-    QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, nv * sizeof(cd)));
-    QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, nv * sizeof(cd)));
+  // moments per irrep e: device output [A0_e | A1_e] blocks (2 nv each); host
+  // outputs A0 and A1 hold the nirr blocks one after the other
+  if (!dev_out) {
+    QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, (size_t)nirr * nv * sizeof(cd)));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, (size_t)nirr * nv * sizeof(cd)));
   }
   if (hipMemcpyAsync(Vd, V, nv * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "V upload failed");
@@ -1255,11 +1277,16 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       if (be[j] <= refine_tol) resid_max = std::max(resid_max, be[j]);
     w->st.nodes_fp64_fallback = (int)w->fb_nodes.size();
   }
-  QB_TRY(qb_moments(ctx, X, nv, nq, w->w_dev, w->w_dev + nq, A0d, A1d));
+  for (int e = 0; e < nirr; ++e) {
+    cd* a0 = dev_out ? dev_out + (size_t)2 * e * nv : A0d + (size_t)e * nv;
+    cd* a1 = dev_out ? dev_out + (size_t)2 * e * nv + nv : A1d + (size_t)e * nv;
+    QB_TRY(qb_moments(ctx, X + (size_t)e * nc * nv, nv, nc, w->w_dev, w->w_dev + nc, a0, a1));
+  }
   if (!dev_out &&
-      (hipMemcpyAsync(A0, A0d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
-       hipMemcpyAsync(A1, A1d, nv * sizeof(cd), hipMemcpyDeviceToHost, ctx->stream) !=
-           hipSuccess)) {
+      (hipMemcpyAsync(A0, A0d, (size_t)nirr * nv * sizeof(cd), hipMemcpyDeviceToHost,
+                      ctx->stream) != hipSuccess ||
+       hipMemcpyAsync(A1, A1d, (size_t)nirr * nv * sizeof(cd), hipMemcpyDeviceToHost,
+                      ctx->stream) != hipSuccess)) {
     rc = qb_fail(ctx, QB_E_HIP, "moment download failed");
     goto fail;
   }
@@ -1281,8 +1308,8 @@ fail:
   qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, D32, (size_t)nq * nv * sizeof(cf));
   if (!dev_out) {
-    qb_dev_free(ctx, A0d, nv * sizeof(cd));
-    qb_dev_free(ctx, A1d, nv * sizeof(cd));
+    qb_dev_free(ctx, A0d, (size_t)nirr * nv * sizeof(cd));
+    qb_dev_free(ctx, A1d, (size_t)nirr * nv * sizeof(cd));
   }
   return rc;
 }
@@ -1303,7 +1330,8 @@ extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, do
   if (r < 1 || r > w->N) return qb_fail(ctx, QB_E_ARG, "r must be in [1, N]");
   (void)hipSetDevice(ctx->device);
   const int64_t N = w->N;
-  const size_t need = (size_t)2 * N * r;
+  const int nirr = w->nirr;
+  const size_t need = (size_t)2 * N * r * nirr;
   if (w->qr_cap != need) {
     qb_dev_free(ctx, w->qr, w->qr_cap * sizeof(cd));
     qb_dev_free(ctx, w->qr_tau, (size_t)w->qr_rcap * sizeof(cd));
@@ -1312,43 +1340,62 @@ extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, do
     w->qr_cap = 0;
     w->qr_rcap = 0;
     int rc = qb_dev_alloc(ctx, (void**)&w->qr, need * sizeof(cd));
-    if (!rc) rc = qb_dev_alloc(ctx, (void**)&w->qr_tau, (size_t)r * sizeof(cd));
+    if (!rc) rc = qb_dev_alloc(ctx, (void**)&w->qr_tau, (size_t)r * nirr * sizeof(cd));
     if (rc) {
       qb_dev_free(ctx, w->qr, need * sizeof(cd));
       w->qr = nullptr;
       return rc;
     }
     w->qr_cap = need;
-    w->qr_rcap = r;
+    w->qr_rcap = r * nirr;
   }
   w->qr_r = 0;
   int rc = win_moments(w, r, V, nullptr, nullptr, refine_tol, nullptr, w->qr);
   if (rc) return rc;
-  rc = qb_house_qr(ctx, w->qr, N, r, 2 * r, w->qr_tau);
-  if (rc) return rc;
-  const cd* Cd = w->qr + (size_t)N * r;
-  hipError_t e = hipMemcpy2DAsync(R, r * sizeof(cd), w->qr, N * sizeof(cd), r * sizeof(cd), r,
-                                  hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess)
-    e = hipMemcpy2DAsync(Cm, r * sizeof(cd), Cd, N * sizeof(cd), r * sizeof(cd), r,
-                         hipMemcpyDeviceToHost, ctx->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  // one QR per irrep on its [A0_e | A1_e] block
+  for (int ie = 0; ie < nirr; ++ie) {
+    cd* blk = w->qr + (size_t)2 * ie * N * r;
+    rc = qb_house_qr(ctx, blk, N, r, 2 * r, w->qr_tau + (size_t)ie * r);
+    if (rc) return rc;
+    const cd* Cd = blk + (size_t)N * r;
+    double* Re = R + (size_t)2 * ie * r * r;
+    double* Ce = Cm + (size_t)2 * ie * r * r;
+    hipError_t e = hipMemcpy2DAsync(Re, r * sizeof(cd), blk, N * sizeof(cd), r * sizeof(cd), r,
+                                    hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(Ce, r * sizeof(cd), Cd, N * sizeof(cd), r * sizeof(cd), r,
+                           hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return qb_fail(ctx, QB_E_HIP, "moment QR failed");
   // R: zero the reflectors stored below the diagonal
-  for (int c = 0; c < r; ++c)
-    for (int i = c + 1; i < r; ++i) R[2 * ((size_t)c * r + i)] = R[2 * ((size_t)c * r + i) + 1] = 0;
+  for (int ie = 0; ie < nirr; ++ie) {
+    double* Re = R + (size_t)2 * ie * r * r;
+    for (int c = 0; c < r; ++c)
+      for (int i = c + 1; i < r; ++i)
+        Re[2 * ((size_t)c * r + i)] = Re[2 * ((size_t)c * r + i) + 1] = 0;
+  }
   w->qr_r = r;
   if (stats) *stats = w->st;
   return QB_OK;
 }
 
 // U = Q [Ur; 0] (N x k) with the reflectors of the last qb_window_moments_qr.
+extern "C" int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* Ur, int32_t k,
+                                     double* U);
 extern "C" int qb_window_basis(qb_window* w, const double* Ur, int32_t k, double* U) {
+  return qb_window_basis_irrep(w, 0, Ur, k, U);
+}
+
+extern "C" int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* Ur, int32_t k,
+                                     double* U) {
   if (!w) return QB_E_ARG;
   qb_ctx* ctx = w->ctx;
   if (!U || (k > 0 && !Ur) || k < 0) return qb_fail(ctx, QB_E_ARG, "null argument");
   if (w->qr_r == 0) return qb_fail(ctx, QB_E_STATE, "no QR of the moments on this window");
   if (k > w->qr_r) return qb_fail(ctx, QB_E_ARG, "k must be <= r");
+  if (irrep < 0 || irrep >= w->nirr) return qb_fail(ctx, QB_E_ARG, "irrep out of range");
   if (k == 0) return QB_OK;
   (void)hipSetDevice(ctx->device);
   const int64_t N = w->N;
@@ -1362,7 +1409,9 @@ extern "C" int qb_window_basis(qb_window* w, const double* Ur, int32_t k, double
     e = hipMemcpy2DAsync(Y, N * sizeof(cd), Ur, r * sizeof(cd), r * sizeof(cd), k,
                          hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
-  if (!rc) rc = qb_house_apply_q(ctx, w->qr, N, r, w->qr_tau, Y, k);
+  if (!rc)
+    rc = qb_house_apply_q(ctx, w->qr + (size_t)2 * irrep * N * r, N, r,
+                          w->qr_tau + (size_t)irrep * r, Y, k);
   if (!rc && (hipMemcpyAsync(U, Y, bytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
               hipStreamSynchronize(ctx->stream) != hipSuccess))
     rc = qb_fail(ctx, QB_E_HIP, "basis download failed");

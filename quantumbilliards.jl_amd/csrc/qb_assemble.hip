@@ -525,6 +525,8 @@ int qb_assemble_deriv_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   if (!t.J) return qb_fail(ctx, QB_E_ARG, "derivative assembly needs the J tables");
+  if (g->sym.m > 0 && g->sym.nirr != 1)
+    return qb_fail(ctx, QB_E_ARG, "derivative assembly takes one irrep");
   if (g->sym.m > 0)
     return qb_assemble_red_deriv_dev(ctx, g, op, z_dev, nz, t, out, dout, ddout, off_sign);
   int64_t N = g->d.N;
@@ -608,16 +610,20 @@ __device__ __forceinline__ TabView tab_view(const cd* H, const cd* J, int64_t q,
   return tv;
 }
 
-template <int OP, int MH, int MJ>
+// One thread per reduced entry (a, b); with several irreps (s.nirr, at most
+// MAXI) every image term is evaluated once and summed into the entry of each
+// irrep with that irrep's weight.  Output matrix e * nz + q (irrep-major).
+template <int OP, int MH, int MJ, int MAXI>
 __global__ void __launch_bounds__(256) k_assemble_red(DevGeom g, DevSym s, const cd* __restrict__ zk,
                                                       const cd* __restrict__ H,
                                                       const cd* __restrict__ J, int np_h, int M_h,
                                                       int np_j, int M_j, double rmin_h,
-                                                      double rmax, int64_t q0,
+                                                      double rmax, int64_t q0, int64_t nz,
                                                       cd* __restrict__ out64,
                                                       cf* __restrict__ out32) {
   const int64_t m = s.m;
   const int G = s.G;
+  const int nirr = s.nirr;
   const int64_t q = q0 + blockIdx.z;
   const TabView tv = tab_view(H, J, q, np_h, M_h, np_j, M_j, rmin_h, rmax);
   const cd k = zk[q];
@@ -627,16 +633,16 @@ __global__ void __launch_bounds__(256) k_assemble_red(DevGeom g, DevSym s, const
   const int64_t i = s.img[a * G];
   const double xi = g.x[i], yi = g.y[i];
   const int ci = g.comp[i];
-  const int64_t base = (q - q0) * m * m;
 #pragma unroll 1
   for (int c = 0; c < 16; ++c) {
     const int64_t b = (int64_t)blockIdx.y * 64 + jl0 + 4 * c;
     if (b >= m) break;
-    cd acc = mk(0, 0);
+    cd acc[MAXI];
+#pragma unroll
+    for (int e = 0; e < MAXI; ++e) acc[e] = mk(0, 0);
 #pragma unroll 1
     for (int l = 0; l < G; ++l) {
       const int64_t j = s.img[b * G + l];
-      const cd sc = s.scale[b * G + l];
       cd v;
       if (i == j) {
         v = diag_entry<OP>(g, j, k);
@@ -645,11 +651,17 @@ __global__ void __launch_bounds__(256) k_assemble_red(DevGeom g, DevSym s, const
         const Spec sp = specials<OP, MH, MJ>(tv, p.r, k, p.same || OP == QB_DLP_KRESS);
         v = offdiag_entry<OP>(g, i, j, p.dx, p.dy, p.invr, p.same, p.Rij, p.lt, k, sp);
       }
-      acc = acc + sc * v;
+#pragma unroll
+      for (int e = 0; e < MAXI; ++e)
+        if (e < nirr) acc[e] = acc[e] + s.scale[((int64_t)e * m + b) * G + l] * v;
     }
-    const int64_t idx = base + a + b * m;
-    if (out64) out64[idx] = acc;
-    if (out32) out32[idx] = to_cf(acc);
+#pragma unroll
+    for (int e = 0; e < MAXI; ++e) {
+      if (e >= nirr) break;
+      const int64_t idx = ((int64_t)e * nz + q) * m * m + a + b * m;
+      if (out64) out64[idx] = acc[e];
+      if (out32) out32[idx] = to_cf(acc[e]);
+    }
   }
 }
 
@@ -863,23 +875,35 @@ __global__ void __launch_bounds__(256) k_assemble_red_deriv(
   }
 }
 
+// o64 / o32: the output base (matrix 0); the kernels index matrix e * nz + q.
 template <int OP, int MH, int MJ>
 static void launch_red(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, int64_t nzc,
-                       const DevTables& t, cd* o64, cf* o32) {
+                       int64_t nz, const DevTables& t, cd* o64, cf* o32) {
   const int64_t m = g->sym.m;
+  const int nirr = g->sym.nirr;
   const unsigned Tm = (unsigned)((m + 63) / 64);
-  const double ent = (double)m * m * nzc;
+  const double ent = (double)m * m * nzc * nirr;
   int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
                            ent * ((o64 ? sizeof(cd) : 0) + (o32 ? sizeof(cf) : 0)));
-  if (g->sym.pair) {
+  const dim3 grid(Tm, Tm, (unsigned)nzc);
+  if (g->sym.pair && nirr == 1) {
     const int64_t pairs = (int64_t)Tm * (Tm + 1) / 2;
     hipLaunchKernelGGL((k_assemble_red_pair<OP, MH, MJ>), dim3((unsigned)pairs, 1, (unsigned)nzc),
                        dim3(256), 0, ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h,
-                       t.np_j, t.M_j, t.rmin_h, t.rmax, q0, (int)Tm, o64, o32);
+                       t.np_j, t.M_j, t.rmin_h, t.rmax, q0, (int)Tm,
+                       o64 ? o64 + q0 * m * m : nullptr, o32 ? o32 + q0 * m * m : nullptr);
+  } else if (nirr == 1) {
+    hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ, 1>), grid, dim3(256), 0, ctx->stream, g->d,
+                       g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0,
+                       nz, o64, o32);
+  } else if (nirr <= 8) {
+    hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ, 8>), grid, dim3(256), 0, ctx->stream, g->d,
+                       g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0,
+                       nz, o64, o32);
   } else {
-    hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ>), dim3(Tm, Tm, (unsigned)nzc), dim3(256), 0,
+    hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ, QB_MAX_IRREPS>), grid, dim3(256), 0,
                        ctx->stream, g->d, g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j,
-                       t.rmin_h, t.rmax, q0, o64, o32);
+                       t.rmin_h, t.rmax, q0, nz, o64, o32);
   }
   qb_prof_end(ctx, slot);
 }
@@ -899,26 +923,23 @@ static void launch_red_deriv(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t
 
 int qb_assemble_red_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
                         const DevTables& t, cd* out64, cf* out32) {
-  const int64_t m = g->sym.m;
   for (int64_t q0 = 0; q0 < nz; q0 += 1024) {
     const int64_t nzc = nz - q0 < 1024 ? nz - q0 : 1024;
-    cd* o64 = out64 ? out64 + q0 * m * m : nullptr;
-    cf* o32 = out32 ? out32 + q0 * m * m : nullptr;
     const int mh = t.M_h, mj = t.M_j;
     if (op == QB_CFIE_KRESS) {
       if (mh == 7 && mj == 7)
-        launch_red<QB_CFIE_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+        launch_red<QB_CFIE_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, nz, t, out64, out32);
       else if (mh == 5 && mj == 5)
-        launch_red<QB_CFIE_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+        launch_red<QB_CFIE_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, nz, t, out64, out32);
       else
-        launch_red<QB_CFIE_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+        launch_red<QB_CFIE_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, nz, t, out64, out32);
     } else {
       if (mh == 7 && mj == 7)
-        launch_red<QB_DLP_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+        launch_red<QB_DLP_KRESS, 7, 7>(ctx, g, z_dev, q0, nzc, nz, t, out64, out32);
       else if (mh == 5 && mj == 5)
-        launch_red<QB_DLP_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+        launch_red<QB_DLP_KRESS, 5, 5>(ctx, g, z_dev, q0, nzc, nz, t, out64, out32);
       else
-        launch_red<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, t, o64, o32);
+        launch_red<QB_DLP_KRESS, 0, 0>(ctx, g, z_dev, q0, nzc, nz, t, out64, out32);
     }
     QB_HIP(ctx, hipGetLastError());
   }

@@ -39,6 +39,10 @@ struct DevGeom {
 struct DevSym {
   int64_t m = 0;  // reduced dimension (number of orbits)
   int32_t G = 0;  // orbit size (group order)
+  // irreps assembled together (qb_geom_set_orbits_multi): scale holds nirr
+  // tables of m x G weights, one per irrep; every assembly then produces
+  // nirr matrices per wavenumber, irrep-major (matrix e * nz + q).
+  int32_t nirr = 1;
   const int32_t* img = nullptr;
   const cd* scale = nullptr;
   // Pair sharing: when the images come from isometries of the node set
@@ -125,6 +129,9 @@ struct qb_geom {
 // Dimension of the matrices assembled on a geometry: the number of orbits
 // when symmetry orbits are set, else the number of nodes.
 static inline int64_t qb_dim(const qb_geom* g) { return g->sym.m > 0 ? g->sym.m : g->d.N; }
+// Matrices per wavenumber of an assembly on g (irreps assembled together).
+static inline int qb_nirr(const qb_geom* g) { return g->sym.m > 0 ? g->sym.nirr : 1; }
+#define QB_MAX_IRREPS 16
 
 // Chebyshev tables of one or more wavenumbers, resident on the device.
 // H: [nz][np_h][M_h+1] of {H0 coeff, H1 coeff}; J: [nz][np_j][M_j+1] of

@@ -26,6 +26,7 @@ EXPORTS = (
     "qb_ctx_autotune", "qb_ctx_set_cache_limit", "qb_residuals", "qb_assemble_deriv",
     "qb_ctx_mem_info", "qb_ctx_set_graphs", "qb_window_moments_qr", "qb_window_basis",
     "qb_geom_set_orbits", "qb_geom_dim", "qb_host_alloc", "qb_host_free",
+    "qb_geom_set_orbits_multi", "qb_geom_irreps", "qb_window_basis_irrep",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
@@ -86,6 +87,9 @@ def lib():
         "qb_host_alloc": (C.c_int, [P, I64, PP]),
         "qb_host_free": (None, [P]),
         "qb_geom_set_orbits": (C.c_int, [P, P, I64, I32, P, P]),
+        "qb_geom_set_orbits_multi": (C.c_int, [P, P, I64, I32, I32, P, P]),
+        "qb_geom_irreps": (I32, [P]),
+        "qb_window_basis_irrep": (C.c_int, [P, I32, P, I32, P]),
         "qb_assemble": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P]),
         "qb_assemble_deriv": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, P, P, I32,
                                         P]),

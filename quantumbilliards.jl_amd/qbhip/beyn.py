@@ -60,6 +60,7 @@ class Window:
         self.ctx = ctx or (geom.ctx if geom is not None else default_context())
         self.geom = geom or Geometry(pts, self.ctx, solver)
         self.N = self.geom.dim  # reduced dimension with a solver symmetry
+        self.nirr = getattr(self.geom, "nirr", 1)  # irreps factored together
         self.nq = nq
         self.k0, self.R = complex(k0), float(R)
         self.precision = precision
@@ -77,14 +78,17 @@ class Window:
         V = np.asfortranarray(V, dtype=np.complex128)
         assert V.shape[0] == self.N
         r = V.shape[1]
-        A0 = np.empty((self.N, r), dtype=np.complex128, order="F")
-        A1 = np.empty((self.N, r), dtype=np.complex128, order="F")
+        raw0 = np.empty((self.nirr, r, self.N), dtype=np.complex128)
+        raw1 = np.empty((self.nirr, r, self.N), dtype=np.complex128)
         st = _lib.Stats()
-        rc = _lib.lib().qb_window_moments(self.ptr, int(r), V.ctypes.data, A0.ctypes.data,
-                                          A1.ctypes.data, float(refine_tol), C.byref(st))
+        rc = _lib.lib().qb_window_moments(self.ptr, int(r), V.ctypes.data, raw0.ctypes.data,
+                                          raw1.ctypes.data, float(refine_tol), C.byref(st))
         _lib.check(self.ctx.ptr, rc)
         self.stats = st.as_dict()
-        return A0, A1
+        if self.nirr == 1:
+            return raw0[0].T, raw1[0].T
+        # one (N x r) block per irrep
+        return raw0.transpose(0, 2, 1), raw1.transpose(0, 2, 1)
 
     def moments_qr(self, V: np.ndarray, refine_tol: float = 1e-13):
         """Moments kept on the device and reduced there: A0 = Q R; returns
@@ -92,23 +96,28 @@ class Window:
         V = np.asfortranarray(V, dtype=np.complex128)
         assert V.shape[0] == self.N
         r = V.shape[1]
-        R = np.empty((r, r), dtype=np.complex128, order="F")
-        Cm = np.empty((r, r), dtype=np.complex128, order="F")
+        R = np.empty((self.nirr, r, r), dtype=np.complex128)
+        Cm = np.empty((self.nirr, r, r), dtype=np.complex128)
         st = _lib.Stats()
         rc = _lib.lib().qb_window_moments_qr(self.ptr, int(r), V.ctypes.data, R.ctypes.data,
                                              Cm.ctypes.data, float(refine_tol), C.byref(st))
         _lib.check(self.ctx.ptr, rc)
         self.stats = st.as_dict()
-        return R, Cm
+        if self.nirr == 1:
+            return R[0].T, Cm[0].T
+        # one (R, C) pair per irrep, stacked: R[e], C[e]
+        return R.transpose(0, 2, 1), Cm.transpose(0, 2, 1)
 
-    def basis(self, Ur: np.ndarray) -> np.ndarray:
-        """U = Q [Ur; 0] (N x k) from the last moments_qr (qb_window_basis)."""
+    def basis(self, Ur: np.ndarray, irrep: int = 0) -> np.ndarray:
+        """U = Q [Ur; 0] (N x k) from the last moments_qr (qb_window_basis);
+        `irrep` selects the irrep of a multi-irrep window."""
         Ur = np.asfortranarray(Ur, dtype=np.complex128)
         k = Ur.shape[1]
         U = np.empty((self.N, k), dtype=np.complex128, order="F")
         if k == 0:
             return U
-        rc = _lib.lib().qb_window_basis(self.ptr, Ur.ctypes.data, int(k), U.ctypes.data)
+        rc = _lib.lib().qb_window_basis_irrep(self.ptr, int(irrep), Ur.ctypes.data, int(k),
+                                              U.ctypes.data)
         _lib.check(self.ctx.ptr, rc)
         return U
 

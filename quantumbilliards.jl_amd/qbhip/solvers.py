@@ -74,7 +74,10 @@ class Geometry:
     orbit table (qb_geom_set_orbits) and every assembly on it is the
     symmetry-reduced one; `N` is the node count, `dim` the matrix dimension."""
 
-    def __init__(self, pts, ctx: Optional[Context] = None, solver=None):
+    def __init__(self, pts, ctx: Optional[Context] = None, solver=None, irreps=None):
+        """irreps: solvers with symmetries of one group (e.g. Rotation(5, m),
+        m = 0..4), assembled together (qb_geom_set_orbits_multi): every
+        assembly or window on this geometry then covers all of them."""
         self.ctx = ctx or default_context()
         comps = _as_list(pts)
         cat = lambda f: np.ascontiguousarray(np.concatenate([f(p) for p in comps]), dtype=np.float64)
@@ -92,8 +95,23 @@ class Geometry:
         _lib.check(self.ctx.ptr, rc)
         self.ptr = h
         self.dim = self.N
+        self.nirr = 1
         self.symmetry = getattr(solver, "symmetry", None)
         orb = orbits_for(solver, pts) if solver is not None else None
+        if irreps:
+            tabs = [orbits_for(s_, pts) for s_ in irreps]
+            img = np.ascontiguousarray(tabs[0][0], dtype=np.int64)
+            if any(t is None or not np.array_equal(t[0], img) for t in tabs):
+                raise ValueError("irreps must share one orbit table (one symmetry group)")
+            sc = np.ascontiguousarray(np.stack([t[1] for t in tabs]), dtype=np.complex128)
+            rc = L.qb_geom_set_orbits_multi(self.ctx.ptr, self.ptr, img.shape[0], img.shape[1],
+                                            len(tabs), img.ctypes.data, sc.ctypes.data)
+            _lib.check(self.ctx.ptr, rc)
+            self.dim = int(img.shape[0])
+            self.nirr = len(tabs)
+            self.irreps = list(irreps)
+            self.orbits = (img, sc)
+            orb = None
         if orb is not None:
             img, sc = orb
             img = np.ascontiguousarray(img, dtype=np.int64)
@@ -131,11 +149,12 @@ def assemble(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_panels_j=3000, M_j=5
     g = geom or Geometry(pts, ctx, solver)
     zj = np.ascontiguousarray(np.asarray(zj, dtype=np.complex128).ravel())
     nz, N = len(zj), g.dim
+    nm = nz * getattr(g, "nirr", 1)  # matrix e * nz + q for irrep e of a multi-irrep geometry
     if out is None:
-        raw = np.empty((nz, N, N), dtype=np.complex128)
+        raw = np.empty((nm, N, N), dtype=np.complex128)
     else:
         raw = out
-        assert raw.shape == (nz, N, N) and raw.dtype == np.complex128 and raw.flags.c_contiguous
+        assert raw.shape == (nm, N, N) and raw.dtype == np.complex128 and raw.flags.c_contiguous
     cp = _cheb(n_panels_h, M_h, n_panels_j, M_j)
     rr = np.zeros(2)
     rc = _lib.lib().qb_assemble(ctx.ptr, g.ptr, solver.op, zj.ctypes.data, nz, C.byref(cp),

@@ -236,3 +236,72 @@ def test_orbit_table_errors():
     assert L.qb_geom_set_orbits(ctx.ptr, g.ptr, 0, 0, None, None) == 0
     assert L.qb_geom_dim(g.ptr) == 122
     g.close()
+
+
+def _star_irreps(N=400):
+    q = qb()
+    b = q.make_star_polar_boundary(1.0, 0.2, 5)
+    solvers = [q.CFIE_kress(10.0, b, symmetry=q.Rotation(5, m)) for m in range(5)]
+    pts = q.evaluate_points(q.CFIE_kress(10.0, b), b, 30.0, N=[N])
+    return b, solvers, pts
+
+
+def test_multi_irrep_assembly_matches_single_irreps():
+    """Assembling the five C_5 irreps together (one evaluation of every image
+    term, qb_geom_set_orbits_multi) gives each irrep's reduced matrices."""
+    q = qb()
+    b, solvers, pts = _star_irreps(600)
+    zj = np.array([30.2 + 0.1j, 29.7 - 0.2j, 30.4 + 0.0j])
+    cp = dict(n_panels_h=3000, M_h=7, n_panels_j=1000, M_j=7)
+    g = q.Geometry(pts, None, irreps=solvers)
+    assert g.nirr == 5 and q.lib().qb_geom_irreps(g.ptr) == 5
+    T = q.assemble(solvers[0], pts, zj, geom=g, **cp)
+    assert T.shape == (15, 120, 120)
+    for m, s in enumerate(solvers):
+        ref = q.assemble(s, pts, zj, **cp)
+        got = T[m * 3:(m + 1) * 3]
+        err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(m, err)
+        assert err < 1e-13
+    # one irrep only for the derivative assembly and the residual check
+    with pytest.raises(AssertionError, match="one irrep"):
+        q.assemble_with_derivatives(solvers[0], pts, zj, geom=g, **cp)
+    with pytest.raises(AssertionError, match="one irrep"):
+        q.residual_norms(solvers[0], pts, zj[:1], np.zeros((120, 1), complex), geom=g, **cp)
+    g.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multi_irrep_window_matches_single_irrep_windows(precision):
+    """One window over all five C_5 irreps (one batched LU of 5 x 48
+    matrices, moments and QR per irrep) gives the eigenvalues and bases of
+    the five single-irrep windows."""
+    q = qb()
+    b, solvers, pts = _star_irreps(400)
+    cp = dict(n_panels_h=3000, M_h=7, n_panels_j=1000, M_j=7)
+    g = q.Geometry(pts, None, irreps=solvers)
+    V = q.probe_block(np.random.default_rng(7), 80, 40)
+    w = q.Window(solvers[0], pts, 30.0 + 0j, 0.3, 48, precision=precision, geom=g, **cp)
+    assert w.nirr == 5
+    R, Cm = w.moments_qr(V, 1e-13)
+    assert R.shape == (5, 40, 40)
+    for m, s in enumerate(solvers):
+        B, Ur, S, rk = q.B_from_qr(R[m], Cm[m], 1e-10)
+        lam = np.sort_complex(np.linalg.eigvals(B))
+        w1 = q.Window(s, pts, 30.0 + 0j, 0.3, 48, precision=precision, **cp)
+        R1, C1 = w1.moments_qr(V, 1e-13)
+        B1, Ur1, S1, rk1 = q.B_from_qr(R1, C1, 1e-10)
+        lam1 = np.sort_complex(np.linalg.eigvals(B1))
+        # leading directions only: those near the 1e-10 rank threshold are
+        # ill-conditioned and turn under rounding-level changes of A0
+        kl = int(np.count_nonzero(S1 >= 1e-6 * S1[0]))
+        U1 = w1.basis(Ur1[:, :kl])
+        w1.close()
+        assert rk == rk1 and np.max(np.abs(S - S1)) <= 1e-12 * S1[0], m
+        inside = np.abs(lam1 - 30.0) <= 0.3
+        assert np.max(np.abs(lam[inside] - lam1[inside])) < 1e-10 * 30, m
+        U = w.basis(Ur[:, :kl], irrep=m)
+        P, P1 = U @ U.conj().T, U1 @ U1.conj().T
+        assert np.max(np.abs(P - P1)) < 1e-9, m
+    w.close()
+    g.close()
