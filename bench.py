@@ -17,6 +17,7 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -66,6 +67,16 @@ def parse():
                     help="contexts that take irrep windows concurrently (--no-sym-batched)")
     ap.add_argument("--no-sym-batched", dest="sym_batched", action="store_false",
                     help="one window per irrep instead of one window over all irreps")
+    ap.add_argument("--uneven", action="store_true",
+                    help="uneven batched windows: the disk with a hole, k0 uniform in "
+                         "[100, 800], N from 10 points per wavelength rounded up to 256")
+    ap.add_argument("--uneven-windows", type=int, default=128)
+    ap.add_argument("--uneven-precision", default="fp64",
+                    help="fp64 (default: faster on this workload) or mixed")
+    ap.add_argument("--uneven-slots", type=int, default=2)
+    ap.add_argument("--uneven-max-group", type=int, default=16)
+    ap.add_argument("--no-group", dest="group", action="store_false",
+                    help="serve every window alone (no batching of windows that share N)")
     return ap.parse_args()
 
 
@@ -98,6 +109,65 @@ def run_serve(a):
                    "rate_per_s": a.serve_rate, "k0": [50.0, a.serve_kmax], "nq": 32,
                    "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns),
                    "symmetry_reduced": a.serve_symmetry}}), flush=True)
+
+
+def run_uneven(a):
+    """Uneven batched windows: `--uneven-windows` disks (R = 0.5, 32 contour
+    nodes) on the unit disk with a hole of radius 0.3 at (0.35, 0), k0 seeded
+    uniform in [100, 800], N = 10 k L / 2 pi rounded up to a multiple of 256
+    (1536 ... 10496, outer : hole = 10 : 3), all submitted at once to the
+    serving loop.  Windows that share N are batched into one window (one
+    assembly, one batched LU and solve over all their contour nodes), as many
+    as fit in HBM; windows of different N run side by side in `--uneven-slots`
+    contexts.  One untimed warmup window first.  Prints one JSON line."""
+    from qbhip import serve as S
+    reqs = S.uneven_windows(a.uneven_windows, seed=4, precision=a.uneven_precision)
+    srv = S.Server(slots=a.uneven_slots, group=a.group, max_group=a.uneven_max_group)
+    srv.submit(S.WindowRequest(-1, "circle_hole", 100.0, 0.5, nq=32,
+                               precision=a.uneven_precision,
+                               N=S.hole_node_counts(100.5))).result()
+    t0 = time.perf_counter()
+    done = [0]
+    lock = threading.Lock()
+
+    def progress(f):
+        with lock:
+            done[0] += 1
+            if done[0] % 8 == 0 or done[0] == len(reqs):
+                print(f"[uneven] {done[0]}/{len(reqs)} windows done at "
+                      f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    futs = []
+    for rq in reqs:  # all arrive at once
+        f = srv.submit(rq, t0)
+        f.add_done_callback(progress)
+        futs.append(f)
+    res = []
+    for f in futs:
+        try:
+            res.append(f.result())
+        except Exception as e:  # noqa: BLE001 - reported per window
+            res.append(e)
+    span = time.perf_counter() - t0
+    srv.close()
+    summ = S.summarize(res, t_span=span)
+    ok = [r for r in res if isinstance(r, S.WindowResult)]
+    Ns = sorted({r.N for r in ok})
+    groups = [r.stats.get("group", 1) for r in ok]
+    print(json.dumps({
+        "metric": "uneven batched windows: eigenvalues/sec (Beyn CFIE-Kress, N 1536-10496)",
+        "value": summ["eigenvalues_per_s"], "unit": "eigenvalues/s", "n_gpus": 1,
+        "windows": summ["windows"], "rejected": summ["rejected"],
+        "eigenvalues": summ["eigenvalues"], "span_s": span,
+        "p50_window_s": summ["p50_latency_s"], "p99_window_s": summ["p99_latency_s"],
+        "N_classes": len(Ns), "mean_group": float(np.mean(groups)) if groups else None,
+        "higher_is_better": True,
+        "dtype": "c64+c128-refine" if a.uneven_precision == "mixed" else "c128",
+        "data": "synthetic (seeded)",
+        "config": {"workload": "circle with hole (0.3 at (0.35, 0)), k0 ~ U[100, 800], R=0.5",
+                   "windows": len(reqs), "nq": 32, "N_min": min(Ns), "N_max": max(Ns),
+                   "slots": a.uneven_slots, "grouped": a.group,
+                   "max_group": a.uneven_max_group}}), flush=True)
 
 
 def run_symmetry(a):
@@ -392,6 +462,9 @@ def main():
         return
     if a.symmetry:
         run_symmetry(a)
+        return
+    if a.uneven:
+        run_uneven(a)
         return
     import qbhip as q
     ctx = q.default_context()

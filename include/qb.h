@@ -189,6 +189,24 @@ int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
                      double k0_im, double R, int32_t nq,
                      const qb_cheb_params* cheb, int32_t precision,
                      qb_window** out, qb_stats* stats);
+/* Several disks on one geometry as one window (continuous batching of
+ * requests that share N): contour c is k0[2c] + i k0[2c+1] + R[c] e^{i 2pi j/nq},
+ * and the nwin * nq matrices (times the irreps of a multi-irrep geometry) are
+ * assembled, factored and solved as one batch with one probe block V.  The
+ * window then has qb_window_blocks(w) = nirr * nwin moment blocks, block
+ * e * nwin + c for irrep e and contour c: qb_window_moments returns that many
+ * blocks of A0 and of A1, qb_window_moments_qr that many blocks of R and of
+ * C, and qb_window_basis_irrep takes the block index.  The admission
+ * footprint is qb_window_footprint(N, nq * nwin * nirr, ...).  Per block the
+ * result is that of a window of its own.  No reference counterpart: the
+ * reference's driver (compute_spectrum_beyn, beyn.jl:778-921) runs its disks
+ * one after another.  nq * nwin * nirr <= 4096. */
+int qb_window_create_multi(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin,
+                           const double* k0, const double* R, int32_t nq,
+                           const qb_cheb_params* cheb, int32_t precision, qb_window** out,
+                           qb_stats* stats);
+/* Moment blocks of a window: irreps x contours (1 for a plain window). */
+int32_t qb_window_blocks(const qb_window* w);
 void qb_window_destroy(qb_window* w);
 /* Device bytes a window of this size would hold (the admission footprint):
  * matrices, pivots, solve buffers for r probes, and up to 32 MiB of tables
@@ -243,8 +261,9 @@ int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, double* R, do
 /* U_k = Q [Ur; 0] (host N x k) from the last qb_window_moments_qr, Ur host
  * r x k: the U_k that construct_B_matrix returns (beyn.jl:311). */
 int qb_window_basis(qb_window* w, const double* Ur, int32_t k, double* U);
-/* The same for irrep e of a window on a multi-irrep geometry
- * (qb_geom_set_orbits_multi); qb_window_basis is irrep 0. */
+/* The same for moment block e (irrep e of a window on a multi-irrep geometry,
+ * qb_geom_set_orbits_multi; block e * nwin + c of qb_window_create_multi);
+ * qb_window_basis is block 0. */
 int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* Ur, int32_t k, double* U);
 
 /* Device evaluation of H0^(1), H1^(1), J0, J1 at n complex points (the values

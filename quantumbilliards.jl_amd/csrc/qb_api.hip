@@ -846,9 +846,11 @@ struct qb_window {
   qb_ctx* ctx = nullptr;
   const qb_geom* g = nullptr;
   // nq: matrices of the window (the batch of every launch) = nc contour
-  // nodes x nirr irreps assembled together (irrep-major: e * nc + j)
+  // nodes x nwin contours x nirr irreps assembled together; matrix
+  // (e * nwin + c) * nc + j for irrep e, contour c, node j.  A moment block
+  // is one (irrep, contour) pair: block e * nwin + c.
   int op = 0, nq = 0, precision = 0;
-  int nc = 0, nirr = 1;
+  int nc = 0, nirr = 1, nwin = 1;
   int64_t N = 0;
   double k0re = 0, k0im = 0, R = 0;
   std::vector<cd> zj, wj, wzj;  // host copies
@@ -897,8 +899,8 @@ extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision,
 static void win_free(qb_window* w) {
   qb_ctx* ctx = w->ctx;
   size_t NN = (size_t)w->N * w->N;
-  qb_dev_free(ctx, w->z_dev, w->nc * sizeof(cd));
-  qb_dev_free(ctx, w->w_dev, 2 * w->nc * sizeof(cd));
+  qb_dev_free(ctx, w->z_dev, (size_t)w->nwin * w->nc * sizeof(cd));
+  qb_dev_free(ctx, w->w_dev, 2 * (size_t)w->nwin * w->nc * sizeof(cd));
   qb_dev_free(ctx, w->T64, (size_t)w->nq * NN * sizeof(cd));
   qb_dev_free(ctx, w->LU32, (size_t)w->nq * NN * sizeof(cf));
   qb_dev_free(ctx, w->ipiv, (size_t)w->nq * w->N * sizeof(int));
@@ -931,18 +933,23 @@ extern "C" void qb_window_destroy(qb_window* w) {
     }                  \
   } while (0)
 
-extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
-                                double k0_im, double R, int32_t nq, const qb_cheb_params* cheb,
-                                int32_t precision, qb_window** out, qb_stats* stats) {
+// nwin contours k0[c] + R[c] e^{i 2 pi j / nq} on one geometry, batched.
+static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, const double* k0,
+                      const double* Rs, int32_t nq, const qb_cheb_params* cheb,
+                      int32_t precision, qb_window** out, qb_stats* stats) {
   if (!ctx) return QB_E_ARG;
-  if (!g || !out) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (!g || !out || !k0 || !Rs) return qb_fail(ctx, QB_E_ARG, "null argument");
   *out = nullptr;
+  if (nwin < 1 || nwin > 4096) return qb_fail(ctx, QB_E_ARG, "nwin must be in [1, 4096]");
+  for (int c = 0; c < nwin; ++c)
+    if (!(Rs[c] > 0.0)) return qb_fail(ctx, QB_E_ARG, "contour radius must be > 0");
+  if (nq >= 1 && (int64_t)nq * nwin * qb_nirr(g) > 4096)
+    return qb_fail(ctx, QB_E_ARG, "nq x nwin x irreps must be <= 4096");
   if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
   if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   if (nq < 1 || nq > 4096) return qb_fail(ctx, QB_E_ARG, "nq must be in [1, 4096]");
-  if (!(R > 0.0)) return qb_fail(ctx, QB_E_ARG, "contour radius must be > 0");
   if (precision != QB_FP64 && precision != QB_MIXED)
     return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64 or QB_MIXED");
   int rc = check_cheb(ctx, cheb);
@@ -953,38 +960,45 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   w->g = g;
   w->op = op;
   w->nc = nq;
+  w->nwin = nwin;
   w->nirr = qb_nirr(g);
-  w->nq = nq * w->nirr;
+  w->nq = nq * nwin * w->nirr;
   w->precision = precision;
   w->N = qb_dim(g);
-  w->k0re = k0_re;
-  w->k0im = k0_im;
-  w->R = R;
+  w->k0re = k0[0];
+  w->k0im = k0[1];
+  w->R = Rs[0];
   const int64_t N = w->N;
   const size_t NN = (size_t)N * N;
-  const int nc = w->nc;  // contour nodes; nq from here on is the batch nc x nirr
+  const int nc = w->nc;     // contour nodes per contour
+  const int nz = nc * nwin;  // wavenumbers assembled; nq from here on is the batch nz x nirr
   nq = w->nq;
   double t0 = now_ms(), t1, t2;
   DevTables tab;
-  std::vector<cd> wbuf(2 * nc);
+  std::vector<cd> wbuf(2 * nz);
   double rmin, rmax;
   op_range(g, op, &rmin, &rmax);
-  // contour nodes and weights (beyn.jl:258-261)
-  w->zj.resize(nc);
-  w->wj.resize(nc);
-  w->wzj.resize(nc);
-  for (int j = 0; j < nc; ++j) {
-    long double th = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)nc;
-    double c = (double)cosl(th), s = (double)sinl(th);
-    cd e = mk(c, s);
-    w->zj[j] = mk(k0_re + R * c, k0_im + R * s);
-    w->wj[j] = (R / (double)nc) * e;
-    w->wzj[j] = w->wj[j] * w->zj[j];
-    wbuf[j] = w->wj[j];
-    wbuf[nc + j] = w->wzj[j];
+  // contour nodes and weights of every contour (beyn.jl:258-261)
+  w->zj.resize(nz);
+  w->wj.resize(nz);
+  w->wzj.resize(nz);
+  for (int cw = 0; cw < nwin; ++cw) {
+    const double R = Rs[cw];
+    for (int j = 0; j < nc; ++j) {
+      long double th =
+          2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)nc;
+      double c = (double)cosl(th), s = (double)sinl(th);
+      cd e = mk(c, s);
+      const int q = cw * nc + j;
+      w->zj[q] = mk(k0[2 * cw] + R * c, k0[2 * cw + 1] + R * s);
+      w->wj[q] = (R / (double)nc) * e;
+      w->wzj[q] = w->wj[q] * w->zj[q];
+      wbuf[q] = w->wj[q];
+      wbuf[nz + q] = w->wzj[q];
+    }
   }
-  QB_TRY(qb_dev_alloc(ctx, (void**)&w->z_dev, nc * sizeof(cd)));
-  QB_TRY(qb_dev_alloc(ctx, (void**)&w->w_dev, 2 * nc * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->z_dev, nz * sizeof(cd)));
+  QB_TRY(qb_dev_alloc(ctx, (void**)&w->w_dev, 2 * nz * sizeof(cd)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->T64, (size_t)nq * NN * sizeof(cd)));
   if (precision == QB_MIXED) {
     QB_TRY(qb_dev_alloc(ctx, (void**)&w->LU32, (size_t)nq * NN * sizeof(cf)));
@@ -994,17 +1008,17 @@ extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, doubl
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->perm, (size_t)nq * N * sizeof(int)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->info, nq * sizeof(int)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&w->scratch_d, 3 * nq * sizeof(double)));
-  if (hipMemcpyAsync(w->z_dev, w->zj.data(), nc * sizeof(cd), hipMemcpyHostToDevice,
+  if (hipMemcpyAsync(w->z_dev, w->zj.data(), nz * sizeof(cd), hipMemcpyHostToDevice,
                      ctx->stream) != hipSuccess ||
-      hipMemcpyAsync(w->w_dev, wbuf.data(), 2 * nc * sizeof(cd), hipMemcpyHostToDevice,
+      hipMemcpyAsync(w->w_dev, wbuf.data(), 2 * nz * sizeof(cd), hipMemcpyHostToDevice,
                      ctx->stream) != hipSuccess ||
       hipMemsetAsync(w->info, 0, nq * sizeof(int), ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "window upload failed");
     goto fail;
   }
   // tables + assembly (construct_boundary_matrices!, beyn.jl:264)
-  QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nc, rmin, rmax, *cheb, true, &tab));
-  QB_TRY(qb_assemble_dev(ctx, g, op, w->z_dev, nc, tab, w->T64,
+  QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nz, rmin, rmax, *cheb, true, &tab));
+  QB_TRY(qb_assemble_dev(ctx, g, op, w->z_dev, nz, tab, w->T64,
                          precision == QB_MIXED ? w->LU32 : nullptr));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "assembly failed");
@@ -1043,6 +1057,22 @@ fail:
   delete w;
   return rc;
 }
+
+extern "C" int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
+                                double k0_im, double R, int32_t nq, const qb_cheb_params* cheb,
+                                int32_t precision, qb_window** out, qb_stats* stats) {
+  const double k0[2] = {k0_re, k0_im};
+  return win_create(ctx, g, op, 1, k0, &R, nq, cheb, precision, out, stats);
+}
+
+extern "C" int qb_window_create_multi(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin,
+                                      const double* k0, const double* R, int32_t nq,
+                                      const qb_cheb_params* cheb, int32_t precision,
+                                      qb_window** out, qb_stats* stats) {
+  return win_create(ctx, g, op, nwin, k0, R, nq, cheb, precision, out, stats);
+}
+
+extern "C" int32_t qb_window_blocks(const qb_window* w) { return w ? w->nirr * w->nwin : 0; }
 
 // FP64 refactorisation of the nodes listed in w->fb_nodes that are not yet
 // factored (mixed-precision fallback when refinement does not converge).
@@ -1109,7 +1139,8 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
   const int nq = w->nq;
   const size_t NN = (size_t)N * N;
   const int64_t nv = N * (int64_t)r;
-  const int nc = w->nc, nirr = w->nirr;  // nq = nc x nirr matrices
+  const int nc = w->nc, nwin = w->nwin, nz = nc * nwin;
+  const int nblk = w->nirr * nwin;  // moment blocks; nq = nc x nblk matrices
   double t0 = now_ms();
   cd *Vd = nullptr, *X = nullptr, *Rr = nullptr, *A0d = nullptr, *A1d = nullptr;
   cf* D32 = nullptr;
@@ -1120,11 +1151,11 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
   std::vector<double> h3(3 * nq);
   QB_TRY(qb_dev_alloc(ctx, (void**)&Vd, nv * sizeof(cd)));
   QB_TRY(qb_dev_alloc(ctx, (void**)&X, (size_t)nq * nv * sizeof(cd)));
-  // moments per irrep e: device output [A0_e | A1_e] blocks (2 nv each); host
-  // outputs A0 and A1 hold the nirr blocks one after the other
+  // moments per block b: device output [A0_b | A1_b] blocks (2 nv each); host
+  // outputs A0 and A1 hold the nblk blocks one after the other
   if (!dev_out) {
-    QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, (size_t)nirr * nv * sizeof(cd)));
-    QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, (size_t)nirr * nv * sizeof(cd)));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&A0d, (size_t)nblk * nv * sizeof(cd)));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&A1d, (size_t)nblk * nv * sizeof(cd)));
   }
   if (hipMemcpyAsync(Vd, V, nv * sizeof(cd), hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "V upload failed");
@@ -1277,15 +1308,17 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       if (be[j] <= refine_tol) resid_max = std::max(resid_max, be[j]);
     w->st.nodes_fp64_fallback = (int)w->fb_nodes.size();
   }
-  for (int e = 0; e < nirr; ++e) {
-    cd* a0 = dev_out ? dev_out + (size_t)2 * e * nv : A0d + (size_t)e * nv;
-    cd* a1 = dev_out ? dev_out + (size_t)2 * e * nv + nv : A1d + (size_t)e * nv;
-    QB_TRY(qb_moments(ctx, X + (size_t)e * nc * nv, nv, nc, w->w_dev, w->w_dev + nc, a0, a1));
+  for (int b = 0; b < nblk; ++b) {
+    const int cw = b % nwin;  // contour of block b (blocks are irrep-major)
+    cd* a0 = dev_out ? dev_out + (size_t)2 * b * nv : A0d + (size_t)b * nv;
+    cd* a1 = dev_out ? dev_out + (size_t)2 * b * nv + nv : A1d + (size_t)b * nv;
+    QB_TRY(qb_moments(ctx, X + (size_t)b * nc * nv, nv, nc, w->w_dev + cw * nc,
+                      w->w_dev + nz + cw * nc, a0, a1));
   }
   if (!dev_out &&
-      (hipMemcpyAsync(A0, A0d, (size_t)nirr * nv * sizeof(cd), hipMemcpyDeviceToHost,
+      (hipMemcpyAsync(A0, A0d, (size_t)nblk * nv * sizeof(cd), hipMemcpyDeviceToHost,
                       ctx->stream) != hipSuccess ||
-       hipMemcpyAsync(A1, A1d, (size_t)nirr * nv * sizeof(cd), hipMemcpyDeviceToHost,
+       hipMemcpyAsync(A1, A1d, (size_t)nblk * nv * sizeof(cd), hipMemcpyDeviceToHost,
                       ctx->stream) != hipSuccess)) {
     rc = qb_fail(ctx, QB_E_HIP, "moment download failed");
     goto fail;
@@ -1308,8 +1341,8 @@ fail:
   qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, D32, (size_t)nq * nv * sizeof(cf));
   if (!dev_out) {
-    qb_dev_free(ctx, A0d, (size_t)nirr * nv * sizeof(cd));
-    qb_dev_free(ctx, A1d, (size_t)nirr * nv * sizeof(cd));
+    qb_dev_free(ctx, A0d, (size_t)nblk * nv * sizeof(cd));
+    qb_dev_free(ctx, A1d, (size_t)nblk * nv * sizeof(cd));
   }
   return rc;
 }
@@ -1330,7 +1363,7 @@ extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, do
   if (r < 1 || r > w->N) return qb_fail(ctx, QB_E_ARG, "r must be in [1, N]");
   (void)hipSetDevice(ctx->device);
   const int64_t N = w->N;
-  const int nirr = w->nirr;
+  const int nirr = w->nirr * w->nwin;  // moment blocks
   const size_t need = (size_t)2 * N * r * nirr;
   if (w->qr_cap != need) {
     qb_dev_free(ctx, w->qr, w->qr_cap * sizeof(cd));
@@ -1352,7 +1385,7 @@ extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, do
   w->qr_r = 0;
   int rc = win_moments(w, r, V, nullptr, nullptr, refine_tol, nullptr, w->qr);
   if (rc) return rc;
-  // one QR per irrep on its [A0_e | A1_e] block
+  // one QR per block on its [A0_b | A1_b]
   for (int ie = 0; ie < nirr; ++ie) {
     cd* blk = w->qr + (size_t)2 * ie * N * r;
     rc = qb_house_qr(ctx, blk, N, r, 2 * r, w->qr_tau + (size_t)ie * r);
@@ -1395,7 +1428,8 @@ extern "C" int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* 
   if (!U || (k > 0 && !Ur) || k < 0) return qb_fail(ctx, QB_E_ARG, "null argument");
   if (w->qr_r == 0) return qb_fail(ctx, QB_E_STATE, "no QR of the moments on this window");
   if (k > w->qr_r) return qb_fail(ctx, QB_E_ARG, "k must be <= r");
-  if (irrep < 0 || irrep >= w->nirr) return qb_fail(ctx, QB_E_ARG, "irrep out of range");
+  if (irrep < 0 || irrep >= w->nirr * w->nwin)
+    return qb_fail(ctx, QB_E_ARG, "block index out of range");
   if (k == 0) return QB_OK;
   (void)hipSetDevice(ctx->device);
   const int64_t N = w->N;

@@ -27,6 +27,7 @@ EXPORTS = (
     "qb_ctx_mem_info", "qb_ctx_set_graphs", "qb_window_moments_qr", "qb_window_basis",
     "qb_geom_set_orbits", "qb_geom_dim", "qb_host_alloc", "qb_host_free",
     "qb_geom_set_orbits_multi", "qb_geom_irreps", "qb_window_basis_irrep",
+    "qb_window_create_multi", "qb_window_blocks",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
 
@@ -95,6 +96,9 @@ def lib():
                                         P]),
         "qb_window_create": (C.c_int, [P, P, I32, D, D, D, I32, C.POINTER(ChebParams), I32, PP,
                                        C.POINTER(Stats)]),
+        "qb_window_create_multi": (C.c_int, [P, P, I32, I32, P, P, I32, C.POINTER(ChebParams),
+                                             I32, PP, C.POINTER(Stats)]),
+        "qb_window_blocks": (I32, [P]),
         "qb_window_destroy": (None, [P]),
         "qb_window_footprint": (I64, [I64, I32, I32, I32]),
         "qb_window_moments": (C.c_int, [P, I32, P, P, P, D, C.POINTER(Stats)]),

@@ -52,9 +52,14 @@ def probe_block(rng: np.random.Generator, N: int, r: int, pinned: bool = False) 
 
 class Window:
     """Contour nodes of one disk with T(z_j) assembled and factored on the
-    device (qb_window).  `moments(V)` may be called repeatedly."""
+    device (qb_window).  `moments(V)` may be called repeatedly.
 
-    def __init__(self, solver, pts, k0: complex, R: float, nq: int, *, n_panels_h=15000, M_h=5,
+    k0 and R may also be sequences: the disks k0[c] + R[c] e^{i theta} on the
+    same geometry then form one window whose matrices are assembled, factored
+    and solved as one batch (qb_window_create_multi); moments come back as one
+    block per (irrep, disk), block e * nwin + c."""
+
+    def __init__(self, solver, pts, k0, R, nq: int, *, n_panels_h=15000, M_h=5,
                  n_panels_j=3000, M_j=5, precision="fp64", geom: Optional[Geometry] = None,
                  ctx: Optional[Context] = None):
         self.ctx = ctx or (geom.ctx if geom is not None else default_context())
@@ -62,14 +67,27 @@ class Window:
         self.N = self.geom.dim  # reduced dimension with a solver symmetry
         self.nirr = getattr(self.geom, "nirr", 1)  # irreps factored together
         self.nq = nq
-        self.k0, self.R = complex(k0), float(R)
+        multi = np.ndim(k0) > 0
+        k0s = np.atleast_1d(np.asarray(k0, dtype=np.complex128))
+        Rs = np.broadcast_to(np.asarray(R, dtype=np.float64), k0s.shape).copy()
+        self.nwin = len(k0s)
+        self.nblocks = self.nirr * self.nwin
+        self.k0, self.R = (k0s, Rs) if multi else (complex(k0s[0]), float(Rs[0]))
         self.precision = precision
         cp = _lib.ChebParams(int(n_panels_h), int(M_h), int(n_panels_j), int(M_j))
         st = _lib.Stats()
         h = C.c_void_p()
-        rc = _lib.lib().qb_window_create(self.ctx.ptr, self.geom.ptr, solver.op, self.k0.real,
-                                         self.k0.imag, self.R, int(nq), C.byref(cp),
-                                         _PREC[precision], C.byref(h), C.byref(st))
+        if multi:
+            kk = np.ascontiguousarray(k0s)
+            rc = _lib.lib().qb_window_create_multi(self.ctx.ptr, self.geom.ptr, solver.op,
+                                                   self.nwin, kk.ctypes.data, Rs.ctypes.data,
+                                                   int(nq), C.byref(cp), _PREC[precision],
+                                                   C.byref(h), C.byref(st))
+        else:
+            rc = _lib.lib().qb_window_create(self.ctx.ptr, self.geom.ptr, solver.op,
+                                             self.k0.real, self.k0.imag, self.R, int(nq),
+                                             C.byref(cp), _PREC[precision], C.byref(h),
+                                             C.byref(st))
         _lib.check(self.ctx.ptr, rc)
         self.ptr = h
         self.stats = st.as_dict()
@@ -78,16 +96,16 @@ class Window:
         V = np.asfortranarray(V, dtype=np.complex128)
         assert V.shape[0] == self.N
         r = V.shape[1]
-        raw0 = np.empty((self.nirr, r, self.N), dtype=np.complex128)
-        raw1 = np.empty((self.nirr, r, self.N), dtype=np.complex128)
+        raw0 = np.empty((self.nblocks, r, self.N), dtype=np.complex128)
+        raw1 = np.empty((self.nblocks, r, self.N), dtype=np.complex128)
         st = _lib.Stats()
         rc = _lib.lib().qb_window_moments(self.ptr, int(r), V.ctypes.data, raw0.ctypes.data,
                                           raw1.ctypes.data, float(refine_tol), C.byref(st))
         _lib.check(self.ctx.ptr, rc)
         self.stats = st.as_dict()
-        if self.nirr == 1:
+        if self.nblocks == 1:
             return raw0[0].T, raw1[0].T
-        # one (N x r) block per irrep
+        # one (N x r) block per irrep and disk
         return raw0.transpose(0, 2, 1), raw1.transpose(0, 2, 1)
 
     def moments_qr(self, V: np.ndarray, refine_tol: float = 1e-13):
@@ -96,21 +114,21 @@ class Window:
         V = np.asfortranarray(V, dtype=np.complex128)
         assert V.shape[0] == self.N
         r = V.shape[1]
-        R = np.empty((self.nirr, r, r), dtype=np.complex128)
-        Cm = np.empty((self.nirr, r, r), dtype=np.complex128)
+        R = np.empty((self.nblocks, r, r), dtype=np.complex128)
+        Cm = np.empty((self.nblocks, r, r), dtype=np.complex128)
         st = _lib.Stats()
         rc = _lib.lib().qb_window_moments_qr(self.ptr, int(r), V.ctypes.data, R.ctypes.data,
                                              Cm.ctypes.data, float(refine_tol), C.byref(st))
         _lib.check(self.ctx.ptr, rc)
         self.stats = st.as_dict()
-        if self.nirr == 1:
+        if self.nblocks == 1:
             return R[0].T, Cm[0].T
-        # one (R, C) pair per irrep, stacked: R[e], C[e]
+        # one (R, C) pair per block, stacked: R[b], C[b]
         return R.transpose(0, 2, 1), Cm.transpose(0, 2, 1)
 
     def basis(self, Ur: np.ndarray, irrep: int = 0) -> np.ndarray:
         """U = Q [Ur; 0] (N x k) from the last moments_qr (qb_window_basis);
-        `irrep` selects the irrep of a multi-irrep window."""
+        `irrep` selects the moment block (irrep e * nwin + disk c)."""
         Ur = np.asfortranarray(Ur, dtype=np.complex128)
         k = Ur.shape[1]
         U = np.empty((self.N, k), dtype=np.complex128, order="F")

@@ -15,6 +15,14 @@ at start-up less 12 GiB.  The host part of
 each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
 windows' device work.  Latency is measured from arrival to result.
 
+Continuous batching: when a slot frees up, the pending requests that share
+the admitted one's geometry and settings (shape, node counts, symmetry,
+precision, nq, tolerances) join it, as many as fit in the budget (up to
+`max_group`), and run as one window whose disks are assembled, factored and
+solved as one batch (qb_window_create_multi) with one probe block; each
+request still gets the eigenvalues of its own disk.  Windows of different N
+run side by side in the slots.
+
 A request with symmetry=True is served as the independent symmetry-reduced
 windows of its shape's symmetry group (both reflections of the ellipse,
 stadium and disk, the C_5 rotations of the star; _all_k_symm_CFIE_chebyshev!,
@@ -113,13 +121,15 @@ def node_counts(billiard: G.Billiard, k: float, ppw: float, mult: int = 2) -> Li
 
 class Server:
     def __init__(self, slots: int = 2, hbm_budget: Optional[int] = None, device: int = 0,
-                 host_threads: int = 4):
+                 host_threads: int = 4, group: bool = True, max_group: int = 16):
         self.slots = [Context(device) for _ in range(slots)]
         for c in self.slots:
             # windows of different sizes come and go: no per-slot buffer cache,
             # so the footprint accounting is the memory actually held
             c.set_cache_limit(0)
         self.free_slots = list(range(slots))
+        self.group = bool(group)
+        self.max_group = int(max_group)
         if hbm_budget is None:
             free, _ = self.slots[0].mem_info()
             hbm_budget = max(free - 12 * 2 ** 30, 1 << 30)
@@ -129,7 +139,8 @@ class Server:
         self.pending: List = []          # (req, future, t_arrival)
         self.geoms: Dict = {}            # (slot, shape, N tuple) -> Geometry
         self.pts_cache: Dict = {}        # (shape, N tuple) -> pts
-        self.cheb_cache: Dict = {}       # (shape, N tuple, k0 bucket, tol) -> cheb params
+        self.cheb_cache: Dict = {}       # (shape, N tuple, disks, nq, tol) -> cheb params
+        self.prep_memo: Dict = {}        # id(job) -> (job, _prepare result)
         self.host_pool = ThreadPoolExecutor(max_workers=host_threads)
         self.workers: List[threading.Thread] = []
         self.closed = False
@@ -202,6 +213,14 @@ class Server:
 
     # -- internals -----------------------------------------------------------
     def _prepare(self, job):
+        memo = self.prep_memo.get(id(job))
+        if memo is not None and memo[0] is job:
+            return memo[1]
+        prep = self._prepare_uncached(job)
+        self.prep_memo[id(job)] = (job, prep)
+        return prep
+
+    def _prepare_uncached(self, job):
         req, sym = job
         b = SHAPES[req.shape]()
         mult = node_divisor(sym)
@@ -237,6 +256,7 @@ class Server:
                             fp = prep[4]
                             if fp > self.budget:
                                 self.pending.pop(idx)
+                                self.prep_memo.pop(id(job), None)
                                 fut.set_exception(MemoryError(
                                     f"request {job[0].rid}: footprint {fp} B exceeds the budget {self.budget} B"))
                                 pick = "rejected"
@@ -248,18 +268,53 @@ class Server:
                         continue
                     if pick is not None:
                         idx, prep = pick
-                        req, fut, ta = self.pending.pop(idx)
+                        entries = [self.pending.pop(idx)]
+                        if self.group:
+                            prep = self._grow_group(entries, prep)
                         slot = self.free_slots.pop(0)
                         self.live_bytes += prep[4]
                         break
                     self.lock.wait(timeout=0.05)
-            t = threading.Thread(target=self._run, args=(slot, req, fut, ta, prep), daemon=True)
+            t = threading.Thread(target=self._run, args=(slot, entries, prep), daemon=True)
             self.workers.append(t)
             t.start()
 
-    def _run(self, slot, job, fut, t_arrival, prep):
+    @staticmethod
+    def _group_key(job, prep):
         req, sym = job
+        return (req.shape, prep[1], sym, prep[5], req.nq, req.tol, req.svd_tol, req.ppw)
+
+    def _grow_group(self, entries, prep):
+        """Pending requests with the admitted one's geometry and settings join
+        it while the batch fits in the budget (called with the lock held).
+        Returns the group's prep: the largest probe count and the footprint
+        of the batched window."""
         b, Ns, N, r, fp, prec = prep
+        key = self._group_key(entries[0][0], prep)
+        nq = entries[0][0][0].nq
+        i = 0
+        while i < len(self.pending) and len(entries) < self.max_group:
+            job = self.pending[i][0]
+            p2 = self._prepare(job)
+            if self._group_key(job, p2) == key:
+                r2 = max(r, p2[3])
+                fp2 = footprint(N, nq * (len(entries) + 1), prec, r2)
+                if self.live_bytes + fp2 <= self.budget and nq * (len(entries) + 1) <= 4096:
+                    entries.append(self.pending.pop(i))
+                    r, fp = r2, fp2
+                    continue
+            i += 1
+        return b, Ns, N, r, fp, prec
+
+    def _run(self, slot, entries, prep):
+        """One window over the disks of `entries` (requests sharing a geometry)."""
+        jobs = [e[0] for e in entries]
+        with self.lock:
+            for j in jobs:
+                self.prep_memo.pop(id(j), None)
+        req, sym = jobs[0]
+        b, Ns, N, r, fp, prec = prep
+        reqs = [j[0] for j in jobs]
         t_admit = time.perf_counter()
         try:
             ctx = self.slots[slot]
@@ -269,39 +324,53 @@ class Server:
             if gkey not in self.geoms:
                 self.geoms[gkey] = Geometry(pts, ctx, solver)
             geom = self.geoms[gkey]
-            zj = req.k0 + req.R * np.exp(2j * np.pi * np.arange(req.nq) / req.nq)
-            ck = (req.shape, Ns, round(req.k0, 3), round(req.R, 3), req.nq, req.tol)
+            th = np.exp(2j * np.pi * np.arange(req.nq) / req.nq)
+            zj = np.concatenate([q.k0 + q.R * th for q in reqs])
+            ck = (req.shape, Ns, tuple((round(q.k0, 3), round(q.R, 3)) for q in reqs), req.nq,
+                  req.tol)
             if ck not in self.cheb_cache:
                 nh, mh, nj, mj, _ = chebyshev_params(CFIE_kress(req.ppw, b), pts, zj,
                                                      tol=max(req.tol, 1e-13), geom=geom)
                 self.cheb_cache[ck] = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
             cheb = self.cheb_cache[ck]
-            win = Window(solver, pts, req.k0, req.R, req.nq, precision=prec, geom=geom, **cheb)
-            V = probe_block(np.random.default_rng(req.seed), N, r)
-            Rm, Cm = win.moments_qr(V, max(req.tol, 1e-13))
+            # the probe block is drawn on the host pool while the window is
+            # assembled and factored (the C calls release the interpreter)
+            vf = self.host_pool.submit(probe_block, np.random.default_rng(req.seed), N, r)
+            if len(reqs) == 1:
+                win = Window(solver, pts, req.k0, req.R, req.nq, precision=prec, geom=geom,
+                             **cheb)
+            else:
+                win = Window(solver, pts, [q.k0 for q in reqs], [q.R for q in reqs], req.nq,
+                             precision=prec, geom=geom, **cheb)
+            Rm, Cm = win.moments_qr(vf.result(), max(req.tol, 1e-13))
+            if len(reqs) == 1:
+                Rm, Cm = Rm[None], Cm[None]
             st = dict(win.stats)
             st["precision_used"] = prec
+            st["group"] = len(reqs)
             win.close()
         except Exception as e:  # release and report
             self._release(slot, fp)
-            fut.set_exception(e)
+            for _, fut, _ in entries:
+                fut.set_exception(e)
             return
         t_dev = time.perf_counter() - t_admit
         self._release(slot, fp)
 
-        def host():
+        def host(c, q, t_arrival):
             th = time.perf_counter()
-            B, Ur, S, rk = B_from_qr(Rm, Cm, req.svd_tol)
+            B, Ur, S, rk = B_from_qr(Rm[c], Cm[c], q.svd_tol)
             lam = np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128)
             t_done = time.perf_counter()
-            inside = np.sort_complex(lam[np.abs(lam - req.k0) <= req.R])
+            inside = np.sort_complex(lam[np.abs(lam - q.k0) <= q.R])
             st["symmetry"] = None if sym is None else repr(sym)
-            return WindowResult(req.rid, lam, inside, rk, r, N, t_arrival, t_admit, t_done, t_dev,
+            return WindowResult(q.rid, lam, inside, rk, r, N, t_arrival, t_admit, t_done, t_dev,
                                 t_done - th, st)
 
-        hf = self.host_pool.submit(host)
-        hf.add_done_callback(lambda f: fut.set_exception(f.exception()) if f.exception()
-                             else fut.set_result(f.result()))
+        for c, ((q, _), fut, ta) in enumerate(entries):
+            hf = self.host_pool.submit(host, c, q, ta)
+            hf.add_done_callback(lambda f, fut=fut: fut.set_exception(f.exception())
+                                 if f.exception() else fut.set_result(f.result()))
 
     def _release(self, slot, fp):
         with self.lock:
@@ -338,6 +407,32 @@ def summarize(results: List, t_span: Optional[float] = None) -> dict:
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "span_s": t_span}
+
+
+def hole_node_counts(k: float, mult: int = 256) -> List[int]:
+    """Node counts of the uneven-window workload on the unit disk with a hole of
+    radius 0.3: N = 10 k L / 2 pi (L = 2.6 pi, the two circumferences) rounded
+    up to a multiple of `mult`, split between the outer circle and the hole in
+    the ratio of their lengths (10 : 3), both even."""
+    n = int(math.ceil(10.0 * k * 2.6 * math.pi / (2 * math.pi) / mult)) * mult
+    n_out = 2 * int(round(n * 10.0 / 13.0 / 2))
+    return [n_out, n - n_out]
+
+
+def uneven_windows(n: int, seed: int = 0, kmin: float = 100.0, kmax: float = 800.0,
+                   R: float = 0.5, nq: int = 32, precision: str = "mixed",
+                   tol: float = 1e-12) -> List[WindowRequest]:
+    """The uneven batched-window workload: n windows on the unit disk with a
+    hole of radius 0.3 centred at (0.35, 0), k0 seeded uniform in
+    [kmin, kmax], N from hole_node_counts(k0 + R), all arriving at once."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        k0 = float(rng.uniform(kmin, kmax))
+        out.append(WindowRequest(rid=i, shape="circle_hole", k0=k0, R=R, tol=tol, nq=nq,
+                                 precision=precision, seed=2000 + i, arrival=0.0,
+                                 N=hole_node_counts(k0 + R)))
+    return out
 
 
 def poisson_stream(n: int, rate: float, seed: int = 0, shapes=("ellipse", "stadium", "star"),

@@ -125,3 +125,35 @@ def test_symmetric_request_equals_the_unreduced_one():
     ex = O.jm_zeros_in_disk(20.0, 0.5)
     got = np.sort(red[3].inside.real)
     assert len(got) == len(ex) and np.max(np.abs(got - ex) / ex) < 1e-10
+
+
+def test_requests_sharing_a_geometry_are_batched():
+    """Requests with the same shape, node counts and settings that wait for a
+    slot run as one batched window (qb_window_create_multi); each returns the
+    eigenvalues of its own disk, as when it runs alone, and on the disk the
+    zeros of J_m."""
+    from qbhip.serve import Server, WindowRequest
+    reqs = [WindowRequest(i, "circle", k0, 0.5, tol=1e-12, nq=24, N=[256], seed=31 + i)
+            for i, k0 in enumerate((19.0, 20.0, 21.3))]
+    reqs += [WindowRequest(3 + i, "circle_hole", k0, 0.35, tol=1e-12, nq=24, N=[200, 60],
+                           seed=41 + i) for i, k0 in enumerate((16.0, 17.1))]
+    srv = Server(slots=1, group=False)
+    alone = {q.rid: srv.submit(q).result() for q in reqs}
+    srv.close()
+    srv = Server(slots=1)
+    # occupy the only slot so that the requests below wait together
+    blocker = srv.submit(WindowRequest(-1, "ellipse", 30.0, 0.4, nq=32, ppw=12))
+    futs = [srv.submit(q) for q in reqs]
+    grouped = {f.result().rid: f.result() for f in futs}
+    blocker.result()
+    srv.close()
+    for rid, res in grouped.items():
+        assert res.stats["group"] == (3 if rid < 3 else 2), rid
+        assert alone[rid].stats["group"] == 1
+        a, m = alone[rid].inside, res.inside
+        assert len(a) == len(m) and len(a) > 0, rid
+        assert np.max(np.abs(a - m) / np.abs(a)) <= 1e-10, rid
+    for rid, k0 in enumerate((19.0, 20.0, 21.3)):
+        ex = O.jm_zeros_in_disk(k0, 0.5)
+        got = np.sort(grouped[rid].inside.real)
+        assert len(got) == len(ex) and np.max(np.abs(got - ex) / ex) < 1e-10, rid

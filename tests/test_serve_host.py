@@ -24,3 +24,25 @@ def test_node_counts_follow_points_per_wavelength():
     assert n == [int(math.ceil(1000.0)) + (-int(math.ceil(1000.0))) % 2,
                  int(math.ceil(300.0)) + (-int(math.ceil(300.0))) % 2]
     assert all(x % 2 == 0 for x in n)
+
+
+def test_uneven_workload_node_classes():
+    """The uneven batched-window workload: N = 10 k L / 2 pi with L = 2.6 pi,
+    rounded up to a multiple of 256 (1536 at k = 100.5, 10496 at k = 800.5),
+    split 10 : 3 between the outer circle and the hole, both even."""
+    from qbhip.serve import hole_node_counts, uneven_windows
+    assert sum(hole_node_counts(100.5)) == 1536 and sum(hole_node_counts(800.5)) == 10496
+    for k in np.linspace(100.5, 800.5, 57):
+        no, nh = hole_node_counts(k)
+        n = no + nh
+        assert n % 256 == 0 and no % 2 == 0 and nh % 2 == 0
+        assert 13 * k <= n < 13 * k + 256
+        assert abs(no / n - 10 / 13) < 2 / n
+    a = uneven_windows(128, seed=4)
+    b = uneven_windows(128, seed=4)
+    assert [(r.k0, r.N) for r in a] == [(r.k0, r.N) for r in b]
+    assert all(100 <= r.k0 <= 800 and r.R == 0.5 and r.nq == 32 for r in a)
+    assert all(sum(r.N) == sum(hole_node_counts(r.k0 + r.R)) for r in a)
+    # windows share node counts: they can be batched
+    classes = {tuple(r.N) for r in a}
+    assert len(classes) < len(a)
