@@ -71,12 +71,13 @@ def parse():
                     help="uneven batched windows: the disk with a hole, k0 uniform in "
                          "[100, 800], N from 10 points per wavelength rounded up to 256")
     ap.add_argument("--uneven-windows", type=int, default=128)
+    ap.add_argument("--uneven-kmax", type=float, default=800.0)
     ap.add_argument("--uneven-precision", default="fp64",
                     help="fp64 (default: faster on this workload) or mixed")
     ap.add_argument("--uneven-slots", type=int, default=2)
     ap.add_argument("--uneven-max-group", type=int, default=16)
-    ap.add_argument("--no-group", dest="group", action="store_false",
-                    help="serve every window alone (no batching of windows that share N)")
+    ap.add_argument("--group", action="store_true",
+                    help="batch windows that share N into one window (qb_window_create_multi)")
     return ap.parse_args()
 
 
@@ -103,7 +104,9 @@ def run_serve(a):
         "metric": "serving: eigenvalues/sec + p50/p99 window latency (Poisson stream of Beyn windows)",
         "value": summ["eigenvalues_per_s"], "unit": "eigenvalues/s", "n_gpus": 1,
         "p50_window_s": summ["p50_latency_s"], "p99_window_s": summ["p99_latency_s"],
-        "windows": summ["windows"], "eigenvalues": summ["eigenvalues"], "span_s": span,
+        "windows": summ["windows"], "eigenvalues": summ["eigenvalues"],
+        "candidates_in_disks": summ["candidates"],
+        "counted": "real roots inside each disk (|Im k| <= 1e-8 |k|)", "span_s": span,
         "higher_is_better": True, "dtype": "c64+c128-refine", "data": "synthetic (seeded)",
         "config": {"workload": "serving stream ellipse/stadium/star", "requests": len(reqs),
                    "rate_per_s": a.serve_rate, "k0": [50.0, a.serve_kmax], "nq": 32,
@@ -116,12 +119,14 @@ def run_uneven(a):
     nodes) on the unit disk with a hole of radius 0.3 at (0.35, 0), k0 seeded
     uniform in [100, 800], N = 10 k L / 2 pi rounded up to a multiple of 256
     (1536 ... 10496, outer : hole = 10 : 3), all submitted at once to the
-    serving loop.  Windows that share N are batched into one window (one
-    assembly, one batched LU and solve over all their contour nodes), as many
-    as fit in HBM; windows of different N run side by side in `--uneven-slots`
-    contexts.  One untimed warmup window first.  Prints one JSON line."""
+    serving loop: windows run side by side in `--uneven-slots` contexts,
+    admitted by HBM footprint; with `--group`, windows that share N are
+    batched into one window (one assembly, one batched LU and solve over all
+    their contour nodes).  Counted: the real roots inside each disk.  One
+    untimed warmup window first.  Prints one JSON line."""
     from qbhip import serve as S
-    reqs = S.uneven_windows(a.uneven_windows, seed=4, precision=a.uneven_precision)
+    reqs = S.uneven_windows(a.uneven_windows, seed=4, precision=a.uneven_precision,
+                            kmax=a.uneven_kmax)
     srv = S.Server(slots=a.uneven_slots, group=a.group, max_group=a.uneven_max_group)
     srv.submit(S.WindowRequest(-1, "circle_hole", 100.0, 0.5, nq=32,
                                precision=a.uneven_precision,
@@ -158,13 +163,15 @@ def run_uneven(a):
         "metric": "uneven batched windows: eigenvalues/sec (Beyn CFIE-Kress, N 1536-10496)",
         "value": summ["eigenvalues_per_s"], "unit": "eigenvalues/s", "n_gpus": 1,
         "windows": summ["windows"], "rejected": summ["rejected"],
-        "eigenvalues": summ["eigenvalues"], "span_s": span,
+        "eigenvalues": summ["eigenvalues"], "candidates_in_disks": summ["candidates"],
+        "counted": "real roots inside each disk (|Im k| <= 1e-8 |k|)", "span_s": span,
         "p50_window_s": summ["p50_latency_s"], "p99_window_s": summ["p99_latency_s"],
         "N_classes": len(Ns), "mean_group": float(np.mean(groups)) if groups else None,
         "higher_is_better": True,
         "dtype": "c64+c128-refine" if a.uneven_precision == "mixed" else "c128",
         "data": "synthetic (seeded)",
-        "config": {"workload": "circle with hole (0.3 at (0.35, 0)), k0 ~ U[100, 800], R=0.5",
+        "config": {"workload": "circle with hole (0.3 at (0.35, 0)), k0 ~ U[100, %g], R=0.5"
+                               % a.uneven_kmax,
                    "windows": len(reqs), "nq": 32, "N_min": min(Ns), "N_max": max(Ns),
                    "slots": a.uneven_slots, "grouped": a.group,
                    "max_group": a.uneven_max_group}}), flush=True)

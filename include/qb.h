@@ -215,10 +215,14 @@ int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision, int32_t r)
 
 /* The moment loop of construct_B_matrix (beyn.jl:278-285): X_j = T(z_j)^{-1} V,
  * A0 = sum_j w_j X_j, A1 = sum_j w_j z_j X_j, w_j = (R/nq) e^{i 2pi j/nq}.
- * V, A0, A1: host N x r complex128.  refine_tol: normwise backward error
- * ||V - T X||_max / (||T||_inf ||X||_max + ||V||_max) at which the
- * mixed-precision refinement of a node stops; a node whose error stops
- * halving is refactored and solved in FP64 (ignored for QB_FP64). */
+ * V, A0, A1: host N x r complex128.  refine_tol: backward error
+ * ||V - T_j X_j||_max / (||T_j||_inf min(||X_j||_max, x_ref) + ||V||_max) at
+ * which the mixed-precision refinement of node j stops, x_ref the median of
+ * ||X_j||_max over the nodes after the first solve (so that every node's
+ * error is small against the typical node's solution, which the moments
+ * add); a node whose error stops halving, or whose target lies below the
+ * resolution of an FP64 residual, is refactored and solved in FP64 (ignored
+ * for QB_FP64). */
 int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0,
                       double* A1, double refine_tol, qb_stats* stats);
 

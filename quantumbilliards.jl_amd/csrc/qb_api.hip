@@ -1111,17 +1111,19 @@ static int win_fp64_fallback(qb_window* w, const std::vector<int>& nodes) {
     w->fb_perm = pm;
     w->fb_cap = cap;
   }
-  for (int node : nodes) {
-    size_t slot = w->fb_nodes.size();
-    QB_HIP(ctx, hipMemcpyAsync(w->fb_lu + slot * NN, w->T64 + (size_t)node * NN, NN * sizeof(cd),
-                               hipMemcpyDeviceToDevice, ctx->stream));
-    int rc = qb_getrf_batched<cd>(ctx, w->fb_lu + slot * NN, (int)N, N, (int64_t)NN,
-                                  w->fb_ipiv + slot * N, N, nullptr, 1);
-    if (rc) return rc;
-    rc = qb_perm_from_ipiv(ctx, w->fb_ipiv + slot * N, w->fb_perm + slot * N, (int)N, 1);
-    if (rc) return rc;
-    w->fb_nodes.push_back(node);
-  }
+  if (nodes.empty()) return QB_OK;
+  // copy the nodes' FP64 matrices to consecutive slots and factor them as one batch
+  const size_t s0 = w->fb_nodes.size();
+  const int nb = (int)nodes.size();
+  for (int i = 0; i < nb; ++i)
+    QB_HIP(ctx, hipMemcpyAsync(w->fb_lu + (s0 + i) * NN, w->T64 + (size_t)nodes[i] * NN,
+                               NN * sizeof(cd), hipMemcpyDeviceToDevice, ctx->stream));
+  int rc = qb_getrf_batched<cd>(ctx, w->fb_lu + s0 * NN, (int)N, N, (int64_t)NN,
+                                w->fb_ipiv + s0 * N, N, nullptr, nb);
+  if (rc) return rc;
+  rc = qb_perm_from_ipiv(ctx, w->fb_ipiv + s0 * N, w->fb_perm + s0 * N, (int)N, nb);
+  if (rc) return rc;
+  for (int node : nodes) w->fb_nodes.push_back(node);
   return QB_OK;
 }
 
@@ -1205,8 +1207,18 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
     const bool trace = getenv("QB_REFINE_TRACE") != nullptr;
     std::vector<int> P, F;
     // residual pass over the nodes of `list` (uploaded to slot `slot` of
-    // actd) on the first `rr` columns; bn[j] = backward error per node
+    // actd) on the first `rr` columns; bn[j] = backward error per node.
+    // The moments add the nodes' solutions, so their errors must be small in
+    // absolute terms, not only relative to each node's own ||X_j||: a node
+    // near an eigenvalue has ||X_j|| far above the others, and a normwise
+    // backward error tol there leaves absolute errors ||X_j|| / ||X||_typ
+    // times those of the other nodes.  The error of node j is therefore
+    // measured against min(||X_j||, xref), xref = the median ||X||max over
+    // the nodes after the first solve.  A node whose target lies below what
+    // an FP64 residual resolves (2 sqrt(N) u64) is refactored in FP64.
     std::vector<double> bn(nq, 0.0);
+    double xref = -1.0;
+    const double floor64 = 2.0 * sqrt((double)N) * 1.1102230246251565e-16;
     auto residual_pass = [&](const std::vector<int>& list, int slot, int rr) -> int {
       const int n = (int)list.size();
       const int64_t nvr = N * (int64_t)rr;
@@ -1231,7 +1243,13 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       }
       if (hipStreamSynchronize(ctx->stream) != hipSuccess)
         return qb_fail(ctx, QB_E_HIP, "refinement norms failed");
-      for (int j : list) bn[j] = h3[2 * nq + j] / (tn[j] * h3[nq + j] + vmax);
+      if (xref < 0.0) {
+        std::vector<double> xs;
+        for (int j : list) xs.push_back(h3[nq + j]);
+        std::nth_element(xs.begin(), xs.begin() + xs.size() / 2, xs.end());
+        xref = xs[xs.size() / 2];
+      }
+      for (int j : list) bn[j] = h3[2 * nq + j] / (tn[j] * std::min(h3[nq + j], xref) + vmax);
       return QB_OK;
     };
     int it = 0;
@@ -1282,6 +1300,9 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       for (int j : act) {
         if (be[j] <= refine_tol) continue;
         if (it >= 1 && !(be[j] < 0.5 * prev[j])) stag[j] = 1;
+        // the node's own normwise target refine_tol * xref / ||X_j|| is
+        // below the FP64 residual's resolution: refinement cannot reach it
+        if (h3[nq + j] > xref && refine_tol * xref / h3[nq + j] < floor64) stag[j] = 1;
         if (!stag[j]) next.push_back(j);
       }
       act.swap(next);

@@ -15,13 +15,16 @@ at start-up less 12 GiB.  The host part of
 each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
 windows' device work.  Latency is measured from arrival to result.
 
-Continuous batching: when a slot frees up, the pending requests that share
-the admitted one's geometry and settings (shape, node counts, symmetry,
+With group=True, when a slot frees up, the pending requests that share the
+admitted one's geometry and settings (shape, node counts, symmetry,
 precision, nq, tolerances) join it, as many as fit in the budget (up to
 `max_group`), and run as one window whose disks are assembled, factored and
 solved as one batch (qb_window_create_multi) with one probe block; each
-request still gets the eigenvalues of its own disk.  Windows of different N
-run side by side in the slots.
+request still gets the eigenvalues of its own disk.  It is off by default:
+from N ~ 1500 with 32 contour nodes one window's batch already fills the
+device and the slots overlap windows, so grouping only coarsens the
+scheduling (measured slower on the uneven-window workload).  Windows of
+different N always run side by side in the slots.
 
 A request with symmetry=True is served as the independent symmetry-reduced
 windows of its shape's symmetry group (both reflections of the ellipse,
@@ -72,6 +75,20 @@ SYM_GROUPS = {
 }
 
 
+# The Dirichlet eigenvalues are real.  Candidates of B whose imaginary part
+# exceeds REAL_TOL |lam| come from directions of A0 above the rank threshold
+# that carry no eigenvalue (spurious roots, the ones the reference's
+# imag_k_check_EXPERIMENTAL, beyn.jl:623-721, residual-checks first); the
+# count of eigenvalues reported by summarize() leaves them out.
+REAL_TOL = 1e-8
+
+
+def real_roots(lam: np.ndarray) -> np.ndarray:
+    """The candidates with |Im lam| <= REAL_TOL |lam|."""
+    lam = np.asarray(lam)
+    return lam[np.abs(lam.imag) <= REAL_TOL * np.abs(lam)]
+
+
 @dataclass
 class WindowRequest:
     rid: int
@@ -104,6 +121,7 @@ class WindowResult:
     host_s: float
     stats: dict = field(default_factory=dict)
     irreps: int = 1             # reduced windows merged into this result
+    real: Optional[np.ndarray] = None  # inside ones with |Im lam| <= REAL_TOL |lam|
 
     @property
     def latency(self) -> float:
@@ -121,7 +139,7 @@ def node_counts(billiard: G.Billiard, k: float, ppw: float, mult: int = 2) -> Li
 
 class Server:
     def __init__(self, slots: int = 2, hbm_budget: Optional[int] = None, device: int = 0,
-                 host_threads: int = 4, group: bool = True, max_group: int = 16):
+                 host_threads: int = 4, group: bool = False, max_group: int = 16):
         self.slots = [Context(device) for _ in range(slots)]
         for c in self.slots:
             # windows of different sizes come and go: no per-slot buffer cache,
@@ -365,7 +383,7 @@ class Server:
             inside = np.sort_complex(lam[np.abs(lam - q.k0) <= q.R])
             st["symmetry"] = None if sym is None else repr(sym)
             return WindowResult(q.rid, lam, inside, rk, r, N, t_arrival, t_admit, t_done, t_dev,
-                                t_done - th, st)
+                                t_done - th, st, real=real_roots(inside))
 
         for c, ((q, _), fut, ta) in enumerate(entries):
             hf = self.host_pool.submit(host, c, q, ta)
@@ -390,19 +408,23 @@ def _merge(parts: List[WindowResult]) -> WindowResult:
                         sum(p.N for p in parts), p0.t_arrival, min(p.t_admit for p in parts),
                         max(p.t_done for p in parts), sum(p.device_s for p in parts),
                         sum(p.host_s for p in parts), {"irreps": [p.stats for p in parts]},
-                        irreps=len(parts))
+                        irreps=len(parts), real=np.sort_complex(np.concatenate(
+                            [p.real for p in parts])))
 
 
 def summarize(results: List, t_span: Optional[float] = None) -> dict:
-    """p50/p99 latency and eigenvalues/s over the completed windows; entries
+    """p50/p99 latency and eigenvalues/s over the completed windows (the real
+    roots inside each disk; `candidates` counts all roots inside); entries
     that are exceptions (rejected requests) are counted separately."""
     failed = [r for r in results if not isinstance(r, WindowResult)]
     results = [r for r in results if isinstance(r, WindowResult)]
     lat = np.array([r.latency for r in results])
-    n_eig = int(sum(len(r.inside) for r in results))
+    n_eig = int(sum(len(r.real) for r in results))
+    n_cand = int(sum(len(r.inside) for r in results))
     if t_span is None:
         t_span = max(r.t_done for r in results) - min(r.t_arrival for r in results)
     return {"windows": len(results), "rejected": len(failed), "eigenvalues": n_eig,
+            "candidates": n_cand,
             "eigenvalues_per_s": n_eig / t_span if t_span > 0 else None,
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),

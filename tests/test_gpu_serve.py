@@ -140,7 +140,7 @@ def test_requests_sharing_a_geometry_are_batched():
     srv = Server(slots=1, group=False)
     alone = {q.rid: srv.submit(q).result() for q in reqs}
     srv.close()
-    srv = Server(slots=1)
+    srv = Server(slots=1, group=True)
     # occupy the only slot so that the requests below wait together
     blocker = srv.submit(WindowRequest(-1, "ellipse", 30.0, 0.4, nq=32, ppw=12))
     futs = [srv.submit(q) for q in reqs]

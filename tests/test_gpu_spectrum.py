@@ -127,3 +127,34 @@ def test_compute_spectrum_beyn_circle(imag_check):
     assert len(ks) == len(ex)
     assert np.max(np.abs(ks - ex) / ex) < 1e-10
     assert len(us) == len(ks) and len(pts) == len(ks) and len(tensN) == len(ks)
+
+
+def test_real_roots_are_those_the_residual_check_keeps():
+    """On a large window (the disk with a hole, N = 6400, k0 ~ 490, 32
+    nodes) A0 has directions above the absolute rank threshold that carry no
+    eigenvalue; their roots have |Im k| far above 1e-8 |k|.  The reference's
+    residual filter (imag_k_check_EXPERIMENTAL with every candidate checked)
+    keeps exactly the real roots that the serving summaries count
+    (serve.real_roots)."""
+    q = qb()
+    from qbhip import serve as S
+    req = S.uneven_windows(128, seed=4)[9]
+    b = S.SHAPES["circle_hole"]()
+    solver = q.CFIE_kress(10.0, b)
+    pts = q.evaluate_points(solver, b, req.k0, N=req.N)
+    assert sum(req.N) == 6400
+    zj = req.k0 + req.R * np.exp(2j * np.pi * np.arange(req.nq) / req.nq)
+    nh, mh, nj, mj, _ = q.chebyshev_params(solver, pts, zj, tol=1e-12)
+    cp = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
+    r = q.probe_count(b, req.k0, req.R, req.nq, 1e-10)
+    lam, Uk, Y, _, _, _ = q.solve_vect(solver, None, pts, req.k0 + 0j, req.R, nq=req.nq, r=r,
+                                       svd_tol=1e-10, rng=np.random.default_rng(req.seed), **cp)
+    ins = np.flatnonzero(np.abs(lam - req.k0) <= req.R)
+    real = set(ins[np.abs(lam[ins].imag) <= S.REAL_TOL * np.abs(lam[ins])].tolist())
+    print(len(ins), len(real), np.sort(np.abs(lam[ins].imag))[-5:])
+    assert 0 < len(real) < len(ins)  # the window has spurious roots
+    idx, resid = q.imag_k_check_EXPERIMENTAL(solver, [lam], [Uk], [Y], [req.k0 + 0j], [req.R],
+                                             [pts], res_tol=1e-6, pad=10 ** 6, group_size=64,
+                                             **cp)
+    assert set(idx[0].tolist()) == real
+    assert np.max(resid[0]) < 1e-8
