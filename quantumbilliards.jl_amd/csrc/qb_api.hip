@@ -1592,9 +1592,11 @@ extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, in
   std::vector<Shape> shapes;
   const int n = (int)N;
   const int lu_ty = precision == QB_MIXED ? 0 : 1;
-  // LU: trailing updates (K = 256), block-row TRSM updates, panel updates
-  // over the whole factorisation (the remaining size shrinks from n to 256)
-  for (int d : {1, 2, 4, 8, 16}) shapes.push_back({n / d - (d == 1 ? 256 : 0), n / d - (d == 1 ? 256 : 0), 256, lu_ty});
+  // LU: trailing updates (K = the panel width), block-row TRSM updates, panel
+  // updates over the whole factorisation (the remaining size shrinks from n)
+  const int nb = qb_lu_nb(n);
+  for (int d : {1, 2, 4, 8, 16})
+    shapes.push_back({n / d - (d == 1 ? nb : 0), n / d - (d == 1 ? nb : 0), nb, lu_ty});
   for (int d : {2, 8}) shapes.push_back({128, n / d, 128, lu_ty});
   for (int d : {2, 8, 32})
     for (int w = 128; w >= 32; w /= 2) shapes.push_back({n / d, w, w, lu_ty});

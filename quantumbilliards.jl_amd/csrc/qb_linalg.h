@@ -46,6 +46,8 @@ int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev
 // GEMM autotuning (qb_linalg.hip): shape class, tuning of one shape (returns
 // the chosen variant or a negative error), and variant names (ty 0: complex64).
 int qb_gemm_class(int M, int N, int K);
+// Panel width of the blocked LU of n x n matrices.
+int qb_lu_nb(int n);
 template <typename T>
 int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflops);
 const char* qb_gemm_variant_name(int ty, int v);
