@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second")
     ap.add_argument("--serve-kmax", type=float, default=1000.0)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-irreps-separate", dest="serve_irreps_batched",
+                    action="store_false",
+                    help="with --serve-symmetry: one job per irrep instead of one window")
     ap.add_argument("--serve-symmetry", action="store_true",
                     help="serve each request as the symmetry-reduced windows of its shape")
     ap.add_argument("--symmetry", action="store_true",
@@ -91,7 +94,7 @@ def run_serve(a):
     from qbhip import serve as S
     reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax,
                             symmetry=a.serve_symmetry)
-    srv = S.Server(slots=a.serve_slots)
+    srv = S.Server(slots=a.serve_slots, irreps_batched=a.serve_irreps_batched)
     srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32,
                                symmetry=a.serve_symmetry)).result()
     t0 = time.perf_counter()
@@ -111,7 +114,9 @@ def run_serve(a):
         "config": {"workload": "serving stream ellipse/stadium/star", "requests": len(reqs),
                    "rate_per_s": a.serve_rate, "k0": [50.0, a.serve_kmax], "nq": 32,
                    "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns),
-                   "symmetry_reduced": a.serve_symmetry}}), flush=True)
+                   "symmetry_reduced": a.serve_symmetry,
+                   "irreps_in_one_window": a.serve_symmetry and a.serve_irreps_batched}}),
+          flush=True)
 
 
 def run_uneven(a):

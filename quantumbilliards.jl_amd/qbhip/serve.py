@@ -26,12 +26,15 @@ device and the slots overlap windows, so grouping only coarsens the
 scheduling (measured slower on the uneven-window workload).  Windows of
 different N always run side by side in the slots.
 
-A request with symmetry=True is served as the independent symmetry-reduced
-windows of its shape's symmetry group (both reflections of the ellipse,
-stadium and disk, the C_5 rotations of the star; _all_k_symm_CFIE_chebyshev!,
-cheb-CFIE:698-824): every irrep is a job of its own, admitted by its own
-(G^2 times smaller) footprint, and the request completes with the union of
-the irrep spectra when its last irrep does.
+A request with symmetry=True is served as the symmetry-reduced windows of its
+shape's symmetry group (both reflections of the ellipse, stadium and disk,
+the C_5 rotations of the star; _all_k_symm_CFIE_chebyshev!,
+cheb-CFIE:698-824), and returns the union of the irrep spectra.  By default
+(irreps_batched) the irreps are one window on a multi-irrep geometry
+(qb_geom_set_orbits_multi): one assembly pass evaluates every image term for
+all irreps, and one batched LU factors the G x nq reduced matrices.  With
+irreps_batched=False every irrep is a job of its own, admitted by its own
+footprint, and the request completes when its last irrep does.
 
 No counterpart in the reference: its driver (compute_spectrum_beyn,
 beyn.jl:778-921) loops over disks serially on the CPU.  The per-window
@@ -139,7 +142,8 @@ def node_counts(billiard: G.Billiard, k: float, ppw: float, mult: int = 2) -> Li
 
 class Server:
     def __init__(self, slots: int = 2, hbm_budget: Optional[int] = None, device: int = 0,
-                 host_threads: int = 4, group: bool = False, max_group: int = 16):
+                 host_threads: int = 4, group: bool = False, max_group: int = 16,
+                 irreps_batched: bool = True):
         self.slots = [Context(device) for _ in range(slots)]
         for c in self.slots:
             # windows of different sizes come and go: no per-slot buffer cache,
@@ -148,6 +152,7 @@ class Server:
         self.free_slots = list(range(slots))
         self.group = bool(group)
         self.max_group = int(max_group)
+        self.irreps_batched = bool(irreps_batched)
         if hbm_budget is None:
             free, _ = self.slots[0].mem_info()
             hbm_budget = max(free - 12 * 2 ** 30, 1 << 30)
@@ -169,8 +174,14 @@ class Server:
     def submit(self, req: WindowRequest, t_arrival: Optional[float] = None) -> Future:
         fut: Future = Future()
         ta = time.perf_counter() if t_arrival is None else t_arrival
-        jobs = [(req, None)] if not req.symmetry else [(req, g) for g in SYM_GROUPS[req.shape]()]
-        if req.symmetry:
+        if not req.symmetry:
+            jobs = [(req, None)]
+        elif self.irreps_batched:
+            # all irreps of the group in one window (qb_geom_set_orbits_multi)
+            jobs = [(req, tuple(SYM_GROUPS[req.shape]()))]
+        else:
+            jobs = [(req, g) for g in SYM_GROUPS[req.shape]()]
+        if req.symmetry and len(jobs) > 1:
             parts: List = [None] * len(jobs)
             left = [len(jobs)]
             guard = threading.Lock()
@@ -190,8 +201,8 @@ class Server:
             if self.closed:
                 raise RuntimeError("server closed")
             for i, (r, g) in enumerate(jobs):
-                f = fut if g is None else Future()
-                if g is not None:
+                f = fut if len(jobs) == 1 else Future()
+                if len(jobs) > 1:
                     f.add_done_callback(lambda ff, i=i: collect(i, ff))
                 self.pending.append(((r, g), f, ta))
             self.lock.notify_all()
@@ -240,6 +251,8 @@ class Server:
 
     def _prepare_uncached(self, job):
         req, sym = job
+        irr = sym if isinstance(sym, tuple) else (sym,)
+        sym = irr[0]
         b = SHAPES[req.shape]()
         mult = node_divisor(sym)
         Ns = (tuple(req.N) if req.N is not None
@@ -254,9 +267,10 @@ class Server:
         N = int(sum(Ns)) // order
         r = min(probe_count(bs, req.k0, req.R, req.nq, req.svd_tol), N)
         prec = req.precision
-        fp = footprint(N, req.nq, prec, r)
+        nm = req.nq * len(irr)  # matrices of the window
+        fp = footprint(N, nm, prec, r)
         if fp > self.budget and prec == "mixed":
-            fp64 = footprint(N, req.nq, "fp64", r)
+            fp64 = footprint(N, nm, "fp64", r)
             if fp64 <= self.budget:
                 prec, fp = "fp64", fp64
         return b, Ns, N, r, fp, prec
@@ -331,16 +345,20 @@ class Server:
             for j in jobs:
                 self.prep_memo.pop(id(j), None)
         req, sym = jobs[0]
+        irr = sym if isinstance(sym, tuple) else (sym,)
+        nirr = len(irr)
         b, Ns, N, r, fp, prec = prep
         reqs = [j[0] for j in jobs]
         t_admit = time.perf_counter()
         try:
             ctx = self.slots[slot]
             pts = self.pts_cache[(req.shape, Ns)]
-            solver = CFIE_kress(req.ppw, b, symmetry=sym)
+            solver = CFIE_kress(req.ppw, b, symmetry=irr[0])
             gkey = (slot, req.shape, Ns, sym)
             if gkey not in self.geoms:
-                self.geoms[gkey] = Geometry(pts, ctx, solver)
+                self.geoms[gkey] = Geometry(
+                    pts, ctx, solver,
+                    irreps=[CFIE_kress(req.ppw, b, symmetry=s_) for s_ in irr] if nirr > 1 else None)
             geom = self.geoms[gkey]
             th = np.exp(2j * np.pi * np.arange(req.nq) / req.nq)
             zj = np.concatenate([q.k0 + q.R * th for q in reqs])
@@ -361,7 +379,7 @@ class Server:
                 win = Window(solver, pts, [q.k0 for q in reqs], [q.R for q in reqs], req.nq,
                              precision=prec, geom=geom, **cheb)
             Rm, Cm = win.moments_qr(vf.result(), max(req.tol, 1e-13))
-            if len(reqs) == 1:
+            if win.nblocks == 1:
                 Rm, Cm = Rm[None], Cm[None]
             st = dict(win.stats)
             st["precision_used"] = prec
@@ -375,15 +393,23 @@ class Server:
         t_dev = time.perf_counter() - t_admit
         self._release(slot, fp)
 
+        nwin = len(reqs)
+
         def host(c, q, t_arrival):
+            # disk c: its block of every irrep (blocks are irrep-major)
             th = time.perf_counter()
-            B, Ur, S, rk = B_from_qr(Rm[c], Cm[c], q.svd_tol)
-            lam = np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128)
+            lams, rk = [], 0
+            for e in range(nirr):
+                B, Ur, S, rk_e = B_from_qr(Rm[e * nwin + c], Cm[e * nwin + c], q.svd_tol)
+                lams.append(np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128))
+                rk += rk_e
+            lam = np.concatenate(lams)
             t_done = time.perf_counter()
             inside = np.sort_complex(lam[np.abs(lam - q.k0) <= q.R])
             st["symmetry"] = None if sym is None else repr(sym)
-            return WindowResult(q.rid, lam, inside, rk, r, N, t_arrival, t_admit, t_done, t_dev,
-                                t_done - th, st, real=real_roots(inside))
+            return WindowResult(q.rid, lam, inside, rk, r * nirr, N * nirr, t_arrival, t_admit,
+                                t_done, t_dev, t_done - th, st, irreps=nirr,
+                                real=real_roots(inside))
 
         for c, ((q, _), fut, ta) in enumerate(entries):
             hf = self.host_pool.submit(host, c, q, ta)

@@ -94,17 +94,19 @@ def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
     assert s["windows"] == 1 and s["rejected"] == 1
 
 
-def test_symmetric_request_equals_the_unreduced_one():
+@pytest.mark.parametrize("batched", [True, False])
+def test_symmetric_request_equals_the_unreduced_one(batched):
     """A request served as the reduced windows of its shape's symmetry group
     (both reflections of the ellipse and stadium, the C_5 rotations of the
-    star) returns the eigenvalues the unreduced window finds in the disk; on
-    the disk they are the zeros of J_m."""
+    star), all irreps in one window or one job per irrep, returns the
+    eigenvalues the unreduced window finds in the disk; on the disk they are
+    the zeros of J_m."""
     from qbhip.serve import Server, WindowRequest
     reqs = [WindowRequest(0, "ellipse", 24.0, 0.4, tol=1e-12, nq=32, ppw=12, seed=21),
             WindowRequest(1, "star", 30.0, 0.3, tol=1e-12, nq=32, ppw=12, seed=22),
             WindowRequest(2, "stadium", 15.0, 0.3, tol=1e-12, nq=32, ppw=12, seed=23),
             WindowRequest(3, "circle", 20.0, 0.5, tol=1e-12, nq=32, ppw=13, seed=24)]
-    srv = Server(slots=2)
+    srv = Server(slots=2, irreps_batched=batched)
     full, red = {}, {}
     for q in reqs:
         # the same node counts (a multiple of 4 / 10) for both
