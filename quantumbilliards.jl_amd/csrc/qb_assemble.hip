@@ -875,6 +875,103 @@ __global__ void __launch_bounds__(256) k_assemble_red_deriv(
   }
 }
 
+// Multi-irrep twin of k_assemble_red_pair: one specials evaluation per image
+// term serves entry (a, b) and entry (b, a) of all nirr irreps at once.  The
+// nirr mirror values of a thread are written directly as rows of the mirror
+// tiles: 64 lanes store 64 scattered 16-byte values per instruction, and the
+// L2 cache merges the 64 rows of every mirror column that the 4 waves of the
+// block write into whole lines.  Diagonal tiles are assembled entry by entry.
+// Outputs indexed as in k_assemble_red (matrix e * nz + q from the base).
+template <int OP, int MH, int MJ, int MAXI>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_assemble_red_pair_multi(DevGeom g, DevSym s, const cd* __restrict__ zk,
+                          const cd* __restrict__ H, const cd* __restrict__ J, int np_h, int M_h,
+                          int np_j, int M_j, double rmin_h, double rmax, int64_t q0, int64_t nz,
+                          int T, cd* __restrict__ out64, cf* __restrict__ out32) {
+  const int64_t m = s.m;
+  const int G = s.G;
+  const int nirr = s.nirr;
+  const int64_t q = q0 + blockIdx.z;
+  int64_t p = blockIdx.x;
+  int d = 0;
+  while (p >= T - d) {
+    p -= T - d;
+    ++d;
+  }
+  const int Jt = (int)p, It = Jt + d;
+  const TabView tv = tab_view(H, J, q, np_h, M_h, np_j, M_j, rmin_h, rmax);
+  const cd k = zk[q];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t a = (int64_t)It * 64 + il;
+  if (a >= m) return;  // no barrier in this kernel
+  const bool diag_tile = (It == Jt);
+  const int64_t mm = m * m;
+  const int64_t i = s.img[a * G];
+  const double xi = g.x[i], yi = g.y[i];
+  const int ci = g.comp[i];
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int64_t b = (int64_t)Jt * 64 + jl0 + 4 * c;
+    if (b >= m) break;
+    cd acc[MAXI], accm[MAXI];
+#pragma unroll
+    for (int e = 0; e < MAXI; ++e) acc[e] = accm[e] = mk(0, 0);
+    if (diag_tile) {
+#pragma unroll 1
+      for (int l = 0; l < G; ++l) {
+        const int64_t j = s.img[b * G + l];
+        cd v;
+        if (i == j) {
+          v = diag_entry<OP>(g, j, k);
+        } else {
+          const PairGeo pg = pair_geo(g, i, j, xi, yi, ci);
+          const Spec sp = specials<OP, MH, MJ>(tv, pg.r, k, pg.same || OP == QB_DLP_KRESS);
+          v = offdiag_entry<OP>(g, i, j, pg.dx, pg.dy, pg.invr, pg.same, pg.Rij, pg.lt, k, sp);
+        }
+#pragma unroll
+        for (int e = 0; e < MAXI; ++e)
+          if (e < nirr) acc[e] = acc[e] + s.scale[((int64_t)e * m + b) * G + l] * v;
+      }
+    } else {
+      const int64_t ib = s.img[b * G];
+      const double xb = g.x[ib], yb = g.y[ib];
+      const int cb = g.comp[ib];
+#pragma unroll 1
+      for (int l = 0; l < G; ++l) {
+        const int64_t j = s.img[b * G + l];
+        const int ls = s.inv[l];
+        const int64_t jm = s.img[a * G + ls];
+        const PairGeo p1 = pair_geo(g, i, j, xi, yi, ci);
+        const PairGeo p2 = pair_geo(g, ib, jm, xb, yb, cb);  // its r equals p1.r
+        const Spec sp = specials<OP, MH, MJ>(tv, p1.r, k,
+                                             p1.same || p2.same || OP == QB_DLP_KRESS);
+        const cd v1 = offdiag_entry<OP>(g, i, j, p1.dx, p1.dy, p1.invr, p1.same, p1.Rij, p1.lt,
+                                        k, sp);
+        const cd v2 = offdiag_entry<OP>(g, ib, jm, p2.dx, p2.dy, p1.invr, p2.same, p2.Rij,
+                                        p2.lt, k, sp);
+#pragma unroll
+        for (int e = 0; e < MAXI; ++e)
+          if (e < nirr) {
+            acc[e] = acc[e] + s.scale[((int64_t)e * m + b) * G + l] * v1;
+            accm[e] = accm[e] + s.scale[((int64_t)e * m + a) * G + ls] * v2;
+          }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < MAXI; ++e) {
+      if (e >= nirr) break;
+      const int64_t idx = ((int64_t)e * nz + q) * mm + a + b * m;
+      if (out64) out64[idx] = acc[e];
+      if (out32) out32[idx] = to_cf(acc[e]);
+      if (!diag_tile) {
+        const int64_t idt = ((int64_t)e * nz + q) * mm + b + a * m;
+        if (out64) out64[idt] = accm[e];
+        if (out32) out32[idt] = to_cf(accm[e]);
+      }
+    }
+  }
+}
+
 // o64 / o32: the output base (matrix 0); the kernels index matrix e * nz + q.
 template <int OP, int MH, int MJ>
 static void launch_red(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, int64_t nzc,
@@ -896,6 +993,12 @@ static void launch_red(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, i
     hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ, 1>), grid, dim3(256), 0, ctx->stream, g->d,
                        g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0,
                        nz, o64, o32);
+  } else if (g->sym.pair && nirr <= 8) {
+    const int64_t pairs = (int64_t)Tm * (Tm + 1) / 2;
+    hipLaunchKernelGGL((k_assemble_red_pair_multi<OP, MH, MJ, 8>),
+                       dim3((unsigned)pairs, 1, (unsigned)nzc), dim3(256), 0, ctx->stream, g->d,
+                       g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0,
+                       nz, (int)Tm, o64, o32);
   } else if (nirr <= 8) {
     hipLaunchKernelGGL((k_assemble_red<OP, MH, MJ, 8>), grid, dim3(256), 0, ctx->stream, g->d,
                        g->sym, z, t.H, t.J, t.np_h, t.M_h, t.np_j, t.M_j, t.rmin_h, t.rmax, q0,
