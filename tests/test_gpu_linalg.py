@@ -49,7 +49,8 @@ def test_gemm_sub(probe, dt, tol, M, N, K, b):
 
 
 @pytest.mark.parametrize("dt,tol", [(np.complex128, 1e-12), (np.complex64, 1e-4)])
-@pytest.mark.parametrize("n,b", [(8, 1), (40, 2), (160, 2), (300, 1), (2200, 2)])
+# 4160: panel width 512 (n >= 4096) with a last panel of 64 columns
+@pytest.mark.parametrize("n,b", [(8, 1), (40, 2), (160, 2), (300, 1), (2200, 2), (4160, 1)])
 def test_getrf_and_solve(probe, dt, tol, n, b):
     L, ctx = probe
     rng = np.random.default_rng(n)
