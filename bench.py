@@ -70,6 +70,9 @@ def parse():
                     help="contexts that take irrep windows concurrently (--no-sym-batched)")
     ap.add_argument("--no-sym-batched", dest="sym_batched", action="store_false",
                     help="one window per irrep instead of one window over all irreps")
+    ap.add_argument("--assembly", action="store_true",
+                    help="assembly throughput: DLP-Kress on the ellipse (1, 0.6), N = 4096, "
+                         "64 nodes at k0 = 400, and CFIE-Kress on the star, N = 8192, 48 nodes")
     ap.add_argument("--uneven", action="store_true",
                     help="uneven batched windows: the disk with a hole, k0 uniform in "
                          "[100, 800], N from 10 points per wavelength rounded up to 256")
@@ -117,6 +120,53 @@ def run_serve(a):
                    "symmetry_reduced": a.serve_symmetry,
                    "irreps_in_one_window": a.serve_symmetry and a.serve_irreps_batched}}),
           flush=True)
+
+
+def run_assembly(a):
+    """Assembly throughput (K1/K2), the device assembly of T(z_j) at every
+    contour node of a window, timed by the library's HIP events on its own
+    stream over `--steps` windows (FP64, so only the complex128 matrices are
+    written):
+      DLP-Kress, full ellipse (1, 0.6), N = 4096, 64 nodes 400 + 0.5 e^{i theta};
+      CFIE-Kress, star r = 1 + 0.2 cos 5t, N = 8192, 48 nodes 800 + 0.5 e^{i theta};
+    tables tuned to 1e-12 on the device (chebyshev_params).  Algorithmic
+    bytes: 16 per entry written; FP64 flops per entry: 3(M_h+M_j)+22 (DLP),
+    6(M_h+M_j)+50 (CFIE).  Prints one JSON line per operator."""
+    import qbhip as q
+    cases = [("DLP_kress", q.make_full_ellipse(1.0, 0.6), 4096, 64, 400.0),
+             ("CFIE_kress", q.make_star_polar_boundary(1.0, 0.2, 5), 8192, 48, 800.0)]
+    for name, b, N, nq, k0 in cases:
+        ctx = q.Context(0)
+        solver = (q.DLP_kress if name == "DLP_kress" else q.CFIE_kress)(10.0, b)
+        pts = q.evaluate_points(solver, b, k0, N=[N])
+        geom = q.Geometry(pts, ctx, solver)
+        zj = k0 + 0.5 * np.exp(2j * np.pi * np.arange(nq) / nq)
+        nh, mh, nj, mj, _ = q.chebyshev_params(solver, pts, zj, tol=1e-12, geom=geom)
+        cheb = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
+        q.Window(solver, pts, k0, 0.5, nq, precision="fp64", geom=geom, **cheb).close()
+        ctx.set_profiling(True)
+        for _ in range(a.steps):
+            q.Window(solver, pts, k0, 0.5, nq, precision="fp64", geom=geom, **cheb).close()
+        ks = ctx.kernel_stats()
+        ctx.set_profiling(False)
+        asm = ks["assemble"]
+        t = asm["ms"] / max(asm["launches"], 1) * 1e-3
+        ent = float(N) * N * nq
+        flops_per = (3 * (mh + mj) + 22) if name == "DLP_kress" else (6 * (mh + mj) + 50)
+        gbs = ent * 16 / t / 1e9
+        print(json.dumps({
+            "metric": f"{name} assembly throughput (GB/s written, complex128 T(z_j))",
+            "value": gbs, "unit": "GB/s", "n_gpus": 1, "steps": a.steps,
+            "ms_per_launch": t * 1e3, "entries_per_s": ent / t,
+            "fp64_tflops": ent * flops_per / t / 1e12, "higher_is_better": True,
+            "dtype": "c128", "data": "synthetic (geometry nodes)",
+            "config": {"workload": name, "N": N, "nq": nq, "k0": k0, "R": 0.5, "cheb": cheb,
+                       "cheb_tol": 1e-12},
+            "roofline": {"kernel": "k_assemble", "bound": "hbm", "achieved": gbs, "peak": 8000.0,
+                         "unit": "GB/s", "frac": gbs / 8000.0,
+                         "fp64_frac": ent * flops_per / t / 1e12 / 78.6}}), flush=True)
+        geom.close()
+        ctx.close()
 
 
 def run_uneven(a):
@@ -477,6 +527,9 @@ def main():
         return
     if a.uneven:
         run_uneven(a)
+        return
+    if a.assembly:
+        run_assembly(a)
         return
     import qbhip as q
     ctx = q.default_context()
