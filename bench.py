@@ -268,10 +268,11 @@ def run_symmetry(a):
         # image term once for all characters, one LU factors 5 x nq matrices
         ctxs = ctxs[:1]
         gmulti = q.Geometry(pts, ctxs[0], irreps=solvers)
-        ctxs[0].autotune(m_dim, a.nq * 5, r_irrep, a.sym_precision)
+        rep = ctxs[0].autotune(m_dim, a.nq * 5, r_irrep, a.sym_precision)
     else:
         for c in ctxs:
-            c.autotune(m_dim, a.nq, r_irrep, a.sym_precision)
+            rep = c.autotune(m_dim, a.nq, r_irrep, a.sym_precision)
+    print("# gemm autotune:\n" + rep, file=sys.stderr, flush=True)
 
     def batched_pass():
         t0 = time.perf_counter()
