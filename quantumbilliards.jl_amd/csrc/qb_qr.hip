@@ -88,99 +88,13 @@ __global__ void __launch_bounds__(256) k_house_apply(const cd* __restrict__ V, i
   }
 }
 
-// Columns c0 + blockIdx.x of A: a <- H_{j1-1}^H ... H_{j0}^H a (ADJ) over rows
-// j0..N-1, the reflectors j0..j1-1 of V applied in order.  The column stays in
-// registers (N - j0 <= 1024 QB_QR_S), so it is read and written once for the
-// whole group instead of once per reflector.
-#define QB_QR_S 16
-#define QB_QR_GROUP 32
-template <bool ADJ>
-__global__ void __launch_bounds__(1024) k_house_apply_group(const cd* __restrict__ V, int64_t ldv,
-                                                            cd* __restrict__ A, int64_t N,
-                                                            int64_t lda, int j0, int j1, int c0,
-                                                            const cd* __restrict__ tau) {
-  __shared__ double red_re[16], red_im[16];
-  cd* a = A + (int64_t)(c0 + blockIdx.x) * lda;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  cd x[QB_QR_S];
-#pragma unroll
-  for (int s = 0; s < QB_QR_S; ++s) {
-    const int64_t i = j0 + tid + 1024 * s;
-    x[s] = i < N ? a[i] : mk(0.0, 0.0);
-  }
-#pragma unroll 1
-  for (int j = j0; j < j1; ++j) {
-    const cd t = ADJ ? cconj(tau[j]) : tau[j];
-    if (t.re == 0.0 && t.im == 0.0) continue;  // uniform over the workgroup
-    const cd* v = V + (int64_t)j * ldv;
-    double wr = 0, wi = 0;
-#pragma unroll
-    for (int s = 0; s < QB_QR_S; ++s) {
-      const int64_t i = j0 + tid + 1024 * s;
-      if (i >= j && i < N) {
-        const cd vi = (i == j) ? mk(1.0, 0.0) : v[i];
-        wr += vi.re * x[s].re + vi.im * x[s].im;
-        wi += vi.re * x[s].im - vi.im * x[s].re;
-      }
-    }
-    for (int off = 32; off >= 1; off >>= 1) {
-      wr += __shfl_xor(wr, off);
-      wi += __shfl_xor(wi, off);
-    }
-    if (lane == 0) {
-      red_re[wv] = wr;
-      red_im[wv] = wi;
-    }
-    __syncthreads();
-    double sr = 0, si = 0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      sr += red_re[q];
-      si += red_im[q];
-    }
-    const cd tw = t * mk(sr, si);
-#pragma unroll
-    for (int s = 0; s < QB_QR_S; ++s) {
-      const int64_t i = j0 + tid + 1024 * s;
-      if (i >= j && i < N) {
-        const cd vi = (i == j) ? mk(1.0, 0.0) : v[i];
-        x[s] = x[s] - vi * tw;
-      }
-    }
-    __syncthreads();  // red_* is written again by the next reflector
-  }
-#pragma unroll
-  for (int s = 0; s < QB_QR_S; ++s) {
-    const int64_t i = j0 + tid + 1024 * s;
-    if (i < N) a[i] = x[s];
-  }
-}
-
 int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau) {
   if (r > N || ncols < r) return qb_fail(ctx, QB_E_ARG, "QR needs r <= N and ncols >= r");
-  if (N > 1024 * QB_QR_S) {  // column too long for the grouped kernel's registers
-    for (int j = 0; j < r; ++j) {
-      hipLaunchKernelGGL(k_house, dim3(1), dim3(1024), 0, ctx->stream, A, N, N, j, tau);
-      if (ncols - j - 1 > 0)
-        hipLaunchKernelGGL(k_house_apply<true>, dim3(ncols - j - 1), dim3(256), 0, ctx->stream,
-                           A, N, A, N, N, j, j + 1, tau);
-    }
-    QB_HIP(ctx, hipGetLastError());
-    return QB_OK;
-  }
-  // panels of QB_QR_GROUP reflectors: reduce the panel's own columns one
-  // reflector at a time, then apply the whole panel to the columns right of it
-  for (int j0 = 0; j0 < r; j0 += QB_QR_GROUP) {
-    const int j1 = j0 + QB_QR_GROUP < r ? j0 + QB_QR_GROUP : r;
-    for (int j = j0; j < j1; ++j) {
-      hipLaunchKernelGGL(k_house, dim3(1), dim3(1024), 0, ctx->stream, A, N, N, j, tau);
-      if (j1 - j - 1 > 0)
-        hipLaunchKernelGGL(k_house_apply<true>, dim3(j1 - j - 1), dim3(256), 0, ctx->stream, A,
-                           N, A, N, N, j, j + 1, tau);
-    }
-    if (ncols - j1 > 0)
-      hipLaunchKernelGGL(k_house_apply_group<true>, dim3(ncols - j1), dim3(1024), 0, ctx->stream,
-                         A, N, A, N, N, j0, j1, j1, tau);
+  for (int j = 0; j < r; ++j) {
+    hipLaunchKernelGGL(k_house, dim3(1), dim3(1024), 0, ctx->stream, A, N, N, j, tau);
+    if (ncols - j - 1 > 0)
+      hipLaunchKernelGGL(k_house_apply<true>, dim3(ncols - j - 1), dim3(256), 0, ctx->stream, A,
+                         N, A, N, N, j, j + 1, tau);
   }
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
