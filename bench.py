@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second")
     ap.add_argument("--serve-kmax", type=float, default=1000.0)
     ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--serve-irreps-separate", dest="serve_irreps_batched",
                     action="store_false",
                     help="with --serve-symmetry: one job per irrep instead of one window")
@@ -96,7 +97,7 @@ def run_serve(a):
     first.  Prints one JSON line with eigenvalues/s and p50/p99 latency."""
     from qbhip import serve as S
     reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax,
-                            symmetry=a.serve_symmetry)
+                            symmetry=a.serve_symmetry, precision=a.serve_precision)
     srv = S.Server(slots=a.serve_slots, irreps_batched=a.serve_irreps_batched)
     srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32,
                                symmetry=a.serve_symmetry)).result()
@@ -113,7 +114,9 @@ def run_serve(a):
         "windows": summ["windows"], "eigenvalues": summ["eigenvalues"],
         "candidates_in_disks": summ["candidates"],
         "counted": "real roots inside each disk (|Im k| <= 1e-8 |k|)", "span_s": span,
-        "higher_is_better": True, "dtype": "c64+c128-refine", "data": "synthetic (seeded)",
+        "higher_is_better": True,
+        "dtype": "c64+c128-refine" if a.serve_precision == "mixed" else "c128",
+        "data": "synthetic (seeded)",
         "config": {"workload": "serving stream ellipse/stadium/star", "requests": len(reqs),
                    "rate_per_s": a.serve_rate, "k0": [50.0, a.serve_kmax], "nq": 32,
                    "slots": a.serve_slots, "N_min": min(Ns), "N_max": max(Ns),
