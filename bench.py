@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--inflight", type=int, default=1,
+                    help="mixed windows in flight at once, each on its own device context")
     ap.add_argument("--cpu-cols", type=int, default=512)
     ap.add_argument("--serve", action="store_true",
                     help="serving stream (Poisson arrivals of mixed windows) instead")
@@ -513,7 +515,7 @@ def q_probe(N, r):
                              * np.sqrt(0.5))
 
 
-def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
+def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps, lanes=None):
     """Device windows back to back on this thread (moments reduced on the
     device to R and C = Q^H A1, qb_window_moments_qr); the host SVD of R, B
     and eig(B) of window s run on a second thread while the device works on
@@ -550,24 +552,46 @@ def pipelined(q, ctx, solver, pts, geom, a, cheb, precision, V, steps):
     hth = threading.Thread(target=host_stage)
     hth.start()
     t_begin = time.perf_counter()
-    for s in range(steps):
-        t_start = time.perf_counter()
-        win = q.Window(solver, pts, complex(a.k0), a.R, a.nq, precision=precision, geom=geom,
-                       **cheb)
-        t_c = time.perf_counter()
-        Rm, Cm = win.moments_qr(V, a.refine_tol)
-        t_m = time.perf_counter()
-        st = dict(win.stats)
-        win.close()
-        t_e = time.perf_counter()
-        st["host_phases_s"] = {"create": round(t_c - t_start, 4), "moments": round(t_m - t_c, 4),
-                               "close": round(t_e - t_m, 4),
-                               "since_prev": round(t_start - (t_prev if s else t_begin), 4)}
-        t_prev = t_e
-        qu.put((s, Rm, Cm, t_start, time.perf_counter() - t_start, st))
+    lanes = lanes or [(ctx, geom)]
+    nxt = [0]
+    lock = threading.Lock()
+
+    def device(c, g):
+        # windows taken in order from a shared counter; with several lanes
+        # (device contexts) that many windows are in flight at once
+        t_prev = t_begin
+        while True:
+            with lock:
+                s = nxt[0]
+                nxt[0] += 1
+            if s >= steps:
+                return
+            t_start = time.perf_counter()
+            win = q.Window(solver, pts, complex(a.k0), a.R, a.nq, precision=precision, geom=g,
+                           ctx=c, **cheb)
+            t_c = time.perf_counter()
+            Rm, Cm = win.moments_qr(V, a.refine_tol)
+            t_m = time.perf_counter()
+            st = dict(win.stats)
+            win.close()
+            t_e = time.perf_counter()
+            st["host_phases_s"] = {"create": round(t_c - t_start, 4),
+                                   "moments": round(t_m - t_c, 4), "close": round(t_e - t_m, 4),
+                                   "since_prev": round(t_start - t_prev, 4)}
+            t_prev = t_e
+            qu.put((s, Rm, Cm, t_start, time.perf_counter() - t_start, st))
+
+    if len(lanes) == 1:
+        device(*lanes[0])
+    else:
+        dth = [threading.Thread(target=device, args=lane) for lane in lanes]
+        for t in dth:
+            t.start()
+        for t in dth:
+            t.join()
     qu.put(None)
     hth.join()
-    return res, res[-1][3] - t_begin
+    return res, max(x[3] for x in res) - t_begin
 
 
 def main():
@@ -621,8 +645,14 @@ def main():
               f"latency {res_d[0][3] - res_d[0][2]:.3f}s rk={res_d[0][1]} stats={res_d[0][6]}",
               file=sys.stderr, flush=True)
     V = q.probe_block(np.random.default_rng(1), a.N, r_used, pinned=True)
+    lanes = [(ctx, geom)]
+    for _ in range(max(1, a.inflight) - 1):
+        c2 = q.Context(0)
+        c2.autotune(a.N, a.nq, r_used, "mixed")
+        lanes.append((c2, q.Geometry(pts, c2)))
     ctx.set_profiling(True)
-    res, t_span = pipelined(q, ctx, solver, pts, geom, a, cheb, "mixed", V, a.steps)
+    res, t_span = pipelined(q, ctx, solver, pts, geom, a, cheb, "mixed", V, a.steps,
+                            lanes=lanes)
     lat, counts, worst = [], [], 0.0
     for s in range(a.steps):
         lam, rk, t_start, t_end, t_dev, t_host, st = res[s]
@@ -652,9 +682,13 @@ def main():
         "data": "synthetic (seeded probes)",
         "config": {"workload": "beyn-cfie-kress star(1,0.2,5) k0=%g R=%g" % (a.k0, a.R),
                    "N": a.N, "nq": a.nq, "r": r_used, "r0": a.r0, "svd_tol": a.svd_tol,
-                   "cheb": cheb, "cheb_tol": a.cheb_tol, "refine_tol": a.refine_tol},
+                   "cheb": cheb, "cheb_tol": a.cheb_tol, "refine_tol": a.refine_tol,
+                   "inflight": max(1, a.inflight)},
         "p50_window_s": float(np.percentile(lat, 50)), "p99_window_s": float(np.percentile(lat, 99)),
-        "pipeline": "device windows back to back; host SVD/eig of window s overlaps window s+1",
+        "pipeline": ("device windows back to back; host SVD/eig of window s overlaps window s+1"
+                     if a.inflight <= 1 else
+                     f"{a.inflight} windows in flight on separate device contexts; host SVD/eig "
+                     "of each window overlaps the device work"),
         "eigenvalues_per_window": int(np.median(counts)),
         "max_rel_eig_err_vs_fp64": worst if ref is not None else None,
         "fp64_window_s": fp64_s,
