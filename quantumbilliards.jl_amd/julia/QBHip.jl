@@ -250,6 +250,52 @@ function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float6
     end
 end
 
+# Several disks on one geometry as one device window (qb_window_create_multi):
+# no counterpart in the reference, whose driver runs its disks one after
+# another; per disk the (B, U_k) of construct_B_matrix with the probes V.
+function construct_B_matrices(solver, pts, N::Int, k0s::Vector{ComplexF64},
+                              Rs::Vector{Float64}; nq::Int=64, r::Int=48, svd_tol=1e-14,
+                              rng=QB.MersenneTwister(0), n_panels_h::Int=15000, M_h::Int=5,
+                              n_panels_j::Int=3000, M_j::Int=5, precision::Symbol=:fp64,
+                              refine_tol::Float64=1e-12)
+    g = geom(solver, pts)
+    w = Ref{Ptr{Cvoid}}(C_NULL)
+    st = Ref(Stats())
+    cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
+    prec = precision === :mixed ? QB_MIXED : QB_FP64
+    nw = length(k0s)
+    k0v = collect(reinterpret(Float64, k0s))
+    try
+        check(ccall((:qb_window_create_multi, LIB), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Int32, Ptr{Float64}, Ptr{Float64}, Int32,
+                     Ref{ChebParams}, Int32, Ref{Ptr{Cvoid}}, Ref{Stats}),
+                    ctx(), g, opcode(solver), Int32(nw), k0v, Rs, Int32(nq), cp, prec, w, st))
+        nb = Int(ccall((:qb_window_blocks, LIB), Int32, (Ptr{Cvoid},), w[]))
+        V = randn(rng, ComplexF64, N, r)
+        Rm = Array{ComplexF64}(undef, r, r, nb); Cm = Array{ComplexF64}(undef, r, r, nb)
+        check(ccall((:qb_window_moments_qr, LIB), Cint,
+                    (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64},
+                     Float64, Ref{Stats}),
+                    w[], Int32(r), V, Rm, Cm, refine_tol, st))
+        out = Vector{Tuple{Matrix{ComplexF64},Matrix{ComplexF64}}}(undef, nb)
+        for c in 1:nb
+            U, Σ, W = svd(Rm[:, :, c])
+            rk = count(>=(svd_tol), Σ)
+            Ur = U[:, 1:rk]; Wk = W[:, 1:rk]
+            B = adjoint(Ur) * ((Cm[:, :, c] * Wk) ./ transpose(Σ[1:rk]))
+            Uk = Matrix{ComplexF64}(undef, N, rk)
+            rk > 0 && check(ccall((:qb_window_basis_irrep, LIB), Cint,
+                                  (Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Int32, Ptr{ComplexF64}),
+                                  w[], Int32(c - 1), Matrix(Ur), Int32(rk), Uk))
+            out[c] = (B, Uk)
+        end
+        return out
+    finally
+        w[] != C_NULL && ccall((:qb_window_destroy, LIB), Cvoid, (Ptr{Cvoid},), w[])
+        geom_destroy(g)
+    end
+end
+
 # residual_and_norm_select (beyn.jl:557-593): T(λ_j) assembled and applied to
 # φ_j = U_k Y[:, j] on the device; only the norms come back
 function residual_and_norm_select(solver, λ::AbstractVector{ComplexF64},
