@@ -98,6 +98,34 @@ def test_window_moments_match_host_solve(precision):
     win.close()
 
 
+def test_refinement_falls_back_to_fp64_factors():
+    """A refinement tolerance no FP64 residual can certify (1e-20) sends every
+    node of a mixed-precision window to the FP64 fallback: the nodes are
+    refactored in FP64 as one batch and solved there, so the moments are the
+    FP64 window's and the host solve's."""
+    q = qb()
+    b, pts = _pts("star", [320])
+    solver = q.CFIE_kress(10.0, b)
+    k0, R, nq = 30.0 + 0.0j, 0.4, 6
+    zj, wj = O.contour(k0, R, nq)
+    cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
+    T = q.assemble(solver, pts, zj, **cp)
+    V = q.probe_block(np.random.default_rng(5), 320, 16)
+    A0r, A1r = O.beyn_moments(np.array(T), zj, wj, V)
+    w64 = q.Window(solver, pts, k0, R, nq, precision="fp64", **cp)
+    B0, B1 = w64.moments(V)
+    w64.close()
+    win = q.Window(solver, pts, k0, R, nq, precision="mixed", **cp)
+    A0, A1 = win.moments(V, 1e-20)
+    st = win.stats
+    win.close()
+    print(st)
+    assert st["nodes_fp64_fallback"] == nq
+    for a, f, r in ((A0, B0, A0r), (A1, B1, A1r)):
+        assert np.max(np.abs(a - f)) / np.max(np.abs(f)) < 1e-13
+        assert np.max(np.abs(a - r)) / np.max(np.abs(r)) < 1e-11
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_beyn_circle_gives_bessel_zeros(precision):
     """cfg1: unit circle, N=256, k0=20, R=0.5, nq=32, 24 probes, svd_tol 1e-10."""
