@@ -5,6 +5,7 @@ profiles/.
   prof_summary.py pmc <fetch_csv> <write_csv>   FETCH_SIZE / WRITE_SIZE runs
   prof_summary.py pmcwin <fetch_csv> <write_csv>  the same over one Beyn window
   prof_summary.py mfma <csv>                    MFMA busy, LDS conflicts, waits per GEMM
+  prof_summary.py l1 <sq_csv> <ta_csv> <pairs>  address-unit / L1 counters of k_assemble
 
 `stats` prints the whole-run per-kernel table and, from the trace, the last
 Beyn window alone (the dispatches from the last table build on), which is the
@@ -136,8 +137,39 @@ def mfma(path, simds=1024, xcds=8):
         print(f"{d:8d} {100 * busy:8.1f}% {100 * lds:8.1f}% {100 * wait:8.1f}%  {k}")
 
 
+def l1(sq_csv, ta_csv, pairs, cus=256, ses=32):
+    """Counters of the k_assemble dispatch, summed over the device's
+    instances: TA and TCP per CU (cus), SQ per shader engine (ses, the
+    wave/busy cycle counters in units of 4 cycles), GRBM_GUI_ACTIVE over the
+    8 XCDs.  `pairs` = entry pairs of the dispatch (N(N+1)/2 per node)."""
+    agg = collections.defaultdict(float)
+    grbm = []
+    for path in (sq_csv, ta_csv):  # one run each; both also read GRBM_GUI_ACTIVE
+        for r in csv.DictReader(open(path)):
+            if "k_assemble<" in r["Kernel_Name"]:
+                if r["Counter_Name"] == "GRBM_GUI_ACTIVE":
+                    grbm.append(float(r["Counter_Value"]))
+                else:
+                    agg[r["Counter_Name"]] += float(r["Counter_Value"])
+    cyc = sum(grbm) / len(grbm) / 8  # per XCD: the kernel's cycles
+    print(f"kernel cycles (GRBM_GUI_ACTIVE / 8 XCDs): {cyc:.4g}")
+    print(f"TA busy per CU:                 {agg['TA_TA_BUSY'] / cus / cyc:.1%} of the cycles")
+    print(f"TA stalled by the L1 per CU:    {agg['TA_ADDR_STALLED_BY_TC_CYCLES'] / cus / cyc:.1%}")
+    print(f"L1 pending-miss stalls per CU:  {agg['TCP_PENDING_STALL_CYCLES'] / cus / cyc:.1%}")
+    print(f"L1 accesses per entry pair:     {agg['TCP_TOTAL_ACCESSES'] / pairs:.1f}")
+    print(f"L1 accesses per vector load:    {agg['TCP_TOTAL_ACCESSES'] / agg['SQ_INSTS_VMEM_RD']:.1f}")
+    print(f"L1 accesses per CU per cycle:   {agg['TCP_TOTAL_ACCESSES'] / cus / cyc:.2f}")
+    print(f"CU cycles per entry pair:       {cus * cyc / pairs:.1f}")
+    print(f"L1 -> L2 read requests:         {agg['TCP_TCC_READ_REQ']:.4g}")
+    print(f"VALU busy per SIMD:             "
+          f"{agg['SQ_ACTIVE_INST_VALU'] * 4 / (4 * cus) / cyc:.1%}")
+    print(f"waves resident per CU:          {agg['SQ_WAVE_CYCLES'] * 4 / cus / cyc:.1f}")
+
+
 if __name__ == "__main__":
-    if sys.argv[1] == "mfma":
+    if sys.argv[1] == "l1":
+        l1(sys.argv[2], sys.argv[3], float(sys.argv[4]))
+    elif sys.argv[1] == "mfma":
         mfma(sys.argv[2])
     elif sys.argv[1] == "stats":
         stats(sys.argv[2])
