@@ -1018,8 +1018,18 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
   }
   // tables + assembly (construct_boundary_matrices!, beyn.jl:264)
   QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nz, rmin, rmax, *cheb, true, &tab));
-  QB_TRY(qb_assemble_dev(ctx, g, op, w->z_dev, nz, tab, w->T64,
-                         precision == QB_MIXED ? w->LU32 : nullptr));
+  {
+    // conjugate node pairs share one Hankel gather per entry when the
+    // contour is symmetric about the real axis (real k0)
+    int crc = getenv("QB_ASM_NO_CONJ")
+                  ? 1
+                  : qb_assemble_conj_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, tab, w->T64,
+                                         precision == QB_MIXED ? w->LU32 : nullptr);
+    if (crc == 1)
+      crc = qb_assemble_dev(ctx, g, op, w->z_dev, nz, tab, w->T64,
+                            precision == QB_MIXED ? w->LU32 : nullptr);
+    QB_TRY(crc);
+  }
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     rc = qb_fail(ctx, QB_E_HIP, "assembly failed");
     goto fail;

@@ -240,6 +240,193 @@ static void launch_asm(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, i
   qb_prof_end(ctx, slot);
 }
 
+// Conjugate-pair assembly for contours symmetric about the real axis (the
+// Beyn windows with real k0): nodes a and b with k_b = conj(k_a) share the
+// panel of every distance, and
+//   J_n(k_a r) = (H_n(k_a r) + conj(H_n(k_b r))) / 2,  J_n(k_b r) = conj(J_n(k_a r))
+// (H^(2)(z) = conj(H^(1)(conj z)) for real order), so one block assembles both
+// matrices from the two nodes' Hankel panels alone: per entry pair and node,
+// one panel gather instead of the H and J gathers of k_assemble.  A
+// self-conjugate node (real k, a == b) gives J = Re H.  The mirror entries
+// (j, i) are written directly (scattered 16-byte stores that the L2 cache
+// merges into whole lines).  pairs[blockIdx.z] = (a, b); outputs indexed by
+// node from out64 / out32.
+template <int OP, int MH>
+__global__ void __launch_bounds__(256)
+k_assemble_conj(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ H, int np_h,
+                double rmin_h, double rmax, const int* __restrict__ pairs, int T,
+                cd* __restrict__ out64, cf* __restrict__ out32) {
+  constexpr int PH = 2 * (MH + 1);
+  const int64_t N = g.N;
+  const int qa = pairs[2 * blockIdx.z], qb = pairs[2 * blockIdx.z + 1];
+  const bool two = qa != qb;
+  int64_t p = blockIdx.x;
+  int d = 0;
+  while (p >= T - d) {
+    p -= T - d;
+    ++d;
+  }
+  const int Jt = (int)p, It = Jt + d;
+  const cd* Ha = H + (int64_t)qa * np_h * PH;
+  const cd* Hb = H + (int64_t)qb * np_h * PH;
+  const cd ka = zk[qa], kb = zk[qb];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t i = (int64_t)It * 64 + il;
+  if (i >= N) return;  // no barrier in this kernel
+  const int64_t ba = (int64_t)qa * N * N, bb = (int64_t)qb * N * N;
+  const bool diag_tile = (It == Jt);
+  const double xi = g.x[i], yi = g.y[i];
+  const int ci = g.comp[i];
+#pragma unroll 1
+  for (int c = 0; c < 16; ++c) {
+    const int64_t j = (int64_t)Jt * 64 + jl0 + 4 * c;
+    if (j >= N) break;
+    cd va, vb, wa = mk(0, 0), wb = mk(0, 0);  // (i, j) and (j, i) of nodes a and b
+    if (i == j) {
+      va = diag_entry<OP>(g, j, ka);
+      vb = two ? diag_entry<OP>(g, j, kb) : va;
+    } else {
+      const double dx = xi - g.x[j], dy = yi - g.y[j];
+      const double r = hypot(dx, dy);
+      const double invr = 1.0 / r;
+      const bool same = (ci == g.comp[j]);
+      double Rij = 0, lt = 0;
+      if (same) {
+        const int64_t o = g.off[ci];
+        const int64_t nc = g.off[ci + 1] - o;
+        int64_t dd = (i - j) % nc;
+        if (dd < 0) dd += nc;
+        Rij = g.rk[o + dd];
+        lt = g.lt[o + dd];  // both circulant columns are even in (i - j)
+      }
+      int pp;
+      double t;
+      qb_panel(r, rmin_h, rmax, np_h, &pp, &t);
+      Spec sa, sb;
+      qb_clenshaw2<MH>(Ha + (int64_t)pp * PH, t, &sa.h0, &sa.h1);
+      if (two)
+        qb_clenshaw2<MH>(Hb + (int64_t)pp * PH, t, &sb.h0, &sb.h1);
+      else
+        sb = sa;
+      if (OP == QB_CFIE_KRESS) {
+        // h0_h1_multi_ks_at_r! (bessels:1190-1206): |kr| < 0.2 directly
+        const cd z = mk(ka.re * r, ka.im * r);
+        if (cabsd(z) < 0.2) {
+          qb_hankel1_01(z, &sa.h0, &sa.h1);
+          if (two) qb_hankel1_01(mk(kb.re * r, kb.im * r), &sb.h0, &sb.h1);
+          else sb = sa;
+        }
+      }
+      if (same || OP == QB_DLP_KRESS) {
+        sa.j0 = 0.5 * (sa.h0 + cconj(sb.h0));
+        sa.j1 = 0.5 * (sa.h1 + cconj(sb.h1));
+      } else {
+        sa.j0 = mk(0, 0);
+        sa.j1 = mk(0, 0);
+      }
+      sb.j0 = cconj(sa.j0);
+      sb.j1 = cconj(sa.j1);
+      va = offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, ka, sa);
+      vb = two ? offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, kb, sb) : va;
+      if (!diag_tile) {
+        wa = offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, ka, sa);
+        wb = two ? offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, kb, sb) : wa;
+      }
+    }
+    const int64_t e1 = i + j * N, e2 = j + i * N;
+    if (out64) {
+      out64[ba + e1] = va;
+      if (two) out64[bb + e1] = vb;
+      if (!diag_tile) {
+        out64[ba + e2] = wa;
+        if (two) out64[bb + e2] = wb;
+      }
+    }
+    if (out32) {
+      out32[ba + e1] = to_cf(va);
+      if (two) out32[bb + e1] = to_cf(vb);
+      if (!diag_tile) {
+        out32[ba + e2] = to_cf(wa);
+        if (two) out32[bb + e2] = to_cf(wb);
+      }
+    }
+  }
+}
+
+int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
+                         const cd* z_dev, int64_t nz, const DevTables& t, cd* out64,
+                         cf* out32) {
+  // usable: plain geometry, tables of static degree 5 or 7, and every node's
+  // conjugate among the nodes (to rounding); returns 1 when not usable
+  if (g->sym.m > 0 || t.M_h != t.M_j || (t.M_h != 5 && t.M_h != 7)) return 1;
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return 1;
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1) return 1;
+  // one-to-one pairing (several disks of one window can share a real node):
+  // a real node pairs with itself, any other node with an unused conjugate
+  std::vector<int> pr;
+  std::vector<char> used(nz, 0);
+  for (int64_t a = 0; a < nz; ++a) {
+    if (used[a]) continue;
+    const cd za = z_host[a];
+    const double sc = fabs(za.re) + fabs(za.im);
+    int64_t b = -1;
+    if (fabs(za.im) <= 1e-14 * sc) {
+      b = a;
+    } else {
+      for (int64_t c = a + 1; c < nz; ++c) {
+        const cd zc = z_host[c];
+        if (!used[c] && fabs(zc.re - za.re) <= 1e-14 * sc && fabs(zc.im + za.im) <= 1e-14 * sc) {
+          b = c;
+          break;
+        }
+      }
+    }
+    if (b < 0) return 1;
+    used[a] = used[b] = 1;
+    pr.push_back((int)a);
+    pr.push_back((int)b);
+  }
+  int npair = (int)pr.size() / 2;
+  if (npair < 1 || npair > 65535) return 1;
+  int* pd = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&pd, pr.size() * sizeof(int));
+  if (rc) return rc;
+  if (hipMemcpyAsync(pd, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice,
+                     ctx->stream) != hipSuccess) {
+    qb_dev_free(ctx, pd, pr.size() * sizeof(int));
+    return qb_fail(ctx, QB_E_HIP, "pair upload failed");
+  }
+  const int64_t N = g->d.N;
+  const int T = (int)((N + 63) / 64);
+  const int64_t tpairs = (int64_t)T * (T + 1) / 2;
+  const double ent = (double)N * N * nz;
+  int slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
+                           ent * ((out64 ? sizeof(cd) : 0) + (out32 ? sizeof(cf) : 0)));
+  const dim3 grid((unsigned)tpairs, 1, (unsigned)npair);
+  if (op == QB_CFIE_KRESS) {
+    if (t.M_h == 5)
+      hipLaunchKernelGGL((k_assemble_conj<QB_CFIE_KRESS, 5>), grid, dim3(256), 0, ctx->stream,
+                         g->d, z_dev, t.H, t.np_h, t.rmin_h, t.rmax, pd, T, out64, out32);
+    else
+      hipLaunchKernelGGL((k_assemble_conj<QB_CFIE_KRESS, 7>), grid, dim3(256), 0, ctx->stream,
+                         g->d, z_dev, t.H, t.np_h, t.rmin_h, t.rmax, pd, T, out64, out32);
+  } else {
+    if (t.M_h == 5)
+      hipLaunchKernelGGL((k_assemble_conj<QB_DLP_KRESS, 5>), grid, dim3(256), 0, ctx->stream,
+                         g->d, z_dev, t.H, t.np_h, t.rmin_h, t.rmax, pd, T, out64, out32);
+    else
+      hipLaunchKernelGGL((k_assemble_conj<QB_DLP_KRESS, 7>), grid, dim3(256), 0, ctx->stream,
+                         g->d, z_dev, t.H, t.np_h, t.rmin_h, t.rmax, pd, T, out64, out32);
+  }
+  qb_prof_end(ctx, slot);
+  hipError_t e = hipGetLastError();
+  // the pair list must outlive the launch: wait before handing it back
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, pd, pr.size() * sizeof(int));
+  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  return QB_OK;
+}
+
 int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
                     const DevTables& t, cd* out64, cf* out32) {
   if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS)

@@ -178,6 +178,12 @@ int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
 // The same two for a geometry with symmetry orbits: m x m reduced matrices
 // (ld = m, stride m*m).  qb_assemble_dev / qb_assemble_deriv_dev call these
 // when g->sym.m > 0.
+// Both matrices of every conjugate node pair from the Hankel tables alone
+// (windows whose contour is symmetric about the real axis); returns 1, having
+// done nothing, when the nodes or tables do not allow it.
+int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
+                         const cd* z_dev, int64_t nz, const DevTables& t, cd* out64,
+                         cf* out32);
 int qb_assemble_red_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
                         const DevTables& t, cd* out64, cf* out32);
 // Checks |x_{rep_a} - x_{img[b,l]}| = |x_{rep_b} - x_{img[a,inv[l]]}| to
