@@ -248,14 +248,15 @@ static void launch_asm(qb_ctx* ctx, const qb_geom* g, const cd* z, int64_t q0, i
 // matrices from the two nodes' Hankel panels alone: per entry pair and node,
 // one panel gather instead of the H and J gathers of k_assemble.  A
 // self-conjugate node (real k, a == b) gives J = Re H.  The mirror entries
-// (j, i) are written directly (scattered 16-byte stores that the L2 cache
-// merges into whole lines).  pairs[blockIdx.z] = (a, b); outputs indexed by
-// node from out64 / out32.
+// (j, i) of both nodes are staged in LDS 32 rows at a time and written as
+// columns of the mirror tiles.  pairs[blockIdx.z] = (a, b); outputs indexed
+// by node from out64 / out32.
 template <int OP, int MH>
 __global__ void __launch_bounds__(256)
 k_assemble_conj(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ H, int np_h,
                 double rmin_h, double rmax, const int* __restrict__ pairs, int T,
                 cd* __restrict__ out64, cf* __restrict__ out32) {
+  __shared__ cd mir[2][64][33];  // [node][i_local][j_local - 32 h] = A[j, i], half h
   constexpr int PH = 2 * (MH + 1);
   const int64_t N = g.N;
   const int qa = pairs[2 * blockIdx.z], qb = pairs[2 * blockIdx.z + 1];
@@ -272,84 +273,108 @@ k_assemble_conj(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ H, 
   const cd ka = zk[qa], kb = zk[qb];
   const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
   const int64_t i = (int64_t)It * 64 + il;
-  if (i >= N) return;  // no barrier in this kernel
+  const bool iok = i < N;
   const int64_t ba = (int64_t)qa * N * N, bb = (int64_t)qb * N * N;
   const bool diag_tile = (It == Jt);
-  const double xi = g.x[i], yi = g.y[i];
-  const int ci = g.comp[i];
+  const int64_t ic = iok ? i : N - 1;
+  const double xi = g.x[ic], yi = g.y[ic];
+  const int ci = g.comp[ic];
 #pragma unroll 1
-  for (int c = 0; c < 16; ++c) {
-    const int64_t j = (int64_t)Jt * 64 + jl0 + 4 * c;
-    if (j >= N) break;
-    cd va, vb, wa = mk(0, 0), wb = mk(0, 0);  // (i, j) and (j, i) of nodes a and b
-    if (i == j) {
-      va = diag_entry<OP>(g, j, ka);
-      vb = two ? diag_entry<OP>(g, j, kb) : va;
-    } else {
-      const double dx = xi - g.x[j], dy = yi - g.y[j];
-      const double r = hypot(dx, dy);
-      const double invr = 1.0 / r;
-      const bool same = (ci == g.comp[j]);
-      double Rij = 0, lt = 0;
-      if (same) {
-        const int64_t o = g.off[ci];
-        const int64_t nc = g.off[ci + 1] - o;
-        int64_t dd = (i - j) % nc;
-        if (dd < 0) dd += nc;
-        Rij = g.rk[o + dd];
-        lt = g.lt[o + dd];  // both circulant columns are even in (i - j)
-      }
-      int pp;
-      double t;
-      qb_panel(r, rmin_h, rmax, np_h, &pp, &t);
-      Spec sa, sb;
-      qb_clenshaw2<MH>(Ha + (int64_t)pp * PH, t, &sa.h0, &sa.h1);
-      if (two)
-        qb_clenshaw2<MH>(Hb + (int64_t)pp * PH, t, &sb.h0, &sb.h1);
-      else
-        sb = sa;
-      if (OP == QB_CFIE_KRESS) {
-        // h0_h1_multi_ks_at_r! (bessels:1190-1206): |kr| < 0.2 directly
-        const cd z = mk(ka.re * r, ka.im * r);
-        if (cabsd(z) < 0.2) {
-          qb_hankel1_01(z, &sa.h0, &sa.h1);
-          if (two) qb_hankel1_01(mk(kb.re * r, kb.im * r), &sb.h0, &sb.h1);
-          else sb = sa;
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll 1
+    for (int c = 8 * h; c < 8 * h + 8; ++c) {
+      const int jl = jl0 + 4 * c;
+      const int64_t j = (int64_t)Jt * 64 + jl;
+      if (!iok || j >= N) continue;
+      cd va, vb, wa = mk(0, 0), wb = mk(0, 0);  // (i, j) and (j, i) of nodes a and b
+      if (i == j) {
+        va = diag_entry<OP>(g, j, ka);
+        vb = two ? diag_entry<OP>(g, j, kb) : va;
+      } else {
+        const double dx = xi - g.x[j], dy = yi - g.y[j];
+        const double r = hypot(dx, dy);
+        const double invr = 1.0 / r;
+        const bool same = (ci == g.comp[j]);
+        double Rij = 0, lt = 0;
+        if (same) {
+          const int64_t o = g.off[ci];
+          const int64_t nc = g.off[ci + 1] - o;
+          int64_t dd = (i - j) % nc;
+          if (dd < 0) dd += nc;
+          Rij = g.rk[o + dd];
+          lt = g.lt[o + dd];  // both circulant columns are even in (i - j)
+        }
+        int pp;
+        double t;
+        qb_panel(r, rmin_h, rmax, np_h, &pp, &t);
+        Spec sa, sb;
+        qb_clenshaw2<MH>(Ha + (int64_t)pp * PH, t, &sa.h0, &sa.h1);
+        if (two)
+          qb_clenshaw2<MH>(Hb + (int64_t)pp * PH, t, &sb.h0, &sb.h1);
+        else
+          sb = sa;
+        if (OP == QB_CFIE_KRESS) {
+          // h0_h1_multi_ks_at_r! (bessels:1190-1206): |kr| < 0.2 directly
+          const cd z = mk(ka.re * r, ka.im * r);
+          if (cabsd(z) < 0.2) {
+            qb_hankel1_01(z, &sa.h0, &sa.h1);
+            if (two)
+              qb_hankel1_01(mk(kb.re * r, kb.im * r), &sb.h0, &sb.h1);
+            else
+              sb = sa;
+          }
+        }
+        if (same || OP == QB_DLP_KRESS) {
+          sa.j0 = 0.5 * (sa.h0 + cconj(sb.h0));
+          sa.j1 = 0.5 * (sa.h1 + cconj(sb.h1));
+        } else {
+          sa.j0 = mk(0, 0);
+          sa.j1 = mk(0, 0);
+        }
+        sb.j0 = cconj(sa.j0);
+        sb.j1 = cconj(sa.j1);
+        va = offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, ka, sa);
+        vb = two ? offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, kb, sb) : va;
+        if (!diag_tile) {
+          wa = offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, ka, sa);
+          wb = two ? offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, kb, sb) : wa;
         }
       }
-      if (same || OP == QB_DLP_KRESS) {
-        sa.j0 = 0.5 * (sa.h0 + cconj(sb.h0));
-        sa.j1 = 0.5 * (sa.h1 + cconj(sb.h1));
-      } else {
-        sa.j0 = mk(0, 0);
-        sa.j1 = mk(0, 0);
+      const int64_t e1 = i + j * N;
+      if (out64) {
+        out64[ba + e1] = va;
+        if (two) out64[bb + e1] = vb;
       }
-      sb.j0 = cconj(sa.j0);
-      sb.j1 = cconj(sa.j1);
-      va = offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, ka, sa);
-      vb = two ? offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, kb, sb) : va;
+      if (out32) {
+        out32[ba + e1] = to_cf(va);
+        if (two) out32[bb + e1] = to_cf(vb);
+      }
       if (!diag_tile) {
-        wa = offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, ka, sa);
-        wb = two ? offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, kb, sb) : wa;
+        mir[0][il][jl - 32 * h] = wa;
+        mir[1][il][jl - 32 * h] = wb;
       }
     }
-    const int64_t e1 = i + j * N, e2 = j + i * N;
-    if (out64) {
-      out64[ba + e1] = va;
-      if (two) out64[bb + e1] = vb;
-      if (!diag_tile) {
-        out64[ba + e2] = wa;
-        if (two) out64[bb + e2] = wb;
+    if (diag_tile) continue;  // uniform over the block
+    __syncthreads();
+    // mirror rows j = Jt*64 + 32h + 0..31 of the 64 columns i: lanes 0..31
+    // write one column's 32 contiguous rows, lanes 32..63 the next column
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c) {
+      const int il2 = 2 * (jl0 + 4 * c) + (threadIdx.x >> 5 & 1);
+      const int jl2 = threadIdx.x & 31;
+      const int64_t ii = (int64_t)It * 64 + il2, jj = (int64_t)Jt * 64 + 32 * h + jl2;
+      if (ii >= N || jj >= N) continue;
+      const int64_t e2 = jj + ii * N;
+      const cd xa = mir[0][il2][jl2];
+      if (out64) out64[ba + e2] = xa;
+      if (out32) out32[ba + e2] = to_cf(xa);
+      if (two) {
+        const cd xb = mir[1][il2][jl2];
+        if (out64) out64[bb + e2] = xb;
+        if (out32) out32[bb + e2] = to_cf(xb);
       }
     }
-    if (out32) {
-      out32[ba + e1] = to_cf(va);
-      if (two) out32[bb + e1] = to_cf(vb);
-      if (!diag_tile) {
-        out32[ba + e2] = to_cf(wa);
-        if (two) out32[bb + e2] = to_cf(wb);
-      }
-    }
+    __syncthreads();
   }
 }
 
