@@ -29,7 +29,7 @@ PEAK = {"c64": 157.3, "c128": 78.6}  # dense MFMA TFLOP/s (fp32, fp64)
 # HBM bytes per complex64 GEMM launch over one window of the default workload,
 # from FETCH_SIZE / WRITE_SIZE counter runs scaled by a copy kernel of known
 # size (profiles/beyn_window_star_N8192_mixed_pmc_traffic.txt)
-TRAFFIC_C64_PER_LAUNCH = 519.4e6
+TRAFFIC_C64_PER_LAUNCH = 509.9e6
 
 
 def parse():
