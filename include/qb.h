@@ -184,7 +184,10 @@ int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
  * factors stay resident in the window so the moment solve can be repeated
  * with more probes (the r -> 2r retry of beyn.jl:291-309).  precision:
  * QB_FP64 (plain complex128 path) or QB_MIXED (complex64 LU on MFMA + FP64
- * iterative refinement against the resident FP64 T). */
+ * iterative refinement against the resident FP64 T).  With k0 real the nodes
+ * come in conjugate pairs, and the window assembles each pair from the two
+ * nodes' Hankel tables, J_n(kr) = (H_n(kr) + conj H_n(conj(k) r)) / 2, instead
+ * of reading the J tables (the same values to the tables' tolerance). */
 int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
                      double k0_im, double R, int32_t nq,
                      const qb_cheb_params* cheb, int32_t precision,
