@@ -54,6 +54,10 @@ def test_multi_disk_window_matches_separate_windows(prec):
 
 
 def test_multi_disk_window_on_the_circle_gives_the_bessel_zeros():
+    """Each disk of a multi-disk window gives the zeros of J_m.  The disks at
+    19.0 and 20.0 (R = 0.5) share the real node 19.5: the conjugate-pair
+    assembly must assemble both copies (each node pairs with itself or with
+    one unused conjugate)."""
     import qbhip as q
     b = q.make_circle(1.0)
     solver = q.CFIE_kress(10.0, b)
