@@ -51,8 +51,8 @@ def test_uneven_workload_node_classes():
 def test_real_roots_filter():
     """summarize() counts the candidates with |Im k| <= REAL_TOL |k|."""
     from qbhip.serve import REAL_TOL, real_roots
-    lam = np.array([500.0 + 1e-12j, 500.1 - 4e-6j, 499.9 + 0.02j, 500.2 + 0j,
-                    500.3 + 0.9 * REAL_TOL * 500.3j])
+    lam = np.array([500.0 + 1e-12j, 500.1 - 4e-5j, 499.9 + 0.02j, 500.2 + 0j,
+                    500.3 + 0.9 * REAL_TOL * 500.3j, 500.4 - 1.1 * REAL_TOL * 500.4j])
     got = real_roots(lam)
     assert np.array_equal(got, lam[[0, 3, 4]])
     assert real_roots(np.zeros(0, complex)).size == 0
