@@ -66,6 +66,26 @@ def test_cfie_assembly_matches_oracle(kind, Ns, k0):
     assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
 
 
+def test_cfie_stadium_assembly_matches_oracle():
+    """The stadium (a C1 composite of five segments; its nodes are pinned by
+    closed forms in tests/test_geometry_host.py) assembled on the device and
+    by the oracle from the same nodes."""
+    q = qb()
+    b = q.make_full_stadium(1.0, 1.0)
+    pts = q.evaluate_points(q.CFIE_kress(10.0, b), b, 1.0, N=[400])
+    p = pts[0]
+    comps = [O.Comp(p.xy[:, 0].copy(), p.xy[:, 1].copy(), p.tangent[:, 0].copy(),
+                    p.tangent[:, 1].copy(), p.tangent_2[:, 0].copy(), p.tangent_2[:, 1].copy(),
+                    p.ts.copy(), p.ws.copy())]
+    zj, _ = O.contour(15.0, 0.3, 8)
+    cp = dict(n_panels_h=4000, M_h=5, n_panels_j=1500, M_j=5)
+    T = q.assemble(q.CFIE_kress(10.0, b), pts, zj, **cp)
+    Tref = O.cfie_assemble(comps, zj, O.ChebParams(4000, 5, 1500, 5))
+    e = _relerr(T, Tref)
+    print("stadium", e.max(axis=0))
+    assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
+
+
 def test_dlp_assembly_matches_oracle():
     q = qb()
     b = q.make_full_ellipse(1.0, 0.6)
