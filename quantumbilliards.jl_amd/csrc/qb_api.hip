@@ -287,24 +287,44 @@ __global__ void k_kress_cols(int64_t N, int ncomp, DevGeom g, double* rk, double
 }
 
 // Padded range of pairwise distances (cheb-CFIE:374-418, cheb-DLP:135-147):
-// same-component pairs skip i == j, all pairs skip r <= eps.
-__global__ void k_rrange(DevGeom g, int only_first, unsigned long long* mm) {
-  int64_t N = only_first ? g.off[1] : g.N;
-  int64_t j = blockIdx.y;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// same-component pairs skip i == j, all pairs skip r <= eps.  A block takes
+// 256 targets i against the sources j = blockIdx.y, blockIdx.y + gridDim.y,
+// ..., reduces in registers and LDS, and issues one atomic pair: one pair
+// per wavefront and source column serialises about N^2/64 atomics on two
+// addresses.  Non-negative doubles order like their bit patterns, so the
+// unsigned 64-bit atomics take the min and max of the distances.
+__global__ __launch_bounds__(256) void k_rrange(DevGeom g, int only_first,
+                                                unsigned long long* mm) {
+  const int64_t N = only_first ? g.off[1] : g.N;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double rmn = 1e300, rmx = 0.0;
-  if (i < N && i != j) {
-    double r = hypot(g.x[i] - g.x[j], g.y[i] - g.y[j]);
-    if (r > 2.220446049250313e-16) {
-      rmn = r;
-      rmx = r;
+  if (i < N) {
+    const double xi = g.x[i], yi = g.y[i];
+    for (int64_t j = blockIdx.y; j < N; j += gridDim.y) {
+      if (j == i) continue;
+      const double r = hypot(xi - g.x[j], yi - g.y[j]);
+      if (r > 2.220446049250313e-16) {
+        rmn = fmin(rmn, r);
+        rmx = fmax(rmx, r);
+      }
     }
   }
   for (int off = 32; off >= 1; off >>= 1) {
     rmn = fmin(rmn, __shfl_xor(rmn, off));
     rmx = fmax(rmx, __shfl_xor(rmx, off));
   }
+  __shared__ double smn[4], smx[4];
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
+    smn[w] = rmn;
+    smx[w] = rmx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < 4; ++q) {
+      rmn = fmin(rmn, smn[q]);
+      rmx = fmax(rmx, smx[q]);
+    }
     atomicMin(&mm[0], (unsigned long long)__double_as_longlong(rmn));
     atomicMax(&mm[1], (unsigned long long)__double_as_longlong(rmx));
   }
@@ -326,7 +346,8 @@ static int rrange(qb_ctx* ctx, const DevGeom& d, int only_first, double* rmin, d
   init[1] = 0ull;
   QB_HIP(ctx, hipMemcpyAsync(mm, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
   int64_t N = only_first ? d.off[1] : d.N;
-  dim3 grid((unsigned)((N + 255) / 256), (unsigned)N);
+  // k_rrange reduces over exactly 4 wavefronts of 256 threads per block
+  dim3 grid((unsigned)((N + 255) / 256), (unsigned)std::min<int64_t>(N, 256));
   hipLaunchKernelGGL(k_rrange, grid, dim3(256), 0, ctx->stream, d, only_first, mm);
   QB_HIP(ctx, hipGetLastError());
   unsigned long long h[2];

@@ -86,6 +86,35 @@ def test_cfie_stadium_assembly_matches_oracle():
     assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
 
 
+@pytest.mark.parametrize("kind,Ns,op", [("circle_hole", [200, 60], "cfie"),
+                                        ("star", [1000], "dlp"), ("star", [4000], "cfie")])
+def test_table_range_matches_oracle(kind, Ns, op):
+    """The padded distance range of the tables that qb_assemble reports
+    (rminmax) against the oracle's: 0.95 x the smallest and 1.05 x the
+    largest pairwise distance above eps, i != j, over all component blocks
+    (CFIE) or over the first component (DLP). Distances agree to the last
+    bit of hypot."""
+    import ctypes as C
+    from qbhip import _lib
+    from qbhip.solvers import _cheb
+    q = qb()
+    b, pts = _pts(kind, Ns)
+    solver = (q.CFIE_kress if op == "cfie" else q.DLP_kress)(10.0, b)
+    ctx = q.default_context()
+    g = q.Geometry(pts, ctx, solver)
+    zj = np.array([20.0 + 0.1j])
+    raw = np.empty((1, g.dim, g.dim), np.complex128)
+    rr = np.zeros(2)
+    cp = _cheb(400, 5, 200, 5)
+    rc = _lib.lib().qb_assemble(ctx.ptr, g.ptr, solver.op, zj.ctypes.data, 1, C.byref(cp),
+                                raw.ctypes.data, 0, rr.ctypes.data)
+    _lib.check(ctx.ptr, rc)
+    g.close()
+    comps = O.make_geometry(kind, Ns)
+    ref = np.array(O._rrange_blocks(comps if op == "cfie" else comps[:1]))
+    assert np.max(np.abs(rr - ref) / ref) < 1e-15, (rr, ref)
+
+
 def test_dlp_assembly_matches_oracle():
     q = qb()
     b = q.make_full_ellipse(1.0, 0.6)
