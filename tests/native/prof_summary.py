@@ -1,7 +1,7 @@
 """Condense rocprofv3 CSV output into the short text summaries kept under
 profiles/.
 
-  prof_summary.py stats <dir>/<prefix>          kernel-trace run of bench.py
+  prof_summary.py stats <dir>/<prefix> [marker] kernel-trace run of bench.py
   prof_summary.py pmc <fetch_csv> <write_csv>   FETCH_SIZE / WRITE_SIZE runs
   prof_summary.py pmcwin <fetch_csv> <write_csv>  the same over one Beyn window
   prof_summary.py mfma <csv>                    MFMA busy, LDS conflicts, waits per GEMM
@@ -9,7 +9,10 @@ profiles/.
 
 `stats` prints the whole-run per-kernel table and, from the trace, the last
 Beyn window alone (the dispatches from the last table build on), which is the
-window bench.py times.  `pmc` scales each counter by what it reads for the
+window bench.py times.  With a marker kernel name, the window is the one that
+holds the marker's last dispatch: from the table build before it up to the
+next table build (bench.py --symmetry runs its unreduced reference window
+after the reduced ones).  `pmc` scales each counter by what it reads for the
 plain copy kernel k_copy16 (known bytes, 1 GiB read + 1 GiB written) and
 prints HBM bytes per dispatch beside the algorithmic bytes of the GEMM
 shapes the probe launches.
@@ -23,7 +26,7 @@ def _short(name):
     return name.split("(")[0].replace("void ", "")
 
 
-def stats(prefix):
+def stats(prefix, marker=None):
     rows = list(csv.DictReader(open(prefix + "_kernel_stats.csv")))
     print("whole run (rocprofv3 --stats):")
     print(f"{'total ms':>10} {'calls':>7} {'avg ms':>9}  kernel")
@@ -31,8 +34,15 @@ def stats(prefix):
         print(f"{int(r['TotalDurationNs']) / 1e6:10.3f} {int(r['Calls']):7d} "
               f"{float(r['AverageNs']) / 1e6:9.4f}  {_short(r['Name'])}")
     tr = list(csv.DictReader(open(prefix + "_kernel_trace.csv")))
-    start = max(i for i, r in enumerate(tr) if "k_build_tables<0>" in r["Kernel_Name"])
-    sel = tr[start:]
+    builds = [i for i, r in enumerate(tr) if "k_build_tables<0>" in r["Kernel_Name"]]
+    end = len(tr)
+    if marker:
+        last = max(i for i, r in enumerate(tr) if marker in r["Kernel_Name"])
+        start = max(i for i in builds if i < last)
+        end = min([i for i in builds if i > last] + [len(tr)])
+    else:
+        start = builds[-1]
+    sel = tr[start:end]
     agg = collections.defaultdict(lambda: [0, 0.0])
     for r in sel:
         d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
@@ -41,7 +51,7 @@ def stats(prefix):
         a[1] += d
     span = (int(sel[-1]["End_Timestamp"]) - int(sel[0]["Start_Timestamp"])) / 1e6
     busy = sum(v[1] for v in agg.values())
-    print(f"\nlast window (trace, dispatches {start}..{len(tr) - 1}): span {span:.1f} ms, "
+    print(f"\nlast window (trace, dispatches {start}..{end - 1}): span {span:.1f} ms, "
           f"kernel busy {busy:.1f} ms (two streams overlap)")
     print(f"{'total ms':>10} {'calls':>7} {'avg ms':>9}  kernel")
     for k, v in sorted(agg.items(), key=lambda kv: -kv[1][1]):
@@ -172,7 +182,7 @@ if __name__ == "__main__":
     elif sys.argv[1] == "mfma":
         mfma(sys.argv[2])
     elif sys.argv[1] == "stats":
-        stats(sys.argv[2])
+        stats(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
     elif sys.argv[1] == "pmcwin":
         pmcwin(sys.argv[2], sys.argv[3])
     else:
