@@ -1437,11 +1437,11 @@ extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, do
   w->qr_r = 0;
   int rc = win_moments(w, r, V, nullptr, nullptr, refine_tol, nullptr, w->qr);
   if (rc) return rc;
-  // one QR per block on its [A0_b | A1_b]
+  // one QR per block on its [A0_b | A1_b], all blocks in the same launches
+  rc = qb_house_qr(ctx, w->qr, N, r, 2 * r, w->qr_tau, nirr, (int64_t)2 * N * r, (int64_t)r);
+  if (rc) return rc;
   for (int ie = 0; ie < nirr; ++ie) {
-    cd* blk = w->qr + (size_t)2 * ie * N * r;
-    rc = qb_house_qr(ctx, blk, N, r, 2 * r, w->qr_tau + (size_t)ie * r);
-    if (rc) return rc;
+    const cd* blk = w->qr + (size_t)2 * ie * N * r;
     const cd* Cd = blk + (size_t)N * r;
     double* Re = R + (size_t)2 * ie * r * r;
     double* Ce = Cm + (size_t)2 * ie * r * r;

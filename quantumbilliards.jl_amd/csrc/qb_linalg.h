@@ -61,8 +61,11 @@ int qb_gemv_norms(qb_ctx* ctx, const cd* T, const cd* phi, int64_t N, int nb, in
 
 // Householder QR (qb_qr.hip): the first r columns of the N x ncols block A
 // (ld = N) are factored in place (R on and above the diagonal, reflectors
-// below, tau[r]) and H_j^H is applied to all columns right of each j.
-int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau);
+// below, tau[r]) and H_j^H is applied to all columns right of each j.  nb
+// such blocks are factored together, block b at A + b * bstride with its tau
+// at tau + b * tstride (2r launches for all of them).
+int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau, int nb,
+                int64_t bstride, int64_t tstride);
 // Y <- Q Y for the N x ncols block Y (ld = N), Q = H_0 ... H_{r-1} from V.
 int qb_house_apply_q(qb_ctx* ctx, const cd* V, int64_t N, int r, const cd* tau, cd* Y,
                      int ncols);

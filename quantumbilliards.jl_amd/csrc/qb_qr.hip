@@ -16,11 +16,15 @@
 #include "qb_internal.h"
 #include "qb_linalg.h"
 
-// zlarfg for column j: rows j..N-1.  One workgroup.
+// zlarfg for column j: rows j..N-1.  One workgroup per block; block b =
+// blockIdx.y holds its matrix at A + b * bstride and its tau at tau + b * tstride.
 __global__ void __launch_bounds__(1024) k_house(cd* __restrict__ A, int64_t N, int64_t ld, int j,
-                                                cd* __restrict__ tau) {
+                                                cd* __restrict__ tau, int64_t bstride,
+                                                int64_t tstride) {
   __shared__ double red[16];
   __shared__ cd s_scale;
+  A += (int64_t)blockIdx.y * bstride;
+  tau += (int64_t)blockIdx.y * tstride;
   cd* x = A + (int64_t)j * ld;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double s = 0;
@@ -52,11 +56,16 @@ __global__ void __launch_bounds__(1024) k_house(cd* __restrict__ A, int64_t N, i
 // Columns c0 + blockIdx.x (< c1) of A: a <- a - t v (v^H a) over rows j..N-1,
 // with v the stored reflector of column j (v[j] = 1) of V and t = conj(tau_j)
 // (ADJ: applying H_j^H, the QR sweep) or tau_j (applying H_j, forming Q).
+// Block b = blockIdx.y: V, A at + b * bstride, tau at + b * tstride.
 template <bool ADJ>
 __global__ void __launch_bounds__(256) k_house_apply(const cd* __restrict__ V, int64_t ldv,
                                                      cd* __restrict__ A, int64_t N, int64_t lda,
-                                                     int j, int c0, const cd* __restrict__ tau) {
+                                                     int j, int c0, const cd* __restrict__ tau,
+                                                     int64_t bstride, int64_t tstride) {
   __shared__ double red_re[4], red_im[4];
+  V += (int64_t)blockIdx.y * bstride;
+  A += (int64_t)blockIdx.y * bstride;
+  tau += (int64_t)blockIdx.y * tstride;
   const cd t = ADJ ? cconj(tau[j]) : tau[j];
   if (t.re == 0.0 && t.im == 0.0) return;  // uniform over the workgroup
   const cd* v = V + (int64_t)j * ldv;
@@ -88,13 +97,16 @@ __global__ void __launch_bounds__(256) k_house_apply(const cd* __restrict__ V, i
   }
 }
 
-int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau) {
+int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau, int nb,
+                int64_t bstride, int64_t tstride) {
   if (r > N || ncols < r) return qb_fail(ctx, QB_E_ARG, "QR needs r <= N and ncols >= r");
+  if (nb < 1 || nb > 65535) return qb_fail(ctx, QB_E_ARG, "QR batch must be in [1, 65535]");
   for (int j = 0; j < r; ++j) {
-    hipLaunchKernelGGL(k_house, dim3(1), dim3(1024), 0, ctx->stream, A, N, N, j, tau);
+    hipLaunchKernelGGL(k_house, dim3(1, nb), dim3(1024), 0, ctx->stream, A, N, N, j, tau,
+                       bstride, tstride);
     if (ncols - j - 1 > 0)
-      hipLaunchKernelGGL(k_house_apply<true>, dim3(ncols - j - 1), dim3(256), 0, ctx->stream, A,
-                         N, A, N, N, j, j + 1, tau);
+      hipLaunchKernelGGL(k_house_apply<true>, dim3(ncols - j - 1, nb), dim3(256), 0,
+                         ctx->stream, A, N, A, N, N, j, j + 1, tau, bstride, tstride);
   }
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
@@ -104,7 +116,7 @@ int qb_house_apply_q(qb_ctx* ctx, const cd* V, int64_t N, int r, const cd* tau, 
                      int ncols) {
   for (int j = r - 1; j >= 0; --j)
     hipLaunchKernelGGL(k_house_apply<false>, dim3(ncols), dim3(256), 0, ctx->stream, V, N, Y, N,
-                       N, j, 0, tau);
+                       N, j, 0, tau, (int64_t)0, (int64_t)0);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
