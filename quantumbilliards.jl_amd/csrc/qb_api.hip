@@ -289,10 +289,11 @@ __global__ void k_kress_cols(int64_t N, int ncomp, DevGeom g, double* rk, double
 // Padded range of pairwise distances (cheb-CFIE:374-418, cheb-DLP:135-147):
 // same-component pairs skip i == j, all pairs skip r <= eps.  A block takes
 // 256 targets i against the sources j = blockIdx.y, blockIdx.y + gridDim.y,
-// ..., reduces in registers and LDS, and issues one atomic pair: one pair
-// per wavefront and source column serialises about N^2/64 atomics on two
-// addresses.  Non-negative doubles order like their bit patterns, so the
-// unsigned 64-bit atomics take the min and max of the distances.
+// ..., reduces in registers and LDS, and issues one atomic pair, so at most
+// 256 x ceil(N/256) atomics meet on the two addresses (an atomic pair per
+// wavefront and source column would be N^2/64 of them, serialised).
+// Non-negative doubles order like their bit patterns, so the unsigned 64-bit
+// atomics take the min and max of the distances.
 __global__ __launch_bounds__(256) void k_rrange(DevGeom g, int only_first,
                                                 unsigned long long* mm) {
   const int64_t N = only_first ? g.off[1] : g.N;
