@@ -191,6 +191,41 @@ def test_beyn_circle_gives_bessel_zeros(precision):
     assert np.max(np.abs(inside - ex) / ex) < 1e-12
 
 
+def test_chebyshev_params_tables_meet_tol_against_amos():
+    """The picks of the device tuner (qb_cheb_tune,
+    chebyshev_optimal_panelization.jl:221-262), started from small tables so
+    that it has to grow them, checked independently: the oracle builds the
+    four tables at the picked (np, M) and evaluates them at random radii (not
+    the tuner's uniform grid) against scipy's AMOS H0, H1, J0, J1. The
+    absolute error stays within 10 tol; a table built for another z or range
+    would miss by orders of magnitude."""
+    q = qb()
+    b, pts = _pts("star", [300])
+    solver = q.CFIE_kress(10.0, b)
+    zj, _ = O.contour(30.0, 0.5, 8)
+    tol = 1e-10
+    nh, mh, nj, mj, errs = q.chebyshev_params(solver, pts, zj, tol=tol, npanels_h_init=200,
+                                              M_h_init=5, npanels_j_init=100, M_j_init=5)
+    print("picks", (nh, mh, nj, mj), errs)
+    assert max(errs) <= tol and (nh, mh, nj, mj) != (200, 5, 100, 5)
+    rmin, rmax = O._rrange_blocks(O.make_geometry("star", [300]))
+    # H tables start at max(min_j 0.2/|z_j|, rmin) (optimal_panelization.jl:222)
+    rmin = max(rmin, float(np.min(0.2 / np.abs(zj))))
+    rng = np.random.default_rng(5)
+    rh, rj = rng.uniform(rmin, rmax, 20000), rng.uniform(0.0, rmax, 20000)
+    ph, pj = O.Panels(rmin, rmax, nh, mh), O.Panels(0.0, rmax, nj, mj)
+    worst = 0.0
+    for z in zj:
+        for kind, nu, P, r in (("h", 0, ph, rh), ("h", 1, ph, rh), ("j", 0, pj, rj),
+                               ("j", 1, pj, rj)):
+            c = O.build_table(kind, nu, z, P.rmin, P.rmax, P.npan, P.M)
+            pidx, t = P.locate(r)
+            f = O.clenshaw(c[pidx], t)
+            d = sp.hankel1(nu, z * r) if kind == "h" else sp.jv(nu, z * r)
+            worst = max(worst, float(np.max(np.abs(f - d))))
+    assert worst <= 10 * tol, worst
+
+
 def test_dlp_ellipse_full_size_sampled_columns():
     """Config 2 at full size: DLP-Kress on the ellipse (1, 0.6), N = 4096,
     z = 400 + 0.5 e^{i theta} (8 of the 64 contour nodes), tables tuned to
