@@ -167,8 +167,10 @@ int qb_bessel_eval_dev(qb_ctx* ctx, const cd* z, int64_t n, cd* out4) {
 // max |table - direct| over `samples` uniform radii, for H0, H1 on
 // [rmin_h, rmax] and J0, J1 on [0, rmax], over all wavenumbers.  Errors are
 // non-negative doubles, so their bit patterns order like unsigned integers
-// and a vector atomicMax on the bits gives the maximum.
-__global__ void k_cheb_check(const cd* __restrict__ zk, int64_t nz, int samples, double rmin_h,
+// and a vector atomicMax on the bits gives the maximum.  The 4 wavefronts of
+// a block reduce through LDS first, so one atomic per error kind and block
+// meets on each address.
+__global__ __launch_bounds__(256) void k_cheb_check(const cd* __restrict__ zk, int64_t nz, int samples, double rmin_h,
                              double rmax, int np_h, int M_h, const cd* __restrict__ H, int np_j,
                              int M_j, const cd* __restrict__ J,
                              unsigned long long* __restrict__ err4) {
@@ -197,10 +199,18 @@ __global__ void k_cheb_check(const cd* __restrict__ zk, int64_t nz, int samples,
     e[2] = cabsd(f0 - d0);
     e[3] = cabsd(f1 - d1);
   }
+  __shared__ double red[4][4];
+  const int w = threadIdx.x >> 6;
   for (int c = 0; c < 4; ++c) {
     double v = e[c];
     for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0) atomicMax(&err4[c], (unsigned long long)__double_as_longlong(v));
+    if ((threadIdx.x & 63) == 0) red[w][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int c = threadIdx.x;
+    const double v = fmax(fmax(red[0][c], red[1][c]), fmax(red[2][c], red[3][c]));
+    atomicMax(&err4[c], (unsigned long long)__double_as_longlong(v));
   }
 }
 
@@ -211,6 +221,7 @@ int qb_cheb_check_dev(qb_ctx* ctx, const DevTables& t, const cd* z_dev, int64_t 
   if (rc) return rc;
   QB_HIP(ctx, hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), ctx->stream));
   int64_t total = nz * (int64_t)samples;
+  // k_cheb_check reduces over exactly 4 wavefronts of 256 threads per block
   hipLaunchKernelGGL(k_cheb_check, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      ctx->stream, z_dev, nz, samples, t.rmin_h, t.rmax, t.np_h, t.M_h, t.H, t.np_j,
                      t.M_j, t.J, d);
