@@ -9,8 +9,12 @@ refinement, compared with the plain complex128 path on the same inputs.
 
 A step is one full window: contour nodes -> tables -> assembly of all 48
 T(z_j) -> LU of all nodes -> moment solves -> thin SVD, rank, B, eig(B).
-Eigenvalues counted: lambda with |lambda - k0| <= R whose relative
-difference from the complex128 path's matching eigenvalue is <= 1e-10.
+Eigenvalues counted: the real roots inside the disk (|lambda - k0| <= R,
+|Im lambda| <= 1e-8 |lambda|) that match a real root of the complex128 path
+within 1e-10 relative.  On the warmup window the reference's residual filter
+(imag_k_check_EXPERIMENTAL with every candidate examined) is run once and
+its kept set is reported beside the real-root set; the two are the same at
+this workload (tests/test_gpu_configs.py).
 """
 import argparse
 import json
@@ -25,7 +29,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "quantumbilliards.jl_amd")]
 
 import numpy as np  # noqa: E402
 
-PEAK = {"c64": 157.3, "c128": 78.6}  # dense MFMA TFLOP/s (fp32, fp64)
+PEAK = {"c64": 157.3, "c128": 78.6, "i8": 5000.0}  # dense MFMA TFLOP/s (fp32, fp64), int8 TOPS
 # HBM bytes per complex64 GEMM launch over one window of the default workload,
 # from FETCH_SIZE / WRITE_SIZE counter runs scaled by a copy kernel of known
 # size (profiles/beyn_window_star_N8192_mixed_pmc_traffic.txt)
@@ -49,6 +53,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-filter-check", action="store_true",
+                    help="skip the residual filter run on the warmup window")
     ap.add_argument("--inflight", type=int, default=1,
                     help="mixed windows in flight at once, each on its own device context")
     ap.add_argument("--cpu-cols", type=int, default=512)
@@ -134,9 +140,11 @@ def run_assembly(a):
     written):
       DLP-Kress, full ellipse (1, 0.6), N = 4096, 64 nodes 400 + 0.5 e^{i theta};
       CFIE-Kress, star r = 1 + 0.2 cos 5t, N = 8192, 48 nodes 800 + 0.5 e^{i theta};
-    tables tuned to 1e-12 on the device (chebyshev_params).  Algorithmic
-    bytes: 16 per entry written; FP64 flops per entry: 3(M_h+M_j)+22 (DLP),
-    6(M_h+M_j)+50 (CFIE).  Prints one JSON line per operator."""
+    tables tuned to 1e-12 on the device (chebyshev_params; the windows'
+    conjugate-pair kernel reads its own envelope tables instead, see
+    k_assemble_env).  Algorithmic bytes: 16 per entry written; FP64 flops per
+    entry: 3(M_h+M_j)+22 (DLP), 6(M_h+M_j)+50 (CFIE), the reference's count
+    for its table route.  Prints one JSON line per operator."""
     import qbhip as q
     cases = [("DLP_kress", q.make_full_ellipse(1.0, 0.6), 4096, 64, 400.0),
              ("CFIE_kress", q.make_star_polar_boundary(1.0, 0.2, 5), 8192, 48, 800.0)]
@@ -167,7 +175,10 @@ def run_assembly(a):
             "dtype": "c128", "data": "synthetic (geometry nodes)",
             "config": {"workload": name, "N": N, "nq": nq, "k0": k0, "R": 0.5, "cheb": cheb,
                        "cheb_tol": 1e-12},
-            "roofline": {"kernel": "k_assemble", "bound": "hbm", "achieved": gbs, "peak": 8000.0,
+            "roofline": {"kernel": ("k_assemble_conj (uniform tables)"
+                                    if os.environ.get("QB_ASM_TABLES") else
+                                    "k_assemble_env (envelope tables in LDS)"),
+                         "bound": "hbm", "achieved": gbs, "peak": 8000.0,
                          "unit": "GB/s", "frac": gbs / 8000.0,
                          "fp64_frac": ent * flops_per / t / 1e12 / 78.6}}), flush=True)
         geom.close()
@@ -429,10 +440,18 @@ def run_symmetry(a):
     print(json.dumps(out), flush=True)
 
 
-def match_count(lam, ref, k0, R, tol=1e-10):
-    """Eigenvalues inside the disk that match a reference eigenvalue."""
+REAL_TOL = 1e-8  # |Im lambda| <= REAL_TOL |lambda|: a real root (qbhip.serve.REAL_TOL)
+
+
+def match_count(lam, ref, k0, R, tol=1e-10, real_only=False):
+    """Eigenvalues inside the disk that match a reference eigenvalue (with
+    real_only: real roots of both sets only).  Returns (matched, worst
+    relative difference, candidates inside the disk)."""
     lam = lam[np.abs(lam - k0) <= R]
     ref = ref[np.abs(ref - k0) <= R + 1e-6]
+    if real_only:
+        lam = lam[np.abs(lam.imag) <= REAL_TOL * np.abs(lam)]
+        ref = ref[np.abs(ref.imag) <= REAL_TOL * np.abs(ref)]
     used = np.zeros(len(ref), bool)
     n, worst = 0, 0.0
     for x in lam:
@@ -448,7 +467,7 @@ def match_count(lam, ref, k0, R, tol=1e-10):
     return n, worst, len(lam)
 
 
-def window(q, solver, pts, geom, a, cheb, precision, r, rng_seed, grow):
+def window(q, solver, pts, geom, a, cheb, precision, r, rng_seed, grow, filter_check=False):
     t0 = time.perf_counter()
     win = q.Window(solver, pts, complex(a.k0), a.R, a.nq, precision=precision, geom=geom, **cheb)
     B, Uk, info = q.construct_B_matrix(solver, pts, geom.dim, complex(a.k0), a.R, nq=a.nq, r=r,
@@ -456,12 +475,30 @@ def window(q, solver, pts, geom, a, cheb, precision, r, rng_seed, grow):
                                        precision=precision, refine_tol=a.refine_tol,
                                        grow_r=grow, window=win, return_info=True)
     te = time.perf_counter()
-    lam = np.linalg.eigvals(B) if B.size else np.zeros(0, complex)
+    if filter_check and B.size:
+        lam, Y = np.linalg.eig(B)
+    else:
+        lam = np.linalg.eigvals(B) if B.size else np.zeros(0, complex)
     info["t_host_eig_s"] = time.perf_counter() - te
     dt = time.perf_counter() - t0
     st = dict(win.stats)
     win.close()
     info["window_stats"] = st
+    if filter_check and B.size:
+        # the reference's spurious-root filter over every candidate in the
+        # disk (beyn.jl:623-721), against the real-root count of the bench
+        tf = time.perf_counter()
+        idx, resid = q.imag_k_check_EXPERIMENTAL(solver, [lam], [Uk], [Y], [complex(a.k0)],
+                                                 [a.R], [pts], res_tol=1e-6, pad=10 ** 6,
+                                                 group_size=64, geoms=[geom], **cheb)
+        ins = np.flatnonzero(np.abs(lam - a.k0) <= a.R)
+        real = ins[np.abs(lam[ins].imag) <= REAL_TOL * np.abs(lam[ins])]
+        info["residual_filter"] = {
+            "candidates_in_disk": int(len(ins)), "kept_by_residual_filter": int(len(idx[0])),
+            "real_roots": int(len(real)),
+            "same_set": bool(set(idx[0].tolist()) == set(real.tolist())),
+            "max_kept_residual": float(np.max(resid[0])) if len(resid[0]) else None,
+            "seconds": round(time.perf_counter() - tf, 2)}
     return lam, dt, info
 
 
@@ -476,10 +513,23 @@ def cpu_baseline(a, cheb, r_used, count, nodes: int = 4):
     from oracle import oracle as O
     try:
         from threadpoolctl import threadpool_info
-        blas = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+        pools = threadpool_info()
+        blas = max([d.get("num_threads", 1) for d in pools] or [1])
+        blas_lib = ",".join(sorted({f"{d.get('internal_api')} {d.get('version')}"
+                                    for d in pools if d.get("user_api") == "blas"}))
     except Exception:
-        blas = 1
+        blas, blas_lib = 1, None
     omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    cpu_model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
     comp = O.make_geometry("star", [a.N])[0]
     cp = O.ChebParams(cheb["n_panels_h"], cheb["M_h"], cheb["n_panels_j"], cheb["M_j"])
     rmin, rmax = O._rrange_single_fast(comp)
@@ -502,6 +552,11 @@ def cpu_baseline(a, cheb, r_used, count, nodes: int = 4):
     t_win = a.nq / nodes * (t_asm + t_lu + t_sv)
     return {"value": count / t_win if t_win > 0 else None, "unit": "eigenvalues/s",
             "cores": int(max(blas, omp)), "kind": "port",
+            "cpu_model": cpu_model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OPENBLAS_NUM_THREADS": os.environ.get("OPENBLAS_NUM_THREADS"),
+            "blas_threads": int(blas), "blas": blas_lib,
+            "sampled_fraction": f"{nodes}/{a.nq} contour nodes, extrapolated",
             "sample": (f"{nodes} contour nodes of the same window: AMOS tables + C/OpenMP "
                        f"assembly ({t_asm:.1f}s) + zgetrf ({t_lu:.1f}s) + zgetrs with {r_used} "
                        f"probes ({t_sv:.1f}s); x{a.nq}/{nodes} -> {t_win:.0f} s per window, "
@@ -614,6 +669,10 @@ def main():
         return
     import qbhip as q
     ctx = q.default_context()
+    # one context runs every window of this process: let its buffer cache
+    # keep a whole mixed window (matrices, int8 residues, solve buffers), so
+    # windows of one size allocate nothing after the first
+    ctx.set_cache_limit(224 << 30)
     b = q.make_star_polar_boundary(1.0, 0.2, 5)
     solver = q.CFIE_kress(10.0, b)
     pts = q.evaluate_points(solver, b, a.k0, N=[a.N])
@@ -626,11 +685,15 @@ def main():
     # probes from Weyl's law for the captured interval; warmup doubles r if
     # the rank still saturates and keeps the r that does not
     r_used = a.r or q.probe_count(b, a.k0, a.R, a.nq, a.svd_tol, a.r0)
+    filt = None
     for w in range(max(1, a.warmup)):
-        lam_m, dt, info = window(q, solver, pts, geom, a, cheb, "mixed", r_used, 1, a.r == 0)
+        last = w == max(1, a.warmup) - 1
+        lam_m, dt, info = window(q, solver, pts, geom, a, cheb, "mixed", r_used, 1, a.r == 0,
+                                 filter_check=last and not a.no_filter_check)
         r_used = info["r"]
+        filt = info.get("residual_filter", filt)
         print(f"# warmup {w}: {dt:.2f}s rk={info['rk']} r={r_used} svd={info['t_host_svd_s']:.2f}s "
-              f"eig={info['t_host_eig_s']:.2f}s stats={info['window_stats']}",
+              f"eig={info['t_host_eig_s']:.2f}s stats={info['window_stats']} filter={filt}",
               file=sys.stderr, flush=True)
     rep = ctx.autotune(a.N, a.nq, r_used, "fp64") + ctx.autotune(a.N, a.nq, r_used, "mixed")
     print("# gemm autotune:\n" + rep, file=sys.stderr, flush=True)
@@ -653,16 +716,19 @@ def main():
     ctx.set_profiling(True)
     res, t_span = pipelined(q, ctx, solver, pts, geom, a, cheb, "mixed", V, a.steps,
                             lanes=lanes)
-    lat, counts, worst = [], [], 0.0
+    lat, counts, cands, worst = [], [], [], 0.0
     for s in range(a.steps):
         lam, rk, t_start, t_end, t_dev, t_host, st = res[s]
         lat.append(t_end - t_start)
+        nin = int(np.count_nonzero(np.abs(lam - a.k0) <= a.R))
+        nreal = int(np.count_nonzero((np.abs(lam - a.k0) <= a.R)
+                                     & (np.abs(lam.imag) <= REAL_TOL * np.abs(lam))))
         if ref is not None:
-            n, wv, nin = match_count(lam, ref, a.k0, a.R)
+            n, wv, _ = match_count(lam, ref, a.k0, a.R, real_only=True)
         else:
-            nin = int(np.count_nonzero(np.abs(lam - a.k0) <= a.R))
-            n, wv = nin, float("nan")
+            n, wv = nreal, float("nan")
         counts.append(n)
+        cands.append(nin)
         worst = max(worst, wv) if not math.isnan(wv) else worst
         print(f"# step {s}: latency {t_end - t_start:.3f}s (device {t_dev:.3f}s, host {t_host:.3f}s) "
               f"eig {n}/{nin} worst {wv:.2e} rk={rk} stats={st}", file=sys.stderr, flush=True)
@@ -671,6 +737,9 @@ def main():
     g = ks["gemm_c64"]
     avg_ms = g["ms"] / max(g["launches"], 1)
     achieved = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    issued = g["issued"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    oz = ks["ozaki"]
+    oz_rate = oz["issued"] / (oz["ms"] * 1e-3) / 1e12 if oz["ms"] > 0 else 0.0
     default_workload = (a.N, a.nq, a.k0, a.R, r_used) == (8192, 48, 800.0, 0.5, 768)
     mean = t_span / a.steps
     value = float(np.sum(counts) / t_span)
@@ -690,6 +759,12 @@ def main():
                      f"{a.inflight} windows in flight on separate device contexts; host SVD/eig "
                      "of each window overlaps the device work"),
         "eigenvalues_per_window": int(np.median(counts)),
+        "counted": ("real roots inside the disk (|Im k| <= 1e-8 |k|) matching a real root of "
+                    "the complex128 path within 1e-10"),
+        "candidates_in_disk_per_window": int(np.median(cands)),
+        "residual_filter_on_warmup_window": filt,
+        "refine_resid_max": max(float(res[s][6].get("refine_resid_max", 0.0))
+                                for s in range(a.steps)),
         "max_rel_eig_err_vs_fp64": worst if ref is not None else None,
         "fp64_window_s": fp64_s,
         "speedup_vs_fp64_path": (fp64_s / mean) if fp64_s else None,
@@ -700,7 +775,13 @@ def main():
                      "frac": achieved / PEAK["c64"], "avg_launch_ms": avg_ms,
                      "traffic": TRAFFIC_C64_PER_LAUNCH if default_workload else None,
                      "alg_bytes_per_launch": g["bytes"] / max(g["launches"], 1),
-                     "flops_per_launch": g["flops"] / max(g["launches"], 1)},
+                     "flops_per_launch": g["flops"] / max(g["launches"], 1),
+                     "issued_tflops": issued, "issued_frac": issued / PEAK["c64"]},
+        "roofline_int8_residual": {
+            "kernel": "k_oz_gemm (refinement residual, int8 MFMA)", "bound": "mfma",
+            "achieved": oz_rate, "peak": PEAK["i8"], "unit": "TOPS", "frac": oz_rate / PEAK["i8"],
+            "launches": oz["launches"],
+            "avg_launch_ms": oz["ms"] / max(oz["launches"], 1)} if oz["launches"] else None,
     }
     if not a.no_cpu:
         try:

@@ -35,11 +35,40 @@ enum {
 /* operators: the two Kress solver families of the reference */
 enum { QB_CFIE_KRESS = 1, QB_DLP_KRESS = 2 };
 
-/* precision of the Beyn LU/solves */
-enum { QB_FP64 = 0, QB_MIXED = 1 };
+/* precision of the Beyn LU/solves
+ *  QB_FP64           complex128 LU and solves
+ *  QB_MIXED          complex64 LU on MFMA, iterative refinement against an
+ *                    FP64 residual formed exactly on the int8 matrix cores
+ *                    from residues of T kept with the window (56 bytes per
+ *                    matrix entry in all, 24 without the residues)
+ *  QB_MIXED_FP64RES  the same refinement with the residual as a complex128
+ *                    GEMM (24 bytes per matrix entry) */
+enum { QB_FP64 = 0, QB_MIXED = 1, QB_MIXED_FP64RES = 2 };
 
-/* flags for qb_assemble */
-enum { QB_OUT_DEVICE = 1, QB_DERIV_REFERENCE_SIGN = 2 };
+/* flags for qb_assemble / qb_assemble_deriv
+ *  QB_OUT_DEVICE           out (and dout, ddout) are device pointers
+ *  QB_DERIV_REFERENCE_SIGN qb_assemble_deriv: the reference's sign of the
+ *                          between-component double-layer derivative terms
+ *                          (cheb-CFIE:1031-1032); the default, kept as an
+ *                          explicit spelling of it
+ *  QB_ASM_CONJ_PAIRS       qb_assemble: take the conjugate-pair kernel that
+ *                          Beyn windows with real k0 use: both matrices of a
+ *                          pair z, conj(z) together, H_n from envelope tables
+ *                          of H_n(zr) e^{-izr} (quarter-octave Chebyshev
+ *                          panels of degree 12, ~1e-14 relative; the table
+ *                          sizes in cheb are then not used) and
+ *                          J_n(zr) = (H_n(zr) + conj H_n(conj(z) r)) / 2;
+ *                          QB_E_ARG when the nodes are not closed under
+ *                          conjugation or the geometry has symmetry orbits
+ *  QB_DERIV_MATRIX_SIGN    qb_assemble_deriv: the wavenumber derivative of the
+ *                          matrix between components (the opposite sign of
+ *                          the reference's two terms) */
+enum {
+  QB_OUT_DEVICE = 1,
+  QB_DERIV_REFERENCE_SIGN = 2,
+  QB_ASM_CONJ_PAIRS = 4,
+  QB_DERIV_MATRIX_SIGN = 8
+};
 
 /* Chebyshev table sizes: the n_panels_h/M_h/n_panels_j/M_j keywords of
  * construct_boundary_matrices! (chebyshev_euclidian_helmholtz_cfie_kress.jl:1431,
@@ -98,7 +127,9 @@ enum {
   QB_K_PANEL = 4,     /* leaf panel factorisation              */
   QB_K_TRSM = 5,      /* triangular-solve leaves               */
   QB_K_OTHER = 6,     /* swaps, gathers, norms, moments        */
-  QB_K_COUNT = 7
+  QB_K_OZAKI = 7,     /* int8 products of the refinement residual (MFMA) */
+  QB_K_GEMM_C64_BF16 = 8, /* complex64 C -= A*B on bf16 MFMA (split operands) */
+  QB_K_COUNT = 9
 };
 /* Enable (1) / disable (0) recording a pair of HIP events on the context's
  * stream around every launch, and clear the counters.  No counterpart in the
@@ -106,9 +137,19 @@ enum {
 int qb_ctx_set_profiling(qb_ctx* ctx, int32_t enable);
 /* Totals since profiling was enabled for one kernel class: device time (ms,
  * from the events), launches, and algorithmic flops / bytes of those
- * launches (complex GEMM: 8 M N K flops; assembly: 16 B per entry written). */
+ * launches (complex GEMM: 8 M N K flops; assembly: 16 B per entry written;
+ * int8 residual products: 2 ops per int8 multiply-add, 64 real products of
+ * M N K per complex product, and the int8 operand bytes). */
 int qb_ctx_kernel_stats(qb_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches,
                         double* flops, double* bytes);
+/* Operations the launches of one class issued to the matrix cores since
+ * profiling was enabled: for the complex GEMMs 6 M N K per launch with the
+ * three-product (Gauss) tiles and 8 M N K otherwise (the algorithmic count
+ * of qb_ctx_kernel_stats is always 8 M N K); for QB_K_OZAKI the int8
+ * operations of the padded products as launched; for QB_K_GEMM_C64_BF16 the
+ * bf16 operations (48 M N K: 4 real products of 6 split products each).  No
+ * reference counterpart. */
+int qb_ctx_kernel_issued(qb_ctx* ctx, int32_t kind, double* issued);
 
 /* Tile autotuning: times each MFMA GEMM tile variant on the shapes a window
  * of size N with nq nodes and r probes launches (LU trailing/panel updates,
@@ -185,9 +226,11 @@ int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
  * with more probes (the r -> 2r retry of beyn.jl:291-309).  precision:
  * QB_FP64 (plain complex128 path) or QB_MIXED (complex64 LU on MFMA + FP64
  * iterative refinement against the resident FP64 T).  With k0 real the nodes
- * come in conjugate pairs, and the window assembles each pair from the two
- * nodes' Hankel tables, J_n(kr) = (H_n(kr) + conj H_n(conj(k) r)) / 2, instead
- * of reading the J tables (the same values to the tables' tolerance). */
+ * come in conjugate pairs, and the window assembles each pair together with
+ * the kernel of QB_ASM_CONJ_PAIRS (envelope tables in LDS; H_n to ~1e-14
+ * relative, J_n = (H_n(kr) + conj H_n(conj(k) r)) / 2), where cheb's table
+ * sizes are not used; other contours use the uniform H and J tables of cheb,
+ * as the reference does. */
 int qb_window_create(qb_ctx* ctx, const qb_geom* g, int32_t op, double k0_re,
                      double k0_im, double R, int32_t nq,
                      const qb_cheb_params* cheb, int32_t precision,
@@ -210,6 +253,14 @@ int qb_window_create_multi(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nw
                            qb_stats* stats);
 /* Moment blocks of a window: irreps x contours (1 for a plain window). */
 int32_t qb_window_blocks(const qb_window* w);
+/* Columns [j0, j0 + ncols) of T(z) of matrix `node` of a window, exactly as
+ * the window assembled it (node = (irrep * nwin + contour) * nq + j for
+ * contour node j), copied column-major (N x ncols, complex128) into host
+ * memory.  Only QB_MIXED windows keep the complex128 matrices resident (for
+ * the refinement residual); a QB_FP64 window holds their LU factors instead
+ * and returns QB_E_STATE.  No reference counterpart: the reference's Tbufs
+ * (beyn.jl:263) are matrices its caller can read. */
+int qb_window_matrix_cols(qb_window* w, int32_t node, int64_t j0, int64_t ncols, double* out);
 void qb_window_destroy(qb_window* w);
 /* Device bytes a window of this size would hold (the admission footprint):
  * matrices, pivots, solve buffers for r probes, and up to 32 MiB of tables
@@ -235,9 +286,10 @@ int qb_window_moments(qb_window* w, int32_t r, const double* V, double* A0,
  * _all_k_nosymm_CFIE_chebyshev_deriv!, :881-1059; DLP: cheb-DLP:1126-1144
  * with _construct_dlp_kress_matrices_derivatives_chebyshev!, :881-952, and
  * its sign flip + identity, :1097-1108).  Between boundary components the
- * default is the wavenumber derivative of the matrix; with
- * QB_DERIV_REFERENCE_SIGN in flags the two double-layer derivative terms take
- * the reference's opposite sign (cheb-CFIE:1031-1032). */
+ * two double-layer derivative terms carry the reference's sign
+ * (cheb-CFIE:1031-1032) by default, so that the drop-in returns the
+ * reference's numbers; with QB_DERIV_MATRIX_SIGN in flags they carry the
+ * opposite sign, which makes dT/dk the derivative of T there. */
 int qb_assemble_deriv(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z, int64_t nz,
                       const qb_cheb_params* cheb, double* out, double* dout, double* ddout,
                       int32_t flags, double* rminmax);

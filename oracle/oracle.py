@@ -342,17 +342,19 @@ def _hankel_direct(z):
 
 
 def cfie_assemble_cols(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int, rmin: float,
-                       rmax: float):
+                       rmax: float, exact: bool = False):
     """Columns [j0, j1) of the single-component CFIE matrix at one z (the
     same-block formulas of cheb-CFIE :579-615), for bounded CPU timing at
-    large N without N x N caches."""
+    large N without N x N caches.  exact=True: H0, H1, J0, J1 from AMOS at
+    every distance instead of the tables."""
     n = len(comp.x)
-    ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
-    pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
-    H0 = build_table("h", 0, z, rmin, rmax, cp.n_panels_h, cp.M_h)
-    H1 = build_table("h", 1, z, rmin, rmax, cp.n_panels_h, cp.M_h)
-    J0 = build_table("j", 0, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
-    J1 = build_table("j", 1, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+    if not exact:
+        ph = Panels(rmin, rmax, cp.n_panels_h, cp.M_h)
+        pj = Panels(0.0, rmax, cp.n_panels_j, cp.M_j)
+        H0 = build_table("h", 0, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+        H1 = build_table("h", 1, z, rmin, rmax, cp.n_panels_h, cp.M_h)
+        J0 = build_table("j", 0, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
+        J1 = build_table("j", 1, z, 0.0, rmax, cp.n_panels_j, cp.M_j)
     js = np.arange(j0, j1)
     DX = comp.x[:, None] - comp.x[None, js]
     DY = comp.y[:, None] - comp.y[None, js]
@@ -366,10 +368,18 @@ def cfie_assemble_cols(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int,
     with np.errstate(divide="ignore"):
         lt = np.log(4.0 * np.sin((comp.ts[:, None] - comp.ts[None, js]) / 2.0) ** 2)
     lt[diag] = 0.0
-    pidx, th = ph.locate(r)
-    pjx, tj = pj.locate(r)
-    h0, h1 = clenshaw(H0[pidx], th), clenshaw(H1[pidx], th)
-    j0v, j1v = clenshaw(J0[pjx], tj), clenshaw(J1[pjx], tj)
+    if exact:
+        h0, h1 = sp.hankel1(0, z * r), sp.hankel1(1, z * r)
+        j0v, j1v = sp.jv(0, z * r), sp.jv(1, z * r)
+    else:
+        pidx, th = ph.locate(r)
+        pjx, tj = pj.locate(r)
+        h0, h1 = clenshaw(H0[pidx], th), clenshaw(H1[pidx], th)
+        j0v, j1v = clenshaw(J0[pjx], tj), clenshaw(J1[pjx], tj)
+        # |k r| < 0.2: direct evaluation (h0_h1_j0_j1_multi_ks_at_r!, bessels:1155-1160)
+        dm = (np.abs(z * r) < H_CUT) & ~diag
+        if np.any(dm):
+            h0[dm], h1[dm] = _hankel_direct(z * r[dm])
     sj = np.sqrt(comp.dx[js] ** 2 + comp.dy[js] ** 2)[None, :]
     wj = comp.ws[js][None, :]
     cD = inner * invr
@@ -464,11 +474,17 @@ def cfie_cols_c(comp: Comp, z: complex, cp: ChebParams, j0: int, j1: int, rmin: 
     return out.T
 
 
-def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None, cols=None):
+def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None, cols=None,
+                  exact: bool = False):
     """construct_boundary_matrices! for CFIE_kress (cheb-CFIE:1431-1450 ->
     _all_k_nosymm_CFIE_chebyshev! :543-665).  Returns [nz, N, N] with
-    A(z) = I - (D + i z S), row = target, column = source."""
+    A(z) = I - (D + i z S), row = target, column = source.  exact=True takes
+    H0, H1, J0, J1 from AMOS at every distance instead of the Chebyshev
+    tables (the same formulas; the tables' interpolation error removed) --
+    the exact-Hankel route of cfie_kress.jl:416-510 carried to complex z."""
     zs = np.asarray(zs, dtype=np.complex128)
+    if exact:
+        return _cfie_assemble_exact(comps, zs)
     if cols is not None:
         assert len(comps) == 1
         c = comps[0]
@@ -552,6 +568,67 @@ def cfie_assemble(comps, zs, cp: ChebParams = ChebParams(), tables=None, cols=No
                     cS = wj * sjb
                     A = -(cD * aL2 * h1 + ik * (cS * aM2 * h0))
                     out[q, ro:ro + len(ca.x), co:co + len(cb.x)] = A
+    return out
+
+
+def _cfie_assemble_exact(comps, zs):
+    offs = np.concatenate([[0], np.cumsum([len(c.x) for c in comps])])
+    N = int(offs[-1])
+    out = np.zeros((len(zs), N, N), np.complex128)
+    caches = [geom_cache(c) for c in comps]
+    for q, k in enumerate(zs):
+        aL1, aL2, aM1, aM2, ik = -k * INV_TWO_PI, 0.5j * k, -INV_TWO_PI, 0.5j, 1j * k
+        for a, ca in enumerate(comps):
+            for b, cb in enumerate(comps):
+                ro, co = offs[a], offs[b]
+                if a == b:
+                    G = caches[a]
+                    n = len(ca.x)
+                    Rk = kress_R(n)
+                    z = k * G.R
+                    h0, h1 = sp.hankel1(0, z), sp.hankel1(1, z)
+                    j0, j1 = sp.jv(0, z), sp.jv(1, z)
+                    sj = G.speed[None, :]
+                    wj = cb.ws[None, :]
+                    cD = G.inner * G.invR
+                    lt = G.logterm
+                    D = (Rk * cD) * (aL1 * j1) + (wj * cD) * (aL2 * h1) - (wj * lt * cD) * (aL1 * j1)
+                    S = (Rk * sj) * (aM1 * j0) + (wj * sj) * (aM2 * h0) - (wj * sj * lt) * (aM1 * j0)
+                    A = -(D + ik * S)
+                    s_ = G.speed
+                    m2 = ((0.5j - EULER_OVER_PI) - INV_TWO_PI * np.log((k * k / 4.0) * (s_ * s_))) * s_
+                    sval = np.diag(Rk) * (aM1 * s_) + ca.ws * m2
+                    A[np.arange(n), np.arange(n)] = 1.0 - (ca.ws * G.kappa + ik * sval)
+                    out[q, ro:ro + n, co:co + n] = A
+                else:
+                    DX = ca.x[:, None] - cb.x[None, :]
+                    DY = ca.y[:, None] - cb.y[None, :]
+                    r = np.hypot(DX, DY)
+                    inn = cb.dy[None, :] * DX - cb.dx[None, :] * DY
+                    h0, h1 = sp.hankel1(0, k * r), sp.hankel1(1, k * r)
+                    sjb = np.sqrt(cb.dx ** 2 + cb.dy ** 2)[None, :]
+                    wj = cb.ws[None, :]
+                    A = -((wj * inn / r) * aL2 * h1 + ik * ((wj * sjb) * aM2 * h0))
+                    out[q, ro:ro + len(ca.x), co:co + len(cb.x)] = A
+    return out
+
+
+def dlp_assemble_exact(comp: Comp, zs):
+    """dlp_assemble with H1, J1 from AMOS at every distance (no tables)."""
+    zs = np.asarray(zs, dtype=np.complex128)
+    G = geom_cache(comp)
+    n = len(comp.x)
+    Rk = kress_R(n)
+    out = np.zeros((len(zs), n, n), np.complex128)
+    cD = G.inner * G.invR
+    w = comp.ws
+    for q, k in enumerate(zs):
+        z = k * G.R
+        h1, j1 = sp.hankel1(1, z), sp.jv(1, z)
+        l1 = (-k * INV_TWO_PI) * j1
+        D = (Rk * cD) * l1 + (w[None, :] * cD) * ((0.5j * k) * h1) - (w[None, :] * G.logterm * cD) * l1
+        D[np.arange(n), np.arange(n)] = w * G.kappa
+        out[q] = np.eye(n) - D
     return out
 
 
@@ -829,6 +906,30 @@ def cauchy_derivatives(assemble, z: complex, rho: float = 0.25, n: int = 24):
     d1 = np.tensordot(np.exp(-1j * th), F, axes=(0, 0)) / (n * rho)
     d2 = 2 * np.tensordot(np.exp(-2j * th), F, axes=(0, 0)) / (n * rho * rho)
     return d1, d2
+
+
+def dlp_matrix_direct_real(comp: Comp, k: float):
+    """construct_dlp_matrix! (dlp_kress.jl:802-828): the Kress-split
+    double-layer matrix D at real k with exact Hankels, J_1 = real(H_1);
+    D_ii = w_i kappa_i."""
+    G = geom_cache(comp)
+    n = len(comp.x)
+    Rk = kress_R(n)
+    h1 = sp.hankel1(1, k * G.R)
+    j1 = h1.real
+    cD = G.inner * G.invR
+    w = comp.ws[None, :]
+    l1 = (-k * INV_TWO_PI) * cD * j1
+    l2 = (0.5j * k) * cD * h1 - l1 * G.logterm
+    D = Rk * l1 + w * l2
+    D[np.arange(n), np.arange(n)] = comp.ws * G.kappa
+    return D
+
+
+def dlp_direct_real(comp: Comp, k: float):
+    """construct_fredholm_matrix! (dlp_kress.jl:906-932): F = I - D with D as
+    in dlp_matrix_direct_real, at real k with exact Hankels."""
+    return np.eye(len(comp.x)) - dlp_matrix_direct_real(comp, k)
 
 
 def cfie_direct_real(comps, k: float):

@@ -233,6 +233,7 @@ int qb_ctx_set_profiling(qb_ctx* ctx, int32_t enable) {
     ctx->prof_n[k] = 0;
     ctx->prof_flops[k] = 0;
     ctx->prof_bytes[k] = 0;
+    ctx->prof_issued[k] = 0;
   }
   ctx->prof = enable != 0;
   return QB_OK;
@@ -248,6 +249,13 @@ int qb_ctx_kernel_stats(qb_ctx* ctx, int32_t kind, double* total_ms, int64_t* la
   if (launches) *launches = ctx->prof_n[kind];
   if (flops) *flops = ctx->prof_flops[kind];
   if (bytes) *bytes = ctx->prof_bytes[kind];
+  return QB_OK;
+}
+
+int qb_ctx_kernel_issued(qb_ctx* ctx, int32_t kind, double* issued) {
+  if (!ctx) return QB_E_ARG;
+  if (kind < 0 || kind >= QB_K_COUNT) return qb_fail(ctx, QB_E_ARG, "unknown kernel class");
+  if (issued) *issued = ctx->prof_issued[kind];
   return QB_OK;
 }
 
@@ -679,14 +687,29 @@ extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const doub
   bool on_dev = (flags & QB_OUT_DEVICE) != 0;
   hipError_t e = hipMemcpyAsync(zd, z, nz * sizeof(cd), hipMemcpyHostToDevice, ctx->stream);
   if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
-  if (!rc) rc = qb_tables_build(ctx, (const double*)zd, nz, rmin, rmax, *cheb, true, &t);
+  // the conjugate-pair kernel of the Beyn windows reads envelope tables of
+  // its own (QB_ASM_TABLES=1: the uniform Hankel tables, as the windows then)
+  const bool pairs = (flags & QB_ASM_CONJ_PAIRS) != 0;
+  const bool env = pairs && getenv("QB_ASM_TABLES") == nullptr;
+  if (!rc && !env) rc = qb_tables_build(ctx, (const double*)zd, nz, rmin, rmax, *cheb, true, &t);
   if (!rc) {
     if (on_dev)
       od = (cd*)out;
     else
       rc = qb_dev_alloc(ctx, (void**)&od, obytes);
   }
-  if (!rc) rc = qb_assemble_dev(ctx, g, op, zd, nz, t, od, nullptr);
+  if (!rc) {
+    if (pairs) {
+      rc = env ? qb_assemble_env_dev(ctx, g, op, (const cd*)z, zd, nz, rmin, rmax, od, nullptr)
+               : qb_assemble_conj_dev(ctx, g, op, (const cd*)z, zd, nz, t, od, nullptr);
+      if (rc == 1)
+        rc = qb_fail(ctx, QB_E_ARG,
+                     "QB_ASM_CONJ_PAIRS needs nodes closed under conjugation and no symmetry "
+                     "orbits (with QB_ASM_TABLES: tables with M_h = M_j in {5, 7})");
+    } else {
+      rc = qb_assemble_dev(ctx, g, op, zd, nz, t, od, nullptr);
+    }
+  }
   if (!rc && !on_dev) {
     e = hipMemcpyAsync(out, od, obytes, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
@@ -714,6 +737,9 @@ extern "C" int qb_assemble_deriv(qb_ctx* ctx, const qb_geom* g, int32_t op, cons
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   int rc = check_cheb(ctx, cheb);
   if (rc) return rc;
+  if ((flags & QB_DERIV_REFERENCE_SIGN) && (flags & QB_DERIV_MATRIX_SIGN))
+    return qb_fail(ctx, QB_E_ARG,
+                   "QB_DERIV_REFERENCE_SIGN and QB_DERIV_MATRIX_SIGN exclude each other");
   if (nz == 0) return QB_OK;
   (void)hipSetDevice(ctx->device);
   double rmin, rmax;
@@ -742,7 +768,7 @@ extern "C" int qb_assemble_deriv(qb_ctx* ctx, const qb_geom* g, int32_t op, cons
   }
   if (!rc)
     rc = qb_assemble_deriv_dev(ctx, g, op, zd, nz, t, od[0], od[1], od[2],
-                               (flags & QB_DERIV_REFERENCE_SIGN) != 0);
+                               (flags & QB_DERIV_MATRIX_SIGN) == 0);
   for (int m = 0; m < 3 && !rc && !on_dev; ++m)
     if (hipMemcpyAsync(host[m], od[m], obytes, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
       rc = qb_fail(ctx, QB_E_HIP, "download failed");
@@ -890,6 +916,10 @@ struct qb_window {
   int* fb_ipiv = nullptr;
   int* fb_perm = nullptr;
   size_t fb_cap = 0;
+  // residues of T for the int8 (Ozaki) refinement residual (QB_MIXED):
+  // [nq][16][2][Np][Np] int8 and the row exponents [nq][N]
+  signed char* ozA = nullptr;
+  int* ozE = nullptr;
   cd* qr = nullptr;      // [A0 | A1] after qb_window_moments_qr (N x 2r)
   cd* qr_tau = nullptr;  // [r]
   size_t qr_cap = 0;     // elements of qr
@@ -897,10 +927,20 @@ struct qb_window {
   qb_stats st{};
 };
 
+// The int8 (Ozaki) residual of the mixed-precision refinement: on unless
+// QB_RESID_FP64 is set (then the residual is a complex128 GEMM), for N that
+// its exponent range covers.
+static bool oz_enabled(int64_t N) { return N <= 65535 && getenv("QB_RESID_FP64") == nullptr; }
+// refinement nodes whose residual runs in one int8 launch (bounds the step buffers)
+#define QB_OZ_SLOTS 16
+
 static size_t win_bytes_core(int64_t N, int nq, int precision) {
   size_t NN = (size_t)N * N;
   size_t b = (size_t)nq * NN * sizeof(cd);
-  if (precision == QB_MIXED) b += (size_t)nq * NN * sizeof(cf);
+  if (precision == QB_MIXED || precision == QB_MIXED_FP64RES) {
+    b += (size_t)nq * NN * sizeof(cf);
+    if (precision == QB_MIXED && oz_enabled(N)) b += qb_oz_window_bytes(N, nq);
+  }
   b += 2 * (size_t)nq * N * sizeof(int) + (size_t)nq * sizeof(int);
   return b;
 }
@@ -912,7 +952,11 @@ extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision,
   // per node in mixed mode, V, A0, A1, and the QR block [A0 | A1] that
   // qb_window_moments_qr keeps
   b += 2 * (size_t)nq * nv * sizeof(cd) + 5 * nv * sizeof(cd);
-  if (precision == QB_MIXED) b += (size_t)nq * nv * sizeof(cf);
+  if (precision == QB_MIXED || precision == QB_MIXED_FP64RES) {
+    b += (size_t)nq * nv * sizeof(cf);
+    if (precision == QB_MIXED && oz_enabled(N))
+      b += qb_oz_step_bytes(N, r, nq < QB_OZ_SLOTS ? nq : QB_OZ_SLOTS);
+  }
   // Chebyshev tables, bounded by 1.2e5 panels of degree 7 per node
   b += (size_t)nq * (32u << 20);
   return (int64_t)b;
@@ -935,6 +979,10 @@ static void win_free(qb_window* w) {
   qb_dev_free(ctx, w->fb_perm, w->fb_cap * w->N * sizeof(int));
   qb_dev_free(ctx, w->qr, w->qr_cap * sizeof(cd));
   qb_dev_free(ctx, w->qr_tau, (size_t)w->qr_rcap * sizeof(cd));
+  if (w->ozA) {
+    qb_dev_free(ctx, w->ozA, qb_oz_window_bytes(w->N, w->nq) - (size_t)w->nq * w->N * sizeof(int));
+    qb_dev_free(ctx, w->ozE, (size_t)w->nq * w->N * sizeof(int));
+  }
 }
 
 extern "C" void qb_window_destroy(qb_window* w) {
@@ -972,10 +1020,13 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   if (nq < 1 || nq > 4096) return qb_fail(ctx, QB_E_ARG, "nq must be in [1, 4096]");
-  if (precision != QB_FP64 && precision != QB_MIXED)
-    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64 or QB_MIXED");
+  if (precision != QB_FP64 && precision != QB_MIXED && precision != QB_MIXED_FP64RES)
+    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64, QB_MIXED or QB_MIXED_FP64RES");
   int rc = check_cheb(ctx, cheb);
   if (rc) return rc;
+  // the int8 residual of QB_MIXED, or the complex128 GEMM of QB_MIXED_FP64RES
+  const bool oz = precision == QB_MIXED && oz_enabled(qb_dim(g));
+  if (precision == QB_MIXED_FP64RES) precision = QB_MIXED;
   (void)hipSetDevice(ctx->device);
   qb_window* w = new qb_window();
   w->ctx = ctx;
@@ -1039,17 +1090,22 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
     goto fail;
   }
   // tables + assembly (construct_boundary_matrices!, beyn.jl:264)
-  QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nz, rmin, rmax, *cheb, true, &tab));
   {
-    // conjugate node pairs share one Hankel gather per entry when the
-    // contour is symmetric about the real axis (real k0)
-    int crc = getenv("QB_ASM_NO_CONJ")
-                  ? 1
-                  : qb_assemble_conj_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, tab, w->T64,
-                                         precision == QB_MIXED ? w->LU32 : nullptr);
-    if (crc == 1)
-      crc = qb_assemble_dev(ctx, g, op, w->z_dev, nz, tab, w->T64,
-                            precision == QB_MIXED ? w->LU32 : nullptr);
+    // With real k0 the contour nodes come in conjugate pairs, assembled
+    // together from envelope tables of H_n e^{-ikr} held in LDS (no uniform
+    // tables).  QB_ASM_TABLES=1: the same pairs from the uniform Hankel
+    // tables; QB_ASM_NO_CONJ=1: every node from its own H and J tables.
+    cf* o32 = precision == QB_MIXED ? w->LU32 : nullptr;
+    const bool no_conj = getenv("QB_ASM_NO_CONJ") != nullptr;
+    int crc = 1;
+    if (!no_conj && !getenv("QB_ASM_TABLES"))
+      crc = qb_assemble_env_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, rmin, rmax, w->T64, o32);
+    if (crc == 1) {
+      QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nz, rmin, rmax, *cheb, true, &tab));
+      crc = no_conj ? 1 : qb_assemble_conj_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, tab,
+                                               w->T64, o32);
+      if (crc == 1) crc = qb_assemble_dev(ctx, g, op, w->z_dev, nz, tab, w->T64, o32);
+    }
     QB_TRY(crc);
   }
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -1057,6 +1113,13 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
     goto fail;
   }
   qb_tables_free(ctx, &tab);
+  if (oz) {
+    // residues of every T(z_j) for the int8 refinement residual
+    const size_t ebytes = (size_t)nq * N * sizeof(int);
+    QB_TRY(qb_dev_alloc(ctx, (void**)&w->ozA, qb_oz_window_bytes(N, nq) - ebytes));
+    QB_TRY(qb_dev_alloc(ctx, (void**)&w->ozE, ebytes));
+    QB_TRY(qb_oz_prepare(ctx, w->T64, N, nq, w->ozA, w->ozE));
+  }
   t1 = now_ms();
   // LU of every node (beyn.jl:270-277)
   if (precision == QB_FP64) {
@@ -1105,6 +1168,27 @@ extern "C" int qb_window_create_multi(qb_ctx* ctx, const qb_geom* g, int32_t op,
 }
 
 extern "C" int32_t qb_window_blocks(const qb_window* w) { return w ? w->nirr * w->nwin : 0; }
+
+extern "C" int qb_window_matrix_cols(qb_window* w, int32_t node, int64_t j0, int64_t ncols,
+                                     double* out) {
+  if (!w) return QB_E_ARG;
+  qb_ctx* ctx = w->ctx;
+  if (!out) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (node < 0 || node >= w->nq) return qb_fail(ctx, QB_E_ARG, "node out of range");
+  if (j0 < 0 || ncols < 0 || j0 + ncols > w->N)
+    return qb_fail(ctx, QB_E_ARG, "column range out of [0, N)");
+  if (w->precision != QB_MIXED)
+    return qb_fail(ctx, QB_E_STATE,
+                   "an FP64 window holds the LU factors, not T(z): only mixed windows keep T");
+  if (ncols == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  const size_t NN = (size_t)w->N * w->N;
+  QB_HIP(ctx, hipMemcpyAsync(out, w->T64 + (size_t)node * NN + (size_t)j0 * w->N,
+                             (size_t)ncols * w->N * sizeof(cd), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  QB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return QB_OK;
+}
 
 // FP64 refactorisation of the nodes listed in w->fb_nodes that are not yet
 // factored (mixed-precision fallback when refinement does not converge).
@@ -1179,6 +1263,9 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
   cd *Vd = nullptr, *X = nullptr, *Rr = nullptr, *A0d = nullptr, *A1d = nullptr;
   cf* D32 = nullptr;
   int* actd = nullptr;
+  signed char *ozB = nullptr, *ozC = nullptr;  // int8 residual step buffers
+  int* ozF = nullptr;
+  size_t oz_bc = 0, oz_f = 0;
   int rc = QB_OK;
   int iters_max = 0;
   double resid_max = 0.0;
@@ -1203,6 +1290,15 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
     // complex64 solve + FP64 iterative refinement against the resident T64
     QB_TRY(qb_dev_alloc(ctx, (void**)&Rr, (size_t)nq * nv * sizeof(cd)));
     QB_TRY(qb_dev_alloc(ctx, (void**)&D32, (size_t)nq * nv * sizeof(cf)));
+    if (w->ozA) {
+      const int ns = std::min(nq, QB_OZ_SLOTS);
+      const size_t Np = (size_t)qb_oz_pad(N), rp = (size_t)qb_oz_pad(r);
+      oz_bc = 2 * 16 * 2 * rp * Np * ns;
+      oz_f = rp * ns * sizeof(int);
+      QB_TRY(qb_dev_alloc(ctx, (void**)&ozB, oz_bc / 2));
+      QB_TRY(qb_dev_alloc(ctx, (void**)&ozC, oz_bc / 2));
+      QB_TRY(qb_dev_alloc(ctx, (void**)&ozF, oz_f));
+    }
     if (hipMemsetAsync(X, 0, (size_t)nq * nv * sizeof(cd), ctx->stream) != hipSuccess) {
       rc = qb_fail(ctx, QB_E_HIP, "memset failed");
       goto fail;
@@ -1217,7 +1313,7 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       rc = qb_fail(ctx, QB_E_HIP, "norm download failed");
       goto fail;
     }
-    std::vector<double> be(nq, 1e300), prev(nq, 1e300), rho(nq, 1.0);
+    std::vector<double> be(nq, 1e300), prev(nq, 1e300);
     std::vector<char> stag(nq, 0);
     std::vector<int> stagnant;
     const int maxit = 8;
@@ -1228,18 +1324,11 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
     // one or two steps; the few close to an eigenvalue need more.
     std::vector<int> act(nq);
     for (int j = 0; j < nq; ++j) act[j] = j;
-    QB_TRY(qb_dev_alloc(ctx, (void**)&actd, 2 * nq * sizeof(int)));
-    // The last correction of a node is usually predictable: once its
-    // backward error contracts by rho per step, be * rho is known.  A node
-    // predicted below tol/10 after the current correction is checked on the
-    // first rs columns only (the probes are i.i.d., so a column subset
-    // measures the same backward error); it must come out below tol/4 there,
-    // otherwise it joins the full residual of the unpredicted nodes.
-    const int rs = r < 16 ? r : 16;
+    QB_TRY(qb_dev_alloc(ctx, (void**)&actd, nq * sizeof(int)));
     const bool trace = getenv("QB_REFINE_TRACE") != nullptr;
-    std::vector<int> P, F;
-    // residual pass over the nodes of `list` (uploaded to slot `slot` of
-    // actd) on the first `rr` columns; bn[j] = backward error per node.
+    // Residual pass over the nodes of `list` on all r columns: bn[j] = the
+    // backward error of node j.  Every node's error at exit is one of these
+    // full-column values, so refine_resid_max certifies the whole block.
     // The moments add the nodes' solutions, so their errors must be small in
     // absolute terms, not only relative to each node's own ||X_j||: a node
     // near an eigenvalue has ||X_j|| far above the others, and a normwise
@@ -1251,18 +1340,28 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
     std::vector<double> bn(nq, 0.0);
     double xref = -1.0;
     const double floor64 = 2.0 * sqrt((double)N) * 1.1102230246251565e-16;
-    auto residual_pass = [&](const std::vector<int>& list, int slot, int rr) -> int {
+    auto residual_pass = [&](const std::vector<int>& list) -> int {
       const int n = (int)list.size();
-      const int64_t nvr = N * (int64_t)rr;
-      if (hipMemcpyAsync(actd + slot * nq, list.data(), n * sizeof(int), hipMemcpyHostToDevice,
+      if (hipMemcpyAsync(actd, list.data(), n * sizeof(int), hipMemcpyHostToDevice,
                          ctx->stream) != hipSuccess)
         return qb_fail(ctx, QB_E_HIP, "node list upload failed");
-      ctx->bix = actd + slot * nq;
-      int e = qb_node_maxabs(ctx, X, nvr, nv, n, w->scratch_d + nq, nq);
-      if (!e) e = qb_bcast(ctx, Vd, Rr, nvr, nv, n);
-      if (!e)
-        e = gemm_sub<cd>(ctx, (int)N, rr, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N, nv, n);
-      if (!e) e = qb_node_maxabs(ctx, Rr, nvr, nv, n, w->scratch_d + 2 * nq, nq);
+      ctx->bix = actd;
+      int e = qb_node_maxabs(ctx, X, nv, nv, n, w->scratch_d + nq, nq);
+      if (w->ozA) {
+        // R = V - T X on the int8 matrix cores, QB_OZ_SLOTS nodes per launch
+        for (int c0 = 0; c0 < n && !e; c0 += QB_OZ_SLOTS) {
+          ctx->bix = actd + c0;
+          e = qb_oz_residual(ctx, w->ozA, w->ozE, N, X, nv, r, Vd, Rr, nv,
+                             std::min(QB_OZ_SLOTS, n - c0), ozB, ozF, ozC);
+        }
+        ctx->bix = actd;
+      } else {
+        if (!e) e = qb_bcast(ctx, Vd, Rr, nv, nv, n);
+        if (!e)
+          e = gemm_sub<cd>(ctx, (int)N, r, (int)N, w->T64, N, (int64_t)NN, X, N, nv, Rr, N, nv,
+                           n);
+      }
+      if (!e) e = qb_node_maxabs(ctx, Rr, nv, nv, n, w->scratch_d + 2 * nq, nq);
       ctx->bix = nullptr;
       if (e) return e;
       if (hipMemcpyAsync(h3.data(), w->scratch_d, 3 * nq * sizeof(double),
@@ -1298,35 +1397,16 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       QB_TRY(qb_trsm_lu_batched<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, D32, r, N, nv, na));
       QB_TRY(qb_add_cf_to_cd_nodes(ctx, D32, X, nv, nv, na));
       ctx->bix = nullptr;
-      P.clear();
-      F.clear();
-      for (int j : act)
-        (it >= 1 && be[j] * rho[j] <= 0.1 * refine_tol ? P : F).push_back(j);
-      const int n_sub = (int)P.size();
-      if (!P.empty()) {
-        QB_TRY(residual_pass(P, 1, rs));
-        for (int j : P) {
-          if (bn[j] <= 0.25 * refine_tol)
-            be[j] = bn[j];  // verified on the subset: converged
-          else
-            F.push_back(j);
-        }
-      }
-      if (!F.empty()) {
-        QB_TRY(residual_pass(F, 0, r));
-        for (int j : F) {
-          if (be[j] < 1e299) rho[j] = bn[j] / be[j];
-          prev[j] = be[j];
-          be[j] = bn[j];
-        }
+      QB_TRY(residual_pass(act));
+      for (int j : act) {
+        prev[j] = be[j];
+        be[j] = bn[j];
       }
       if (trace) {
         double bmax = 0;
         for (int j : act) bmax = std::max(bmax, be[j]);
-        fprintf(stderr,
-                "[qb refine] it %d: %d active nodes, %d full residuals, %d subset checks, max "
-                "backward error %.3e\n",
-                it, na, (int)F.size(), n_sub, bmax);
+        fprintf(stderr, "[qb refine] it %d: %d active nodes, max backward error %.3e\n", it, na,
+                bmax);
       }
       std::vector<int> next;
       for (int j : act) {
@@ -1340,7 +1420,8 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
       act.swap(next);
     }
     iters_max = it;
-    // nodes that did not reach the tolerance: FP64 refactorisation and solve
+    // nodes that did not reach the tolerance: FP64 refactorisation and solve,
+    // then one residual pass that measures their backward error as well
     for (int j = 0; j < nq; ++j)
       if (!(be[j] <= refine_tol)) stagnant.push_back(j);
     if (!stagnant.empty()) {
@@ -1356,9 +1437,10 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
         QB_TRY(qb_trsm_lu_batched<cd>(ctx, w->fb_lu + slot * NN, (int)N, N, (int64_t)NN, Xj, r,
                                       N, nv, 1));
       }
+      QB_TRY(residual_pass(stagnant));
+      for (int j : stagnant) be[j] = bn[j];
     }
-    for (int j = 0; j < nq; ++j)
-      if (be[j] <= refine_tol) resid_max = std::max(resid_max, be[j]);
+    for (int j = 0; j < nq; ++j) resid_max = std::max(resid_max, be[j]);
     w->st.nodes_fp64_fallback = (int)w->fb_nodes.size();
   }
   for (int b = 0; b < nblk; ++b) {
@@ -1388,11 +1470,14 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
 fail:
   ctx->bix = nullptr;
   (void)hipStreamSynchronize(ctx->stream);
-  qb_dev_free(ctx, actd, 2 * nq * sizeof(int));
+  qb_dev_free(ctx, actd, nq * sizeof(int));
   qb_dev_free(ctx, Vd, nv * sizeof(cd));
   qb_dev_free(ctx, X, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, Rr, (size_t)nq * nv * sizeof(cd));
   qb_dev_free(ctx, D32, (size_t)nq * nv * sizeof(cf));
+  qb_dev_free(ctx, ozB, oz_bc / 2);
+  qb_dev_free(ctx, ozC, oz_bc / 2);
+  qb_dev_free(ctx, ozF, oz_f);
   if (!dev_out) {
     qb_dev_free(ctx, A0d, (size_t)nblk * nv * sizeof(cd));
     qb_dev_free(ctx, A1d, (size_t)nblk * nv * sizeof(cd));
@@ -1615,8 +1700,9 @@ extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, in
   if (!ctx) return QB_E_ARG;
   if (N < 64 || N > 65535 || nq < 1 || r < 1 || r > N)
     return qb_fail(ctx, QB_E_ARG, "autotune needs 64 <= N <= 65535, nq >= 1, 1 <= r <= N");
-  if (precision != QB_FP64 && precision != QB_MIXED)
-    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64 or QB_MIXED");
+  if (precision != QB_FP64 && precision != QB_MIXED && precision != QB_MIXED_FP64RES)
+    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64, QB_MIXED or QB_MIXED_FP64RES");
+  if (precision == QB_MIXED_FP64RES) precision = QB_MIXED;
   (void)hipSetDevice(ctx->device);
   struct Shape {
     int M, N, K, ty;

@@ -378,17 +378,11 @@ k_assemble_conj(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ H, 
   }
 }
 
-int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
-                         const cd* z_dev, int64_t nz, const DevTables& t, cd* out64,
-                         cf* out32) {
-  // usable: plain geometry, tables of static degree 5 or 7, and every node's
-  // conjugate among the nodes (to rounding); returns 1 when not usable
-  if (g->sym.m > 0 || t.M_h != t.M_j || (t.M_h != 5 && t.M_h != 7)) return 1;
-  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return 1;
-  if (op == QB_DLP_KRESS && g->d.ncomp != 1) return 1;
-  // one-to-one pairing (several disks of one window can share a real node):
-  // a real node pairs with itself, any other node with an unused conjugate
-  std::vector<int> pr;
+// One-to-one pairing of the wavenumbers with their conjugates (several
+// disks of one window can share a real node): a real node pairs with itself,
+// any other node with an unused conjugate.  False when some node has none.
+bool qb_conj_pairs(const cd* z_host, int64_t nz, std::vector<int>* pr) {
+  pr->clear();
   std::vector<char> used(nz, 0);
   for (int64_t a = 0; a < nz; ++a) {
     if (used[a]) continue;
@@ -406,13 +400,26 @@ int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host
         }
       }
     }
-    if (b < 0) return 1;
+    if (b < 0) return false;
     used[a] = used[b] = 1;
-    pr.push_back((int)a);
-    pr.push_back((int)b);
+    pr->push_back((int)a);
+    pr->push_back((int)b);
   }
+  const int64_t npair = (int64_t)pr->size() / 2;
+  return npair >= 1 && npair <= 65535;
+}
+
+int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
+                         const cd* z_dev, int64_t nz, const DevTables& t, cd* out64,
+                         cf* out32) {
+  // usable: plain geometry, tables of static degree 5 or 7, and every node's
+  // conjugate among the nodes (to rounding); returns 1 when not usable
+  if (g->sym.m > 0 || t.M_h != t.M_j || (t.M_h != 5 && t.M_h != 7)) return 1;
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return 1;
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1) return 1;
+  std::vector<int> pr;
+  if (!qb_conj_pairs(z_host, nz, &pr)) return 1;
   int npair = (int)pr.size() / 2;
-  if (npair < 1 || npair > 65535) return 1;
   int* pd = nullptr;
   int rc = qb_dev_alloc(ctx, (void**)&pd, pr.size() * sizeof(int));
   if (rc) return rc;
@@ -448,6 +455,304 @@ int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host
   // the pair list must outlive the launch: wait before handing it back
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   qb_dev_free(ctx, pd, pr.size() * sizeof(int));
+  if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
+  return QB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Conjugate-pair assembly from envelope tables (the kernel of Beyn windows
+// with real k0)
+// ---------------------------------------------------------------------------
+// H_n^(1)(z) = e^{iz} E_n(z) with E_n smooth and non-oscillatory (~ (2 / pi
+// z)^{1/2} (1 + O(1/z)) for large z; the log / pole of H_n at z = 0 for
+// small z).  At fixed wavenumber k the envelope E_n(k r) is a Chebyshev
+// series of low degree on geometric panels: quarter octaves
+// [2^o (1 + q/4), 2^o (1 + (q+1)/4)], degree 12, about 1e-15 relative on
+// every panel (E_n is analytic in the Bernstein ellipse of a quarter octave,
+// whose nearest singularity, r = 0, lies 9 half-widths from its centre).
+// From rmin ~ 1e-3 to rmax ~ 2.6 that is 52 panels and 21 KB per wavenumber
+// for E_0 and E_1, against the ~1.2e5 panels of the uniform tables of H_n
+// itself (their oscillation e^{ikr} at k = 800 needs them), so both nodes'
+// tables of a conjugate pair sit in LDS.  Per distance: the panel from the
+// exponent and the top mantissa bits of r (exact), one Clenshaw sweep per
+// node, one sincos(Re k r) and one exp(Im k r) shared by the pair
+// (e^{i conj(k) r} = e^{i Re k r} e^{Im k r}).  H comes out to ~1e-14
+// relative, at least as accurate as the reference's uniform tables at their
+// 1e-12 tolerance; J, the |k r| < 0.2 direct branch and the entry formulas
+// are those of k_assemble_conj.
+#define QB_ENV_M 12
+#define QB_ENV_NC (QB_ENV_M + 1)
+#define QB_ENV_PMAX 128  // 32 octaves
+
+struct EnvLob {
+  double t[QB_ENV_NC];                 // cospi(j / M)
+  double C[QB_ENV_NC * QB_ENV_NC];     // C[m (M+1) + j] = cospi(j m / M)
+};
+
+// Quarter-octave panel p (0-based from octave o0) and local coordinate t of r.
+__device__ __forceinline__ void env_panel(double r, int o0, int P, int* p, double* t) {
+  const int e = ilogb(r);
+  const double m = ldexp(r, -e);  // [1, 2)
+  int q = (int)((m - 1.0) * 4.0);
+  q = q > 3 ? 3 : q;
+  int pp = 4 * (e - o0) + q;
+  pp = pp < 0 ? 0 : (pp > P - 1 ? P - 1 : pp);
+  *p = pp;
+  *t = 8.0 * m - (9.0 + 2.0 * q);
+}
+
+// Envelope tables: one thread per (wavenumber, panel); coefficients of the
+// interpolant at the M+1 Lobatto points (DCT-I with c_0 and c_M halved),
+// E_0 and E_1 interleaved: env[((q P + p) (M+1) + m) 2 + {0, 1}].
+__global__ __launch_bounds__(64) void k_env_tables(const cd* __restrict__ zk, int64_t nz, int o0,
+                                                   int P, EnvLob lc, cd* __restrict__ env) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nz * (int64_t)P) return;
+  const int64_t q = gid / P;
+  const int p = (int)(gid - q * P);
+  const int oct = o0 + p / 4, qq = p % 4;
+  const double a = ldexp(1.0 + 0.25 * qq, oct), b = ldexp(1.25 + 0.25 * qq, oct);
+  const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+  const cd k = zk[q];
+  cd f0[QB_ENV_NC], f1[QB_ENV_NC];
+#pragma unroll
+  for (int j = 0; j < QB_ENV_NC; ++j) {
+    const double r = fma(h, lc.t[j], c);
+    const cd z = mk(k.re * r, k.im * r);
+    cd h0, h1;
+    qb_hankel1_01(z, &h0, &h1);
+    double sn, cs;
+    sincos(z.re, &sn, &cs);
+    const double g = exp(z.im);
+    const cd e = mk(cs * g, -sn * g);  // e^{-i z}
+    f0[j] = h0 * e;
+    f1[j] = h1 * e;
+  }
+  cd* o = env + (q * P + p) * (int64_t)(2 * QB_ENV_NC);
+  const double s2 = 2.0 / QB_ENV_M;
+#pragma unroll
+  for (int m = 0; m <= QB_ENV_M; ++m) {
+    const double sg = (m & 1) ? -1.0 : 1.0;
+    cd s0 = 0.5 * f0[0] + 0.5 * (sg * f0[QB_ENV_M]);
+    cd s1 = 0.5 * f1[0] + 0.5 * (sg * f1[QB_ENV_M]);
+    for (int j = 1; j < QB_ENV_M; ++j) {
+      const double cc = lc.C[m * QB_ENV_NC + j];
+      s0 = s0 + f0[j] * cc;
+      s1 = s1 + f1[j] * cc;
+    }
+    const double w = (m == 0 || m == QB_ENV_M) ? 0.5 * s2 : s2;
+    o[2 * m] = w * s0;
+    o[2 * m + 1] = w * s1;
+  }
+}
+
+template <int OP>
+__global__ void __launch_bounds__(256)
+k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env, int o0, int P,
+               const int* __restrict__ pairs, int T, cd* __restrict__ out64,
+               cf* __restrict__ out32) {
+  extern __shared__ __align__(16) cd smem[];
+  const int tsz = P * 2 * QB_ENV_NC;  // complex coefficients per node
+  cd* Ta = smem;
+  cd* Tb = smem + tsz;
+  cd (*mir)[64][17] = (cd (*)[64][17])(smem + 2 * tsz);  // [node][i_local][j_local - 16 h]
+  const int64_t N = g.N;
+  const int qa = pairs[2 * blockIdx.z], qb = pairs[2 * blockIdx.z + 1];
+  const bool two = qa != qb;
+  for (int e = threadIdx.x; e < tsz; e += 256) {
+    Ta[e] = env[(int64_t)qa * tsz + e];
+    Tb[e] = env[(int64_t)qb * tsz + e];
+  }
+  int64_t p = blockIdx.x;
+  int d = 0;
+  while (p >= T - d) {
+    p -= T - d;
+    ++d;
+  }
+  const int Jt = (int)p, It = Jt + d;
+  const cd ka = zk[qa], kb = zk[qb];
+  const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
+  const int64_t i = (int64_t)It * 64 + il;
+  const bool iok = i < N;
+  const int64_t ba = (int64_t)qa * N * N, bb = (int64_t)qb * N * N;
+  const bool diag_tile = (It == Jt);
+  const int64_t ic = iok ? i : N - 1;
+  const double xi = g.x[ic], yi = g.y[ic];
+  const int ci = g.comp[ic];
+  __syncthreads();
+#pragma unroll 1
+  for (int h = 0; h < 4; ++h) {
+#pragma unroll 1
+    for (int c = 4 * h; c < 4 * h + 4; ++c) {
+      const int jl = jl0 + 4 * c;
+      const int64_t j = (int64_t)Jt * 64 + jl;
+      if (!iok || j >= N) continue;
+      cd va, vb, wa = mk(0, 0), wb = mk(0, 0);  // (i, j) and (j, i) of nodes a and b
+      if (i == j) {
+        va = diag_entry<OP>(g, j, ka);
+        vb = two ? diag_entry<OP>(g, j, kb) : va;
+      } else {
+        const double dx = xi - g.x[j], dy = yi - g.y[j];
+        const double r = hypot(dx, dy);
+        const double invr = 1.0 / r;
+        const bool same = (ci == g.comp[j]);
+        double Rij = 0, lt = 0;
+        if (same) {
+          const int64_t o = g.off[ci];
+          const int64_t nc = g.off[ci + 1] - o;
+          int64_t dd = (i - j) % nc;
+          if (dd < 0) dd += nc;
+          Rij = g.rk[o + dd];
+          lt = g.lt[o + dd];  // both circulant columns are even in (i - j)
+        }
+        Spec sa, sb;
+        const cd za = mk(ka.re * r, ka.im * r);
+        if (OP == QB_CFIE_KRESS && cabsd(za) < 0.2) {
+          // h0_h1_multi_ks_at_r! (bessels:1190-1206): |kr| < 0.2 directly
+          qb_hankel1_01(za, &sa.h0, &sa.h1);
+          if (two)
+            qb_hankel1_01(mk(kb.re * r, kb.im * r), &sb.h0, &sb.h1);
+          else
+            sb = sa;
+        } else {
+          int pp;
+          double t;
+          env_panel(r, o0, P, &pp, &t);
+          double sn, cs;
+          sincos(za.re, &sn, &cs);
+          const double ga = exp(-za.im);  // |e^{i k_a r}|; node b has k_b = conj(k_a)
+          cd e0, e1;
+          qb_clenshaw2<QB_ENV_M>(Ta + pp * (2 * QB_ENV_NC), t, &e0, &e1);
+          const cd ea = mk(cs * ga, sn * ga);
+          sa.h0 = e0 * ea;
+          sa.h1 = e1 * ea;
+          if (two) {
+            qb_clenshaw2<QB_ENV_M>(Tb + pp * (2 * QB_ENV_NC), t, &e0, &e1);
+            const double gb = 1.0 / ga;
+            const cd eb = mk(cs * gb, sn * gb);
+            sb.h0 = e0 * eb;
+            sb.h1 = e1 * eb;
+          } else {
+            sb = sa;
+          }
+        }
+        if (same || OP == QB_DLP_KRESS) {
+          sa.j0 = 0.5 * (sa.h0 + cconj(sb.h0));
+          sa.j1 = 0.5 * (sa.h1 + cconj(sb.h1));
+        } else {
+          sa.j0 = mk(0, 0);
+          sa.j1 = mk(0, 0);
+        }
+        sb.j0 = cconj(sa.j0);
+        sb.j1 = cconj(sa.j1);
+        va = offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, ka, sa);
+        vb = two ? offdiag_entry<OP>(g, i, j, dx, dy, invr, same, Rij, lt, kb, sb) : va;
+        if (!diag_tile) {
+          wa = offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, ka, sa);
+          wb = two ? offdiag_entry<OP>(g, j, i, -dx, -dy, invr, same, Rij, lt, kb, sb) : wa;
+        }
+      }
+      const int64_t e1 = i + j * N;
+      if (out64) {
+        out64[ba + e1] = va;
+        if (two) out64[bb + e1] = vb;
+      }
+      if (out32) {
+        out32[ba + e1] = to_cf(va);
+        if (two) out32[bb + e1] = to_cf(vb);
+      }
+      if (!diag_tile) {
+        mir[0][il][jl - 16 * h] = wa;
+        mir[1][il][jl - 16 * h] = wb;
+      }
+    }
+    if (diag_tile) continue;  // uniform over the block
+    __syncthreads();
+    // mirror rows j = Jt*64 + 16h + 0..15 of the 64 columns i: each group of
+    // 16 lanes writes one column's 16 contiguous rows
+#pragma unroll 1
+    for (int c = 0; c < 4; ++c) {
+      const int il2 = (threadIdx.x >> 4) + 16 * c;
+      const int jl2 = threadIdx.x & 15;
+      const int64_t ii = (int64_t)It * 64 + il2, jj = (int64_t)Jt * 64 + 16 * h + jl2;
+      if (ii >= N || jj >= N) continue;
+      const int64_t e2 = jj + ii * N;
+      const cd xa = mir[0][il2][jl2];
+      if (out64) out64[ba + e2] = xa;
+      if (out32) out32[ba + e2] = to_cf(xa);
+      if (two) {
+        const cd xb = mir[1][il2][jl2];
+        if (out64) out64[bb + e2] = xb;
+        if (out32) out32[bb + e2] = to_cf(xb);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
+                        const cd* z_dev, int64_t nz, double rmin, double rmax, cd* out64,
+                        cf* out32) {
+  // usable: plain geometry and every node's conjugate among the nodes;
+  // returns 1 when not usable
+  if (g->sym.m > 0) return 1;
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return 1;
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1) return 1;
+  if (!(rmin > 0.0) || !(rmax > rmin)) return 1;
+  std::vector<int> pr;
+  if (!qb_conj_pairs(z_host, nz, &pr)) return 1;
+  const int o0 = ilogb(rmin), o1 = ilogb(rmax);
+  const int P = 4 * (o1 - o0 + 1);
+  if (P > QB_ENV_PMAX) return 1;
+  const int npair = (int)pr.size() / 2;
+  const size_t ebytes = (size_t)nz * P * 2 * QB_ENV_NC * sizeof(cd);
+  cd* env = nullptr;
+  int* pd = nullptr;
+  int rc = qb_dev_alloc(ctx, (void**)&env, ebytes);
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&pd, pr.size() * sizeof(int));
+  if (rc) {
+    qb_dev_free(ctx, env, ebytes);
+    return rc;
+  }
+  EnvLob lc;
+  qb_lobatto_host(QB_ENV_M, lc.t, lc.C);
+  const int64_t nt = nz * (int64_t)P;
+  int slot = qb_prof_begin(ctx, QB_K_TABLES, 0.0, (double)ebytes);
+  hipLaunchKernelGGL(k_env_tables, dim3((unsigned)((nt + 63) / 64)), dim3(64), 0, ctx->stream,
+                     z_dev, nz, o0, P, lc, env);
+  qb_prof_end(ctx, slot);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(pd, pr.data(), pr.size() * sizeof(int), hipMemcpyHostToDevice,
+                       ctx->stream);
+  if (e == hipSuccess) {
+    const int64_t N = g->d.N;
+    const int T = (int)((N + 63) / 64);
+    const int64_t tpairs = (int64_t)T * (T + 1) / 2;
+    const size_t lds = (size_t)2 * P * 2 * QB_ENV_NC * sizeof(cd) + 2 * 64 * 17 * sizeof(cd);
+    const double ent = (double)N * N * nz;
+    const dim3 grid((unsigned)tpairs, 1, (unsigned)npair);
+    slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
+                         ent * ((out64 ? sizeof(cd) : 0) + (out32 ? sizeof(cf) : 0)));
+    if (op == QB_CFIE_KRESS) {
+      (void)hipFuncSetAttribute((const void*)k_assemble_env<QB_CFIE_KRESS>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_assemble_env<QB_CFIE_KRESS>), grid, dim3(256), lds, ctx->stream,
+                         g->d, z_dev, env, o0, P, pd, T, out64, out32);
+    } else {
+      (void)hipFuncSetAttribute((const void*)k_assemble_env<QB_DLP_KRESS>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_assemble_env<QB_DLP_KRESS>), grid, dim3(256), lds, ctx->stream,
+                         g->d, z_dev, env, o0, P, pd, T, out64, out32);
+    }
+    qb_prof_end(ctx, slot);
+    e = hipGetLastError();
+  }
+  // the tables and the pair list must outlive the launch
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  else (void)hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, pd, pr.size() * sizeof(int));
+  qb_dev_free(ctx, env, ebytes);
   if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
   return QB_OK;
 }

@@ -9,7 +9,7 @@
 #include "../../include/qb.h"
 
 #define QB_MAX_COMP 8
-#define QB_GEMM_VARIANTS 13
+#define QB_GEMM_VARIANTS 17
 #define QB_GEMM_CLASSES 45
 
 // Boundary geometry resident on the device.  Node arrays are concatenated
@@ -106,6 +106,9 @@ struct qb_ctx {
   int64_t prof_n[QB_K_COUNT] = {0};
   double prof_flops[QB_K_COUNT] = {0};
   double prof_bytes[QB_K_COUNT] = {0};
+  // operations issued to the matrix cores (GEMM classes): 6 M N K for the
+  // three-product tiles, 8 M N K otherwise; the int8 products as launched
+  double prof_issued[QB_K_COUNT] = {0};
 };
 
 // Launch accounting: qb_prof_begin records an event before a launch and
@@ -163,6 +166,12 @@ int qb_tables_build(qb_ctx* ctx, const double* z_dev_or_host, int64_t nz,
                     double rmin_h, double rmax, const qb_cheb_params& cp,
                     bool want_h0j0, DevTables* t);
 void qb_tables_free(qb_ctx* ctx, DevTables* t);
+// Bytes of the H and J tables of nz wavenumbers (hbytes: the H part).
+size_t qb_tables_bytes(int64_t nz, const qb_cheb_params& cp, size_t* hbytes);
+// The same tables built into caller memory (mem_bytes >= qb_tables_bytes):
+// H first, J after it.  Allocates nothing; the tables are not freed.
+int qb_tables_build_in(qb_ctx* ctx, const double* z_dev, int64_t nz, double rmin_h, double rmax,
+                       const qb_cheb_params& cp, void* mem, size_t mem_bytes, DevTables* t);
 void qb_lobatto_host(int M, double* t, double* C);  // nodes cospi(j/M), DCT matrix
 
 // assembly (qb_assemble.hip) -------------------------------------------------
@@ -184,6 +193,15 @@ int qb_assemble_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev,
 int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
                          const cd* z_dev, int64_t nz, const DevTables& t, cd* out64,
                          cf* out32);
+// The same matrices from envelope tables of H_n e^{-ikr} held in LDS (no
+// uniform tables needed; rmin, rmax: the distance range of the geometry);
+// returns 1, having done nothing, when the nodes do not pair up.
+int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
+                        const cd* z_dev, int64_t nz, double rmin, double rmax, cd* out64,
+                        cf* out32);
+// Pairs (a, b) of wavenumbers with z_b = conj(z_a) (a real one with itself),
+// flattened; false when some wavenumber has no partner.
+bool qb_conj_pairs(const cd* z_host, int64_t nz, std::vector<int>* pr);
 int qb_assemble_red_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_dev, int64_t nz,
                         const DevTables& t, cd* out64, cf* out32);
 // Checks |x_{rep_a} - x_{img[b,l]}| = |x_{rep_b} - x_{img[a,inv[l]]}| to

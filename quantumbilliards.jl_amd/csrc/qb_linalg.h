@@ -69,3 +69,20 @@ int qb_house_qr(qb_ctx* ctx, cd* A, int64_t N, int r, int ncols, cd* tau, int nb
 // Y <- Q Y for the N x ncols block Y (ld = N), Q = H_0 ... H_{r-1} from V.
 int qb_house_apply_q(qb_ctx* ctx, const cd* V, int64_t N, int r, const cd* tau, cd* Y,
                      int ncols);
+
+// FP64-accurate refinement residual on int8 MFMA (qb_ozaki.hip).
+// Padded size (multiple of 128) of the residue planes.
+int64_t qb_oz_pad(int64_t n);
+// Device bytes of the residues of T for nq nodes, and of the per-step
+// buffers of nslots nodes with r probes.
+size_t qb_oz_window_bytes(int64_t N, int nq);
+size_t qb_oz_step_bytes(int64_t N, int r, int nslots);
+// Row exponents and residues of T (nq matrices N x N, stride N*N):
+// A [nq][16][2][Np][Np] int8, erow [nq][N].
+int qb_oz_prepare(qb_ctx* ctx, const cd* T, int64_t N, int nq, signed char* A, int* erow);
+// R[b] = V - T[b] X[b] for the nb nodes of ctx->bix (all nodes 0..nb-1 when
+// null); X and R strided sX / sR per node, ld N; Bw, fcol, Cw per-step
+// buffers of nb slots (qb_oz_step_bytes).
+int qb_oz_residual(qb_ctx* ctx, const signed char* A, const int* erow, int64_t N, const cd* X,
+                   int64_t sX, int r, const cd* V, cd* R, int64_t sR, int nb, signed char* Bw,
+                   int* fcol, signed char* Cw);

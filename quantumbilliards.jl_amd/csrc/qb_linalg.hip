@@ -312,6 +312,218 @@ static void launch_gemm(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t ld
                      ctx->stream, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
+// ---------------------------------------------------------------------------
+// complex64 C[b] -= A[b] * B[b] on the bf16 matrix cores
+// ---------------------------------------------------------------------------
+// Each FP32 operand x is split into three bf16 parts, x = h + m + l with
+// h = bf16(x), m = bf16(x - h), l = bf16(x - h - m) (round to nearest; both
+// differences are exact in FP32), which keeps 24 significant bits.  A real
+// product a b is then the six products h h + h m + m h + h l + m m + l h with
+// FP32 accumulation (the dropped m l, l m, l l are below 2^-24 |a b|): the
+// accuracy of an FP32 product at 6 bf16 MFMAs (2.5 PFLOP/s dense) instead of
+// one FP32 one (157.3 TFLOP/s), i.e. up to 2.7x the FP32 rate.  A complex
+// block product is 4 real ones (Re = Ar Br + Ai (-Bi), Im = Ar Bi + Ai Br):
+// 24 v_mfma_f32_16x16x32_bf16 per 16 x 16 block and 32-deep k-tile.
+//
+// Tiles: workgroup BM x BN, k-tile 32, 4 waves WGM x WGN; operands staged in
+// LDS as FP32 planes [row][k] (k contiguous, row stride 36 floats: the
+// 32-byte fragment reads of 16 lanes cover the 64 banks once) and split into
+// bf16 in registers when the fragments are read.  Operand lanes: row / column
+// lane % 16, k = 8 (lane / 16) + 0..7 on both sides; result register v of a
+// lane holds D[4 (lane / 16) + v][lane % 16].
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split3_bf16(const float* x, bf16x8& h, bf16x8& m, bf16x8& l) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const __bf16 a = (__bf16)x[e];
+    const float r1 = x[e] - (float)a;
+    const __bf16 b = (__bf16)r1;
+    const float r2 = r1 - (float)b;
+    h[e] = a;
+    m[e] = b;
+    l[e] = (__bf16)r2;
+  }
+}
+
+__device__ __forceinline__ bf16x8 neg_bf16(bf16x8 v) {
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = -v[e];
+  return v;
+}
+
+__device__ __forceinline__ floatx4 mfma_bf6(const bf16x8* a, const bf16x8* b, floatx4 c) {
+  // a[0..2], b[0..2] = h, m, l: the six products down to 2^-24 (small ones first)
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], c, 0, 0, 0);
+  return c;
+}
+
+template <int BM, int BN, int WGM, int WGN, bool XCD>
+__global__ void __launch_bounds__(256, 1) k_gemm_bf6(int M, int N, int K, const cf* __restrict__ A,
+                                                     int64_t lda, int64_t sA,
+                                                     const cf* __restrict__ B, int64_t ldb,
+                                                     int64_t sB, cf* __restrict__ C, int64_t ldc,
+                                                     int64_t sC, const int* __restrict__ bix) {
+  static_assert(WGM * WGN == 4, "4 waves");
+  constexpr int BK = 32, ST = BK + 4;
+  constexpr int WM = BM / WGM, WN = BN / WGN;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  __shared__ __align__(16) float As[2][2][BM * ST];  // [buf][re/im][m][k]
+  __shared__ __align__(16) float Bs[2][2][BN * ST];  // [buf][re/im][n][k]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave % WGM) * WM, wn0 = (wave / WGM) * WN;
+  int mt = blockIdx.x, ntile = blockIdx.y;
+  if constexpr (XCD) {
+    // the XCD-aware tile order of k_gemm_sub
+    const unsigned gx = gridDim.x, gy = gridDim.y, ntiles = gx * gy;
+    if ((ntiles & 7u) == 0) {
+      const unsigned L = blockIdx.x + gx * blockIdx.y;
+      const unsigned t = (L & 7u) * (ntiles >> 3) + (L >> 3);
+      const unsigned grp = t / (8u * gy), first = grp * 8u;
+      const unsigned gm = gx - first < 8u ? gx - first : 8u;
+      const unsigned loc = t - first * gy;
+      mt = (int)(first + loc % gm);
+      ntile = (int)(loc / gm);
+    }
+  }
+  const int m0 = mt * BM, n0 = ntile * BN;
+  const int64_t b = bix ? bix[blockIdx.z] : blockIdx.z;
+  A += b * sA;
+  B += b * sB;
+  C += b * sC;
+  const int li = lane & 15, lg = lane >> 4;
+  floatx4 acc_re[FM][FN], acc_im[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      acc_re[i][j] = floatx4{0, 0, 0, 0};
+      acc_im[i][j] = floatx4{0, 0, 0, 0};
+    }
+  cf ra[NA], rb[NB];
+  const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N);
+  auto gload = [&](int k0) {
+    if (full_mn && k0 + BK <= K) {
+#pragma unroll
+      for (int p = 0; p < NA; ++p) {
+        const int e = tid + 256 * p, m = e % BM, kk = e / BM;
+        ra[p] = A[(m0 + m) + (int64_t)(k0 + kk) * lda];
+      }
+#pragma unroll
+      for (int p = 0; p < NB; ++p) {
+        const int e = tid + 256 * p, kk = e % BK, n = e / BK;
+        rb[p] = B[(k0 + kk) + (int64_t)(n0 + n) * ldb];
+      }
+      return;
+    }
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+      const int e = tid + 256 * p, m = e % BM, kk = e / BM;
+      const int gm = m0 + m, gk = k0 + kk;
+      ra[p] = (gm < M && gk < K) ? A[gm + (int64_t)gk * lda] : cf{0, 0};
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int e = tid + 256 * p, kk = e % BK, n = e / BK;
+      const int gn = n0 + n, gk = k0 + kk;
+      rb[p] = (gn < N && gk < K) ? B[gk + (int64_t)gn * ldb] : cf{0, 0};
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NA; ++p) {
+      const int e = tid + 256 * p, m = e % BM, kk = e / BM;
+      As[buf][0][m * ST + kk] = ra[p].re;
+      As[buf][1][m * ST + kk] = ra[p].im;
+    }
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int e = tid + 256 * p, kk = e % BK, n = e / BK;
+      Bs[buf][0][n * ST + kk] = rb[p].re;
+      Bs[buf][1][n * ST + kk] = rb[p].im;
+    }
+  };
+  auto compute = [&](int buf) {
+    bf16x8 ar[FM][3], ai[FM][3];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const int off = (wm0 + 16 * i + li) * ST + 8 * lg;
+      float xr[8], xi[8];
+      *(float4*)&xr[0] = *(const float4*)&As[buf][0][off];
+      *(float4*)&xr[4] = *(const float4*)&As[buf][0][off + 4];
+      *(float4*)&xi[0] = *(const float4*)&As[buf][1][off];
+      *(float4*)&xi[4] = *(const float4*)&As[buf][1][off + 4];
+      split3_bf16(xr, ar[i][0], ar[i][1], ar[i][2]);
+      split3_bf16(xi, ai[i][0], ai[i][1], ai[i][2]);
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int off = (wn0 + 16 * j + li) * ST + 8 * lg;
+      float yr[8], yi[8];
+      *(float4*)&yr[0] = *(const float4*)&Bs[buf][0][off];
+      *(float4*)&yr[4] = *(const float4*)&Bs[buf][0][off + 4];
+      *(float4*)&yi[0] = *(const float4*)&Bs[buf][1][off];
+      *(float4*)&yi[4] = *(const float4*)&Bs[buf][1][off + 4];
+      bf16x8 br[3], bi[3], nbi[3];
+      split3_bf16(yr, br[0], br[1], br[2]);
+      split3_bf16(yi, bi[0], bi[1], bi[2]);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) nbi[s] = neg_bf16(bi[s]);
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        acc_re[i][j] = mfma_bf6(ar[i], br, acc_re[i][j]);
+        acc_re[i][j] = mfma_bf6(ai[i], nbi, acc_re[i][j]);
+        acc_im[i][j] = mfma_bf6(ar[i], bi, acc_im[i][j]);
+        acc_im[i][j] = mfma_bf6(ai[i], br, acc_im[i][j]);
+      }
+    }
+  };
+  const int nt = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int t = 0; t < nt; ++t) {
+    const int buf = t & 1;
+    if (t + 1 < nt) gload((t + 1) * BK);
+    compute(buf);
+    if (t + 1 < nt) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn0 + j * 16 + li;
+      if (gn >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + wm0 + i * 16 + lg * 4 + v;
+        if (gm < M) {
+          cf* p = C + gm + (int64_t)gn * ldc;
+          cf c = *p;
+          c.re -= acc_re[i][j][v];
+          c.im -= acc_im[i][j][v];
+          *p = c;
+        }
+      }
+    }
+}
+
+template <int BM, int BN, int WGM, int WGN, bool XCD = false>
+static void launch_gemm_bf6(qb_ctx* ctx, int M, int N, int K, const cf* A, int64_t lda,
+                            int64_t sA, const cf* B, int64_t ldb, int64_t sB, cf* C, int64_t ldc,
+                            int64_t sC, int batch) {
+  dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  hipLaunchKernelGGL((k_gemm_bf6<BM, BN, WGM, WGN, XCD>), g, dim3(256), 0, ctx->stream, M, N, K,
+                     A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
+}
+
 // Tile variants and the per-context choice.  Variant 0/1 are the defaults
 // (wide / narrow N); the autotuner (qb_gemm_tune) may pick another per shape
 // class, measured on the device.
@@ -326,7 +538,9 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 128, 128, 16, 2, 2, true>, launch_gemm<cf, 128, 64, 16, 2, 2, false, true>,
     launch_gemm<cf, 128, 128, 16, 2, 2, false, true>, launch_gemm<cf, 128, 64, 16, 2, 2, true, true>,
     launch_gemm<cf, 64, 64, 16, 2, 2, false, true>, launch_gemm<cf, 64, 64, 16, 2, 2, true, true>,
-    launch_gemm<cf, 64, 64, 16, 2, 2>};
+    launch_gemm<cf, 64, 64, 16, 2, 2>,         launch_gemm_bf6<128, 128, 2, 2>,
+    launch_gemm_bf6<128, 128, 2, 2, true>,     launch_gemm_bf6<128, 64, 2, 2>,
+    launch_gemm_bf6<64, 64, 2, 2>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
@@ -334,20 +548,45 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2, true>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true>,
     launch_gemm<cd, 64, 32, 16, 2, 2, false, true>, launch_gemm<cd, 64, 64, 16, 2, 2, true, false, 2>,
     launch_gemm<cd, 64, 32, 16, 2, 2, false, true, 2>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true, 1, 2>,
-    launch_gemm<cd, 64, 64, 16, 2, 2, true, true, 2, 2>};
+    launch_gemm<cd, 64, 64, 16, 2, 2, true, true, 2, 2>,
+    // no bf16-split counterpart for complex128: the slots repeat the defaults
+    // and the tuner skips them (kGemmAvail)
+    launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 64, 16, 2, 2>,
+    launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 64, 16, 2, 2>};
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
-     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16"},
+     "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
+     "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
-     "64x64x16/3m/xcd/pf2"}};
+     "64x64x16/3m/xcd/pf2", "-", "-", "-", "-"}};
+// Variants that exist for the type (the tuner times only these).
+static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
+    {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
+     true},
+    {true, true, true, true, true, true, true, true, true, true, true, true, true, false, false,
+     false, false}};
+// Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
+static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
+    {false, false, false, false, false, false, false, false, false, false, false, false, false,
+     true, true, true, true},
+    {false}};
+
+// Variants built on Gauss's three-product form (3 real MFMA products per
+// complex block product instead of 4).
+static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
+    {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
+     false, false, false},
+    {false, false, false, false, false, false, false, true, true, false, true, true, true, false,
+     false, false, false}};
 
 // Variants that pack 4 waves per SIMD (small tiles, low register and LDS
 // use): fastest alone, but they leave a concurrently running kernel (the
 // lookahead panel) few issue slots, so gemm_pick_conc excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
-    {false, false, false, false, false, false, false, false, false, false, true, true, true},
-    {false, false, false, false, false, false, false, false, false, false, false, false, false}};
+    {false, false, false, false, false, false, false, false, false, false, true, true, true, false,
+     false, false, false},
+    {false}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);
@@ -380,10 +619,13 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
   const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
   int v = (ctx->gemm_conc ? ctx->gemm_pick_conc : ctx->gemm_pick)[ty][qb_gemm_class(M, N, K)];
-  if (v < 0) v = gemm_default<T>(N);
-  int slot = qb_prof_begin(ctx, ty == 0 ? QB_K_GEMM_C64 : QB_K_GEMM_C128,
-                           8.0 * M * N * K * (double)batch,
+  if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v]) v = gemm_default<T>(N);
+  const int kind = ty == 0 ? (kGemmBf6[ty][v] ? QB_K_GEMM_C64_BF16 : QB_K_GEMM_C64) : QB_K_GEMM_C128;
+  int slot = qb_prof_begin(ctx, kind, 8.0 * M * N * K * (double)batch,
                            (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
+  if (ctx->prof)
+    ctx->prof_issued[kind] +=
+        (kGemmBf6[ty][v] ? 48.0 : kGemmG3[ty][v] ? 6.0 : 8.0) * M * N * K * (double)batch;
   gemm_launch_variant<T>(ctx, v, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
   qb_prof_end(ctx, slot);
   QB_HIP(ctx, hipGetLastError());
@@ -437,7 +679,7 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
       }
     }
     for (int v = 0; v < QB_GEMM_VARIANTS; ++v) {
-      if (skip[v]) continue;
+      if (skip[v] || !kGemmAvail[ty][v]) continue;
       gemm_launch_variant<T>(ctx, v, M, N, K, A, M, (int64_t)M * K, B, K, (int64_t)K * N, C, M,
                              (int64_t)M * N, b);
       (void)hipEventRecord(e0, ctx->stream);

@@ -94,6 +94,66 @@ static LobCoef make_lob(int M) {
   return lc;
 }
 
+size_t qb_tables_bytes(int64_t nz, const qb_cheb_params& cp, size_t* hbytes) {
+  const size_t hb = (size_t)nz * cp.n_panels_h * (cp.M_h + 1) * 2 * sizeof(cd);
+  const size_t jb = (size_t)nz * cp.n_panels_j * (cp.M_j + 1) * 2 * sizeof(cd);
+  if (hbytes) *hbytes = hb;
+  return hb + jb;
+}
+
+static int tables_check(qb_ctx* ctx, double rmin_h, double rmax, const qb_cheb_params& cp) {
+  if (cp.M_h < 2 || cp.M_h > QB_MMAX || cp.M_j < 2 || cp.M_j > QB_MMAX)
+    return qb_fail(ctx, QB_E_ARG, "Chebyshev degree M must be in [2, 16]");
+  if (cp.n_panels_h < 1 || cp.n_panels_j < 1)
+    return qb_fail(ctx, QB_E_ARG, "n_panels must be >= 1");
+  if (!(rmin_h > 0.0) || !(rmax > rmin_h))
+    return qb_fail(ctx, QB_E_ARG, "table range needs 0 < rmin < rmax");
+  return QB_OK;
+}
+
+static int tables_launch(qb_ctx* ctx, const double* z_dev, int64_t nz, double rmin_h,
+                         double rmax, const qb_cheb_params& cp, DevTables* t) {
+  size_t hb;
+  const size_t tb = qb_tables_bytes(nz, cp, &hb);
+  LobCoef lh = make_lob(cp.M_h), lj = make_lob(cp.M_j);
+  const cd* zk = (const cd*)z_dev;
+  int64_t nh = nz * (int64_t)cp.n_panels_h, nj = nz * (int64_t)cp.n_panels_j;
+  int slot = qb_prof_begin(ctx, QB_K_TABLES, 0.0, (double)tb);
+  hipLaunchKernelGGL(k_build_tables<0>, dim3((unsigned)((nh + 63) / 64)), dim3(64), 0, ctx->stream,
+                     zk, nz, rmin_h, rmax, cp.n_panels_h, lh, t->H);
+  QB_HIP(ctx, hipGetLastError());
+  hipLaunchKernelGGL(k_build_tables<1>, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, ctx->stream,
+                     zk, nz, 0.0, rmax, cp.n_panels_j, lj, t->J);
+  qb_prof_end(ctx, slot);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+static void tables_describe(int64_t nz, double rmin_h, double rmax, const qb_cheb_params& cp,
+                            DevTables* t) {
+  t->nz = nz;
+  t->np_h = cp.n_panels_h;
+  t->M_h = cp.M_h;
+  t->np_j = cp.n_panels_j;
+  t->M_j = cp.M_j;
+  t->rmin_h = rmin_h;
+  t->rmax = rmax;
+}
+
+int qb_tables_build_in(qb_ctx* ctx, const double* z_dev, int64_t nz, double rmin_h, double rmax,
+                       const qb_cheb_params& cp, void* mem, size_t mem_bytes, DevTables* t) {
+  int rc = tables_check(ctx, rmin_h, rmax, cp);
+  if (rc) return rc;
+  size_t hb;
+  const size_t tb = qb_tables_bytes(nz, cp, &hb);
+  if (tb > mem_bytes) return qb_fail(ctx, QB_E_ARG, "table memory too small");
+  tables_describe(nz, rmin_h, rmax, cp, t);
+  t->H = (cd*)mem;
+  t->J = (cd*)((char*)mem + hb);
+  // the memory belongs to the caller: these tables are not passed to qb_tables_free
+  return tables_launch(ctx, z_dev, nz, rmin_h, rmax, cp, t);
+}
+
 int qb_tables_build(qb_ctx* ctx, const double* z_dev, int64_t nz, double rmin_h, double rmax,
                     const qb_cheb_params& cp, bool /*want_h0j0*/, DevTables* t) {
   if (cp.M_h < 2 || cp.M_h > QB_MMAX || cp.M_j < 2 || cp.M_j > QB_MMAX)
@@ -109,24 +169,13 @@ int qb_tables_build(qb_ctx* ctx, const double* z_dev, int64_t nz, double rmin_h,
   t->M_j = cp.M_j;
   t->rmin_h = rmin_h;
   t->rmax = rmax;
-  size_t hb = (size_t)nz * cp.n_panels_h * (cp.M_h + 1) * 2 * sizeof(cd);
-  size_t jb = (size_t)nz * cp.n_panels_j * (cp.M_j + 1) * 2 * sizeof(cd);
+  size_t hb;
+  const size_t jb = qb_tables_bytes(nz, cp, &hb) - hb;
   int rc = qb_dev_alloc(ctx, (void**)&t->H, hb);
   if (rc) return rc;
   rc = qb_dev_alloc(ctx, (void**)&t->J, jb);
   if (rc) return rc;
-  LobCoef lh = make_lob(cp.M_h), lj = make_lob(cp.M_j);
-  const cd* zk = (const cd*)z_dev;
-  int64_t nh = nz * (int64_t)cp.n_panels_h, nj = nz * (int64_t)cp.n_panels_j;
-  int slot = qb_prof_begin(ctx, QB_K_TABLES, 0.0, (double)(hb + jb));
-  hipLaunchKernelGGL(k_build_tables<0>, dim3((unsigned)((nh + 63) / 64)), dim3(64), 0, ctx->stream,
-                     zk, nz, rmin_h, rmax, cp.n_panels_h, lh, t->H);
-  QB_HIP(ctx, hipGetLastError());
-  hipLaunchKernelGGL(k_build_tables<1>, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, ctx->stream,
-                     zk, nz, 0.0, rmax, cp.n_panels_j, lj, t->J);
-  qb_prof_end(ctx, slot);
-  QB_HIP(ctx, hipGetLastError());
-  return QB_OK;
+  return tables_launch(ctx, z_dev, nz, rmin_h, rmax, cp, t);
 }
 
 void qb_tables_free(qb_ctx* ctx, DevTables* t) {
@@ -169,7 +218,7 @@ int qb_bessel_eval_dev(qb_ctx* ctx, const cd* z, int64_t n, cd* out4) {
 // non-negative doubles, so their bit patterns order like unsigned integers
 // and a vector atomicMax on the bits gives the maximum.  The 4 wavefronts of
 // a block reduce through LDS first, so one atomic per error kind and block
-// meets on each address.
+// meets on each address.  NaN errors count as +inf.
 __global__ __launch_bounds__(256) void k_cheb_check(const cd* __restrict__ zk, int64_t nz, int samples, double rmin_h,
                              double rmax, int np_h, int M_h, const cd* __restrict__ H, int np_j,
                              int M_j, const cd* __restrict__ J,
@@ -198,6 +247,11 @@ __global__ __launch_bounds__(256) void k_cheb_check(const cd* __restrict__ zk, i
     qb_bessel_j01(mk(k.re * rj, k.im * rj), &d0, &d1);
     e[2] = cabsd(f0 - d0);
     e[3] = cabsd(f1 - d1);
+    // a NaN table or direct value must fail the tolerance (the reference's
+    // max propagates NaN, optimal_panelization.jl:51); fmax and the bit-order
+    // atomic below would drop it, +inf keeps it
+    for (int c = 0; c < 4; ++c)
+      if (isnan(e[c])) e[c] = INFINITY;
   }
   __shared__ double red[4][4];
   const int w = threadIdx.x >> 6;

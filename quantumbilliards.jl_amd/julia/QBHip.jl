@@ -162,8 +162,9 @@ end
 
 # construct_boundary_matrices_with_derivatives! (cheb-CFIE:1507-1530,
 # cheb-DLP:1126-1144): T, dT/dk, d2T/dk2 in one launch.  Between boundary
-# components the default is the derivative of the matrix; reference_sign=true
-# reproduces the reference's sign of those terms (cheb-CFIE:1031-1032).
+# components the two double-layer derivative terms carry the reference's sign
+# (cheb-CFIE:1031-1032) by default, so callers get the reference's numbers;
+# reference_sign=false gives them the opposite sign, the derivative of T.
 function construct_boundary_matrices_with_derivatives!(Tbufs::Vector{Matrix{ComplexF64}},
                                                        dTbufs::Vector{Matrix{ComplexF64}},
                                                        ddTbufs::Vector{Matrix{ComplexF64}},
@@ -174,7 +175,7 @@ function construct_boundary_matrices_with_derivatives!(Tbufs::Vector{Matrix{Comp
                                                        n_panels_h::Int=15000, M_h::Int=5,
                                                        n_panels_j::Int=3000, M_j::Int=5,
                                                        timeit::Bool=false,
-                                                       reference_sign::Bool=false)
+                                                       reference_sign::Bool=true)
     use_chebyshev || error("Direct derivative matrix construction is only for real k currently")
     Mk = length(zj)
     @assert length(Tbufs) == Mk && length(dTbufs) == Mk && length(ddTbufs) == Mk
@@ -187,7 +188,7 @@ function construct_boundary_matrices_with_derivatives!(Tbufs::Vector{Matrix{Comp
                     (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Ptr{ComplexF64}, Int64, Ref{ChebParams},
                      Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{ComplexF64}, Int32, Ptr{Float64}),
                     ctx(), g, opcode(solver), collect(zj), Int64(Mk), cp, outs[1], outs[2],
-                    outs[3], Int32(reference_sign ? 2 : 0), C_NULL))
+                    outs[3], Int32(reference_sign ? 2 : 8), C_NULL))
         for m in 1:Mk
             copyto!(Tbufs[m], view(outs[1], :, :, m))
             copyto!(dTbufs[m], view(outs[2], :, :, m))

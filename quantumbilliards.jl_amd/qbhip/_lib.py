@@ -14,9 +14,11 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libqbhip.so"))
 
 QB_OK, QB_E_ARG, QB_E_HIP, QB_E_NOMEM, QB_E_STATE, QB_E_NUMERIC = 0, -1, -2, -3, -4, -5
 QB_CFIE_KRESS, QB_DLP_KRESS = 1, 2
-QB_FP64, QB_MIXED = 0, 1
+QB_FP64, QB_MIXED, QB_MIXED_FP64RES = 0, 1, 2
 QB_OUT_DEVICE = 1
 QB_DERIV_REFERENCE_SIGN = 2
+QB_ASM_CONJ_PAIRS = 4
+QB_DERIV_MATRIX_SIGN = 8
 
 EXPORTS = (
     "qb_ctx_create", "qb_ctx_destroy", "qb_last_error", "qb_ctx_device_bytes",
@@ -27,9 +29,11 @@ EXPORTS = (
     "qb_ctx_mem_info", "qb_ctx_set_graphs", "qb_window_moments_qr", "qb_window_basis",
     "qb_geom_set_orbits", "qb_geom_dim", "qb_host_alloc", "qb_host_free",
     "qb_geom_set_orbits_multi", "qb_geom_irreps", "qb_window_basis_irrep",
-    "qb_window_create_multi", "qb_window_blocks",
+    "qb_window_create_multi", "qb_window_blocks", "qb_window_matrix_cols",
+    "qb_ctx_kernel_issued",
 )
-KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other")
+KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other",
+                  "ozaki", "gemm_c64_bf16")
 
 
 class QBError(RuntimeError):
@@ -99,6 +103,7 @@ def lib():
         "qb_window_create_multi": (C.c_int, [P, P, I32, I32, P, P, I32, C.POINTER(ChebParams),
                                              I32, PP, C.POINTER(Stats)]),
         "qb_window_blocks": (I32, [P]),
+        "qb_window_matrix_cols": (C.c_int, [P, I32, I64, I64, P]),
         "qb_window_destroy": (None, [P]),
         "qb_window_footprint": (I64, [I64, I32, I32, I32]),
         "qb_window_moments": (C.c_int, [P, I32, P, P, P, D, C.POINTER(Stats)]),
@@ -115,6 +120,7 @@ def lib():
                                    P]),
         "qb_ctx_kernel_stats": (C.c_int, [P, I32, C.POINTER(D), C.POINTER(I64), C.POINTER(D),
                                           C.POINTER(D)]),
+        "qb_ctx_kernel_issued": (C.c_int, [P, I32, C.POINTER(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -28,13 +28,17 @@ class Context:
         _lib.check(self.ptr, _lib.lib().qb_ctx_set_profiling(self.ptr, 1 if enable else 0))
 
     def kernel_stats(self) -> dict:
-        """Per kernel class: device ms, launches, algorithmic flops and bytes."""
+        """Per kernel class: device ms, launches, algorithmic flops and bytes,
+        and the operations issued to the matrix cores (qb_ctx_kernel_issued)."""
         out = {}
         for k, name in enumerate(_lib.KERNEL_CLASSES):
             ms, n, fl, by = C.c_double(), C.c_int64(), C.c_double(), C.c_double()
             _lib.check(self.ptr, _lib.lib().qb_ctx_kernel_stats(self.ptr, k, C.byref(ms), C.byref(n),
                                                                 C.byref(fl), C.byref(by)))
-            out[name] = {"ms": ms.value, "launches": n.value, "flops": fl.value, "bytes": by.value}
+            iss = C.c_double()
+            _lib.check(self.ptr, _lib.lib().qb_ctx_kernel_issued(self.ptr, k, C.byref(iss)))
+            out[name] = {"ms": ms.value, "launches": n.value, "flops": fl.value, "bytes": by.value,
+                         "issued": iss.value}
         return out
 
     def set_graphs(self, on: bool):

@@ -8,9 +8,10 @@ admitted when a slot is free and its HBM footprint (the N^2 matrices of all
 live contour nodes plus the N x r solve buffers, `qbhip.footprint`) fits in
 the budget next to the windows already running; requests that do not fit
 wait while smaller ones behind them may start (first fit).  A mixed-precision
-request larger than the whole budget runs in FP64 when that footprint (16
-instead of 24 bytes per matrix entry) fits; one that fits in neither is
-rejected with MemoryError.  The default budget is the device's free memory
+request larger than the whole budget runs with the complex128-GEMM residual
+(24 instead of 56 bytes per matrix entry: no int8 residues of T), or else in
+FP64 (16 bytes), when that footprint fits; one that fits in none is rejected
+with MemoryError.  The default budget is the device's free memory
 at start-up less 12 GiB.  The host part of
 each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
 windows' device work.  Latency is measured from arrival to result.
@@ -44,6 +45,7 @@ from __future__ import annotations
 
 import math
 import threading
+import warnings
 import time
 from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -107,6 +109,7 @@ class WindowRequest:
     arrival: float = 0.0    # seconds after the stream starts
     N: Optional[List[int]] = None  # explicit node counts per component
     symmetry: bool = False  # serve as the symmetry-reduced windows of SYM_GROUPS[shape]
+    r: Optional[int] = None  # starting probe count (default: the Weyl estimate, probe_count)
 
 
 @dataclass
@@ -265,14 +268,19 @@ class Server:
         bs = b if order == 1 else dataclasses.replace(b, area=b.area / order,
                                                       length=b.length / order)
         N = int(sum(Ns)) // order
-        r = min(probe_count(bs, req.k0, req.R, req.nq, req.svd_tol), N)
+        r = min(req.r if req.r is not None else probe_count(bs, req.k0, req.R, req.nq,
+                                                             req.svd_tol), N)
         prec = req.precision
         nm = req.nq * len(irr)  # matrices of the window
         fp = footprint(N, nm, prec, r)
         if fp > self.budget and prec == "mixed":
-            fp64 = footprint(N, nm, "fp64", r)
-            if fp64 <= self.budget:
-                prec, fp = "fp64", fp64
+            # the int8 residual keeps residues of T (32 more bytes per matrix
+            # entry): without them, then in FP64, when that is what fits
+            for alt in ("mixed_fp64res", "fp64"):
+                fa = footprint(N, nm, alt, r)
+                if fa <= self.budget:
+                    prec, fp = alt, fa
+                    break
         return b, Ns, N, r, fp, prec
 
     def _dispatch(self):
@@ -350,6 +358,7 @@ class Server:
         b, Ns, N, r, fp, prec = prep
         reqs = [j[0] for j in jobs]
         t_admit = time.perf_counter()
+        win = None
         try:
             ctx = self.slots[slot]
             pts = self.pts_cache[(req.shape, Ns)]
@@ -365,10 +374,14 @@ class Server:
             ck = (req.shape, Ns, tuple((round(q.k0, 3), round(q.R, 3)) for q in reqs), req.nq,
                   req.tol)
             if ck not in self.cheb_cache:
-                nh, mh, nj, mj, _ = chebyshev_params(CFIE_kress(req.ppw, b), pts, zj,
-                                                     tol=max(req.tol, 1e-13), geom=geom)
-                self.cheb_cache[ck] = dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj)
-            cheb = self.cheb_cache[ck]
+                ttol = max(req.tol, 1e-13)
+                with warnings.catch_warnings():
+                    warnings.simplefilter("ignore", RuntimeWarning)
+                    nh, mh, nj, mj, errs = chebyshev_params(CFIE_kress(req.ppw, b), pts, zj,
+                                                            tol=ttol, geom=geom)
+                self.cheb_cache[ck] = (dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj),
+                                       bool(np.all(errs < ttol)))
+            cheb, cheb_ok = self.cheb_cache[ck]
             # the probe block is drawn on the host pool while the window is
             # assembled and factored (the C calls release the interpreter)
             vf = self.host_pool.submit(probe_block, np.random.default_rng(req.seed), N, r)
@@ -381,15 +394,40 @@ class Server:
             Rm, Cm = win.moments_qr(vf.result(), max(req.tol, 1e-13))
             if win.nblocks == 1:
                 Rm, Cm = Rm[None], Cm[None]
+            # Rank saturation: with rk == r every probe direction is used and
+            # the disk may hold more eigenvalues than r probes resolve.  The
+            # factors stay resident, so the moments are solved again with
+            # twice the probes until the rank no longer saturates (the
+            # reference retries with 2r, beyn.jl:291-309).  The extra solve
+            # buffers are taken from the budget; when they do not fit, the
+            # result is marked saturated instead.
+            r_used, saturated, grow_rng = r, False, np.random.default_rng(req.seed + 7919)
+            while _saturated(Rm, min(q.svd_tol for q in reqs)):
+                r2 = min(2 * r_used, N)
+                extra = footprint(N, req.nq * nirr * len(reqs), prec, r2) - fp
+                if r2 == r_used or not self._reserve(extra):
+                    saturated = True
+                    break
+                fp += extra
+                Rm, Cm = win.moments_qr(probe_block(grow_rng, N, r2), max(req.tol, 1e-13))
+                if win.nblocks == 1:
+                    Rm, Cm = Rm[None], Cm[None]
+                r_used = r2
+            r = r_used
             st = dict(win.stats)
             st["precision_used"] = prec
             st["group"] = len(reqs)
-            win.close()
+            st["rank_saturated"] = saturated
+            st["cheb_tol_met"] = cheb_ok
         except Exception as e:  # release and report
             self._release(slot, fp)
             for _, fut, _ in entries:
-                fut.set_exception(e)
+                # no traceback: its frames would keep the window's buffers alive
+                fut.set_exception(e.with_traceback(None))
             return
+        finally:
+            if win is not None:
+                win.close()
         t_dev = time.perf_counter() - t_admit
         self._release(slot, fp)
 
@@ -416,12 +454,33 @@ class Server:
             hf.add_done_callback(lambda f, fut=fut: fut.set_exception(f.exception())
                                  if f.exception() else fut.set_result(f.result()))
 
+    def _reserve(self, extra: int) -> bool:
+        """Take `extra` bytes of the budget for a running window if they fit."""
+        with self.lock:
+            if self.live_bytes + extra > self.budget:
+                return False
+            self.live_bytes += extra
+            return True
+
     def _release(self, slot, fp):
         with self.lock:
             self.live_bytes -= fp
             self.free_slots.append(slot)
             self.free_slots.sort()
             self.lock.notify_all()
+
+
+def _saturated(Rm: np.ndarray, svd_tol: float) -> bool:
+    """True when some moment block has full rank r at the threshold: all r
+    singular values of its R (A0 = Q R) are >= svd_tol.  sigma_min(R) <=
+    min_i |R_ii| (R is triangular, its eigenvalues are its diagonal), so the
+    SVD is only taken when every diagonal entry clears the threshold."""
+    for Rb in Rm:
+        if np.min(np.abs(np.diag(Rb))) < svd_tol:
+            continue
+        if np.linalg.svd(Rb, compute_uv=False)[-1] >= svd_tol:
+            return True
+    return False
 
 
 def _merge(parts: List[WindowResult]) -> WindowResult:

@@ -15,6 +15,7 @@ CFIE (cheb-CFIE:1447), this mirror raises as well rather than return zeros.
 from __future__ import annotations
 
 import ctypes as C
+import warnings
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Union
 
@@ -140,11 +141,14 @@ def _cheb(n_panels_h, M_h, n_panels_j, M_j):
 
 def assemble(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_panels_j=3000, M_j=5,
              geom: Optional[Geometry] = None, ctx: Optional[Context] = None,
-             out: Optional[np.ndarray] = None) -> np.ndarray:
+             out: Optional[np.ndarray] = None, conj_pairs: bool = False) -> np.ndarray:
     """All T(z_q) in one launch.  Returns an array `T` of shape (nz, N, N)
     with T[q] the matrix T(z_q) in the usual (row, column) indexing; its
     storage is column-major per matrix (T[q].flags.f_contiguous).  With a
-    solver symmetry, N is the number of orbits (the reduced matrices)."""
+    solver symmetry, N is the number of orbits (the reduced matrices).
+    conj_pairs=True takes the kernel of Beyn windows with real k0
+    (QB_ASM_CONJ_PAIRS): zj must be closed under conjugation, and both
+    matrices of a pair come from the two nodes' Hankel tables."""
     ctx = ctx or (geom.ctx if geom is not None else default_context())
     g = geom or Geometry(pts, ctx, solver)
     zj = np.ascontiguousarray(np.asarray(zj, dtype=np.complex128).ravel())
@@ -157,8 +161,9 @@ def assemble(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_panels_j=3000, M_j=5
         assert raw.shape == (nm, N, N) and raw.dtype == np.complex128 and raw.flags.c_contiguous
     cp = _cheb(n_panels_h, M_h, n_panels_j, M_j)
     rr = np.zeros(2)
+    flags = _lib.QB_ASM_CONJ_PAIRS if conj_pairs else 0
     rc = _lib.lib().qb_assemble(ctx.ptr, g.ptr, solver.op, zj.ctypes.data, nz, C.byref(cp),
-                                raw.ctypes.data, 0, rr.ctypes.data)
+                                raw.ctypes.data, flags, rr.ctypes.data)
     _lib.check(ctx.ptr, rc)
     # raw[q] holds T(z_q) column-major: raw[q, j, i] = T[i, j]
     return raw.transpose(0, 2, 1)
@@ -192,11 +197,13 @@ def construct_boundary_matrices(Tbufs, solver, pts, zj, *, multithreaded: bool =
 def assemble_with_derivatives(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_panels_j=3000,
                               M_j=5, geom: Optional[Geometry] = None,
                               ctx: Optional[Context] = None,
-                              reference_offblock_sign: bool = False):
+                              reference_offblock_sign: bool = True):
     """T(z_q), dT/dk(z_q) and d2T/dk2(z_q) in one launch (qb_assemble_deriv).
     Returns three arrays of shape (nz, N, N) indexed like `assemble`.
-    reference_offblock_sign=True gives the between-component double-layer
-    derivative terms the reference's (opposite) sign, cheb-CFIE:1031-1032."""
+    Between boundary components the two double-layer derivative terms carry
+    the reference's sign (cheb-CFIE:1031-1032) by default, so the drop-in
+    returns the reference's numbers; reference_offblock_sign=False gives them
+    the opposite sign, which makes dT/dk the derivative of T there."""
     ctx = ctx or (geom.ctx if geom is not None else default_context())
     g = geom or Geometry(pts, ctx, solver)
     zj = np.ascontiguousarray(np.asarray(zj, dtype=np.complex128).ravel())
@@ -204,7 +211,7 @@ def assemble_with_derivatives(solver, pts, zj, *, n_panels_h=15000, M_h=5, n_pan
     raws = [np.empty((nz, N, N), dtype=np.complex128) for _ in range(3)]
     cp = _cheb(n_panels_h, M_h, n_panels_j, M_j)
     rr = np.zeros(2)
-    flags = _lib.QB_DERIV_REFERENCE_SIGN if reference_offblock_sign else 0
+    flags = _lib.QB_DERIV_REFERENCE_SIGN if reference_offblock_sign else _lib.QB_DERIV_MATRIX_SIGN
     rc = _lib.lib().qb_assemble_deriv(ctx.ptr, g.ptr, solver.op, zj.ctypes.data, nz, C.byref(cp),
                                       *[r.ctypes.data for r in raws], flags, rr.ctypes.data)
     _lib.check(ctx.ptr, rc)
@@ -219,7 +226,7 @@ def construct_boundary_matrices_with_derivatives(Tbufs, dTbufs, ddTbufs, solver,
                                                  timeit: bool = False,
                                                  geom: Optional[Geometry] = None,
                                                  ctx: Optional[Context] = None,
-                                                 reference_offblock_sign: bool = False):
+                                                 reference_offblock_sign: bool = True):
     """construct_boundary_matrices_with_derivatives!(Tbufs, dTbufs, ddTbufs,
     solver, pts, zj; ...) (cheb-CFIE:1507-1530, cheb-DLP:1126-1144): the
     three buffers of every wavenumber are overwritten with T, dT/dk and
@@ -259,6 +266,13 @@ def chebyshev_params(solver, pts, zj, *, npanels_h_init=15000, M_h_init=5, npane
                                  int(sampling_points), int(max_iter), C.byref(cp),
                                  errs.ctypes.data)
     _lib.check(ctx.ptr, rc)
+    if not np.all(errs < tol):
+        # the reference warns and returns its last tables as well
+        # (chebyshev_optimal_panelization.jl:260-261)
+        warnings.warn(f"chebyshev_params: tolerance {tol:g} not met after {max_iter} iterations "
+                      f"(errors H0 {errs[0]:.3g}, H1 {errs[1]:.3g}, J0 {errs[2]:.3g}, "
+                      f"J1 {errs[3]:.3g}); returning the last table sizes",
+                      RuntimeWarning, stacklevel=2)
     return cp.n_panels_h, cp.M_h, cp.n_panels_j, cp.M_j, errs
 
 

@@ -12,6 +12,14 @@ build's code).  Run from the repository root:  python tests/golden/make_golden.p
   circle_multipliers.json  Fourier multipliers of the unit-circle CFIE and DLP
                            operators, pi z J_m (H_m - i H_m') and -i pi z J_m H_m',
                            m = -20..20
+  circle_r07_multipliers.json
+                           the same on the circle of radius a = 0.7 (speed a,
+                           curvature 1/a): pi a z J_m(a z) (H_m(a z) - i H_m'(a z))
+                           and -i pi a z J_m(a z) H_m'(a z)
+  annulus_zeros.json       Dirichlet eigenvalues of the concentric annulus
+                           0.3 < |x| < 1 in |k - 14| <= 0.5: zeros of
+                           J_m(k) Y_m(0.3 k) - J_m(0.3 k) Y_m(k), multiplicity 2
+                           for m > 0
 """
 import json
 import os
@@ -69,6 +77,40 @@ def bessel_points():
     return pts
 
 
+def circle_multipliers(a, zs):
+    out = {}
+    for z in zs:
+        ent = []
+        w = a * z
+        for m in range(-20, 21):
+            J = mp.besselj(m, w)
+            H = mp.hankel1(m, w)
+            Hp = (mp.hankel1(m - 1, w) - mp.hankel1(m + 1, w)) / 2
+            ent.append({"m": m, "cfie": c2(mp.pi * w * J * (H - 1j * Hp)),
+                        "dlp": c2(-1j * mp.pi * w * J * Hp)})
+        out[f"{float(z.real)}{float(z.imag):+}j"] = {"z": c2(z), "radius": float(a),
+                                                      "multipliers": ent}
+    return out
+
+
+def annulus_zeros(rho, lo, hi, mmax=60):
+    """Zeros in [lo, hi] of f_m(k) = J_m(k) Y_m(rho k) - J_m(rho k) Y_m(k):
+    sign changes on a fine grid, then mpmath's root finder."""
+    f = lambda m, k: (mp.besselj(m, k) * mp.bessely(m, rho * k)
+                      - mp.besselj(m, rho * k) * mp.bessely(m, k))
+    out = []
+    for m in range(mmax):
+        ks = [lo - 0.05 + i * (hi - lo + 0.1) / 2000 for i in range(2001)]
+        vs = [f(m, mp.mpf(k)) for k in ks]
+        for i in range(len(ks) - 1):
+            if vs[i] * vs[i + 1] < 0:
+                z = mp.findroot(lambda k: f(m, k), (mp.mpf(ks[i]), mp.mpf(ks[i + 1])),
+                                solver="anderson")
+                if lo <= z <= hi:
+                    out.append({"m": m, "k": float(z), "mult": 1 if m == 0 else 2})
+    return sorted(out, key=lambda e: e["k"])
+
+
 def main():
     disk = jm_zeros(19.5, 20.5)
     disk = [e for e in disk if abs(e["k"] - 20.0) <= 0.5]
@@ -93,6 +135,11 @@ def main():
                         "dlp": c2(-1j * mp.pi * z * J * Hp)})
         mult[f"{float(z.real)}{float(z.imag):+}j"] = {"z": c2(z), "multipliers": ent}
     json.dump(mult, open(os.path.join(HERE, "circle_multipliers.json"), "w"), indent=1)
+    json.dump(circle_multipliers(mp.mpf("0.7"), (mp.mpc(9.0, 0.2), mp.mpc(13.0, -0.25))),
+              open(os.path.join(HERE, "circle_r07_multipliers.json"), "w"), indent=1)
+    json.dump({"inner_radius": 0.3, "center": 14.0, "radius": 0.5,
+               "zeros": annulus_zeros(mp.mpf("0.3"), 13.5, 14.5)},
+              open(os.path.join(HERE, "annulus_zeros.json"), "w"), indent=1)
 
 
 if __name__ == "__main__":

@@ -145,3 +145,68 @@ extern "C" void qbt_force_variant(qb_ctx* ctx, int ty, int v) {
   for (int k = 0; k < QB_GEMM_CLASSES; ++k) ctx->gemm_pick[ty][k] = ctx->gemm_pick_conc[ty][k] = v;
   ++ctx->gemm_epoch;
 }
+
+// The int8 (Ozaki) residual R[b] = V - T[b] X[b] for nb nodes (host arrays:
+// T [nb][N][N], X [nb][N][r] column-major, V [N][r]); R out like X.
+extern "C" int qbt_oz_residual(qb_ctx* ctx, int N, int r, int nb, const cd* T, const cd* X,
+                               const cd* V, cd* R) {
+  const size_t NN = (size_t)N * N, nv = (size_t)N * r;
+  const size_t abytes = qb_oz_window_bytes(N, nb) - (size_t)nb * N * sizeof(int);
+  const size_t sb = qb_oz_step_bytes(N, r, nb);
+  const size_t Np = (size_t)qb_oz_pad(N), rp = (size_t)qb_oz_pad(r);
+  const size_t bc = 16 * 2 * rp * Np * nb;
+  cd *dT, *dX, *dV, *dR;
+  signed char *A, *Bw, *Cw;
+  int *E, *F;
+  if (hipMalloc(&dT, NN * nb * sizeof(cd)) || hipMalloc(&dX, nv * nb * sizeof(cd)) ||
+      hipMalloc(&dV, nv * sizeof(cd)) || hipMalloc(&dR, nv * nb * sizeof(cd)) ||
+      hipMalloc(&A, abytes) || hipMalloc(&E, (size_t)nb * N * sizeof(int)) ||
+      hipMalloc(&Bw, bc) || hipMalloc(&Cw, bc) || hipMalloc(&F, rp * nb * sizeof(int)))
+    return -2;
+  (void)sb;
+  hipMemcpy(dT, T, NN * nb * sizeof(cd), hipMemcpyHostToDevice);
+  hipMemcpy(dX, X, nv * nb * sizeof(cd), hipMemcpyHostToDevice);
+  hipMemcpy(dV, V, nv * sizeof(cd), hipMemcpyHostToDevice);
+  int rc = qb_oz_prepare(ctx, dT, N, nb, A, E);
+  if (!rc) rc = qb_oz_residual(ctx, A, E, N, dX, (int64_t)nv, r, dV, dR, (int64_t)nv, nb, Bw, F, Cw);
+  hipStreamSynchronize(ctx->stream);
+  hipMemcpy(R, dR, nv * nb * sizeof(cd), hipMemcpyDeviceToHost);
+  hipFree(dT); hipFree(dX); hipFree(dV); hipFree(dR); hipFree(A); hipFree(E); hipFree(Bw);
+  hipFree(Cw); hipFree(F);
+  return rc;
+}
+
+// The int8 product itself on one modulus-free path: time of qb_oz_residual
+// (ms per call, after one warmup) for nb nodes of N x N with r columns.
+extern "C" double qbt_oz_bench(qb_ctx* ctx, int N, int r, int nb, int iters) {
+  const size_t NN = (size_t)N * N, nv = (size_t)N * r;
+  const size_t abytes = qb_oz_window_bytes(N, nb) - (size_t)nb * N * sizeof(int);
+  const size_t Np = (size_t)qb_oz_pad(N), rp = (size_t)qb_oz_pad(r);
+  const size_t bc = 16 * 2 * rp * Np * nb;
+  cd *dT, *dX, *dV, *dR;
+  signed char *A, *Bw, *Cw;
+  int *E, *F;
+  if (hipMalloc(&dT, NN * nb * sizeof(cd)) || hipMalloc(&dX, nv * nb * sizeof(cd)) ||
+      hipMalloc(&dV, nv * sizeof(cd)) || hipMalloc(&dR, nv * nb * sizeof(cd)) ||
+      hipMalloc(&A, abytes) || hipMalloc(&E, (size_t)nb * N * sizeof(int)) ||
+      hipMalloc(&Bw, bc) || hipMalloc(&Cw, bc) || hipMalloc(&F, rp * nb * sizeof(int)))
+    return -1.0;
+  hipMemset(dT, 0x3f, NN * nb * sizeof(cd));
+  hipMemset(dX, 0x3f, nv * nb * sizeof(cd));
+  hipMemset(dV, 0, nv * sizeof(cd));
+  qb_oz_prepare(ctx, dT, N, nb, A, E);
+  qb_oz_residual(ctx, A, E, N, dX, (int64_t)nv, r, dV, dR, (int64_t)nv, nb, Bw, F, Cw);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, ctx->stream);
+  for (int it = 0; it < iters; ++it)
+    qb_oz_residual(ctx, A, E, N, dX, (int64_t)nv, r, dV, dR, (int64_t)nv, nb, Bw, F, Cw);
+  hipEventRecord(e1, ctx->stream);
+  hipEventSynchronize(e1);
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipFree(dT); hipFree(dX); hipFree(dV); hipFree(dR); hipFree(A); hipFree(E); hipFree(Bw);
+  hipFree(Cw); hipFree(F);
+  return ms / iters;
+}

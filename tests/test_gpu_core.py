@@ -127,9 +127,12 @@ def test_dlp_assembly_matches_oracle():
     assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_fp64res"])
 def test_window_moments_match_host_solve(precision):
-    """LU + solves + moments on the device against LAPACK on the same T."""
+    """LU + solves + moments on the device against LAPACK on the same T
+    (mixed: refinement residual on the int8 matrix cores; mixed_fp64res:
+    residual as a complex128 GEMM), and every node's full-column backward
+    error at exit below the refinement tolerance."""
     q = qb()
     b, pts = _pts("star", [320])
     solver = q.CFIE_kress(10.0, b)
@@ -144,6 +147,8 @@ def test_window_moments_match_host_solve(precision):
     print(win.stats)
     for a, r in ((A0, A0r), (A1, A1r)):
         assert np.max(np.abs(a - r)) / np.max(np.abs(r)) < (1e-11 if precision == "fp64" else 1e-10)
+    if precision != "fp64":
+        assert 0.0 < win.stats["refine_resid_max"] <= 1e-13
     win.close()
 
 
@@ -403,3 +408,84 @@ def test_moments_qr_on_device(precision):
     P2 = U2[:, :lead] @ U2[:, :lead].conj().T
     assert np.max(np.abs(U1.conj().T @ U1 - np.eye(U1.shape[1]))) < 1e-12
     assert np.max(np.abs(P1 - P2)) < 1e-8
+
+
+@pytest.mark.parametrize("kind,Ns,k0,op,M,nq", [
+    ("star", [512], 60.0, "cfie", 5, 8),
+    ("star", [512], 60.0, "cfie", 7, 8),
+    ("circle_hole", [200, 60], 25.0, "cfie", 5, 8),
+    ("ellipse", [384], 40.0, "dlp", 7, 6),
+    ("star", [512], 10.0, "cfie", 5, 8),
+])
+def test_conj_pair_assembly_matches_oracle(kind, Ns, k0, op, M, nq):
+    """k_assemble_env, the kernel that assembles the T(z_j) of every Beyn
+    window with real k0 (QB_ASM_CONJ_PAIRS routes qb_assemble through it),
+    against the oracle, which reads H and J from the reference's uniform
+    tables (cheb-CFIE:543-665, cheb-DLP:702-769), normwise to 1e-12 and
+    entrywise to 1e-11, and entry by entry against the exact-Hankel route
+    (AMOS at every distance) to 1e-12:
+      * each pair (z, conj z) of contour nodes is assembled together from the
+        two nodes' envelope tables of H_n e^{-izr} (quarter-octave Chebyshev
+        panels in LDS) and takes J_n(z r) = (H_n(z r) + conj H_n(conj(z) r)) / 2;
+        the real nodes theta = 0, pi pair with themselves (J = Re H);
+      * the mirror entries (j, i) of both nodes go through LDS;
+      * the two-component circle with a hole exercises the off-component
+        block (no J there) and the DLP ellipse the DLP entry;
+      * at k0 = 10 the nearest neighbours of the star lie below |k r| = 0.2,
+        where both H and the derived J come from the direct evaluation.
+    The uniform-table variant of the pair kernel (k_assemble_conj,
+    QB_ASM_TABLES=1) passes the same checks."""
+    q = qb()
+    if op == "dlp":
+        b = q.make_full_ellipse(1.0, 0.6)
+        solver = q.DLP_kress(10.0, b)
+        pts = q.evaluate_points(solver, b, 1.0, N=Ns)
+    else:
+        b, pts = _pts(kind, Ns)
+        solver = q.CFIE_kress(10.0, b)
+    zj, _ = O.contour(k0, 0.5, nq)
+    assert np.sum(np.abs(zj.imag) < 1e-14) == 2  # two self-paired real nodes
+    nh = 4000 if op == "cfie" else 3000
+    cp = dict(n_panels_h=nh, M_h=M, n_panels_j=1500, M_j=M)
+    T = q.assemble(solver, pts, zj, conj_pairs=True, **cp)
+    ocp = O.ChebParams(nh, M, 1500, M)
+    comps = O.make_geometry(kind, Ns)
+    if op == "dlp":
+        Tref = O.dlp_assemble(comps[0], zj, ocp)
+    else:
+        Tref = O.cfie_assemble(comps, zj, ocp)
+    if k0 == 10.0:
+        c = comps[0]
+        r = np.hypot(c.x - np.roll(c.x, 1), c.y - np.roll(c.y, 1))
+        assert np.min(r) * np.min(np.abs(zj)) < 0.2  # the direct branch is reached
+    e = _relerr(T, Tref)
+    print(kind, op, M, k0, e.max(axis=0))
+    assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
+    Tex = O.dlp_assemble_exact(comps[0], zj) if op == "dlp" else O.cfie_assemble(comps, zj,
+                                                                                  exact=True)
+    ex = _relerr(T, Tex)
+    print("vs exact Hankels", ex.max(axis=0))
+    assert ex[:, 0].max() < 1e-12 and ex[:, 1].max() < 1e-12
+    # the J-table kernel (k_assemble) gives the same matrices to the tables' tolerance
+    P = q.assemble(solver, pts, zj, **cp)
+    assert _relerr(T, P)[:, 0].max() < 1e-12
+    # the uniform-table variant of the pair kernel
+    import os
+    os.environ["QB_ASM_TABLES"] = "1"
+    try:
+        Tt = q.assemble(solver, pts, zj, conj_pairs=True, **cp)
+    finally:
+        del os.environ["QB_ASM_TABLES"]
+    et = _relerr(Tt, Tref)
+    assert et[:, 0].max() < 1e-12 and et[:, 1].max() < 1e-11
+
+
+def test_conj_pair_assembly_rejects_unpaired_nodes():
+    """QB_ASM_CONJ_PAIRS with a node whose conjugate is missing is an argument
+    error (the library does not fall back silently)."""
+    from qbhip._lib import QBArgError
+    q = qb()
+    b, pts = _pts("star", [128])
+    with pytest.raises(QBArgError):
+        q.assemble(q.CFIE_kress(10.0, b), pts, [20.0 + 0.3j, 20.0 - 0.2j], conj_pairs=True,
+                   n_panels_h=2000, M_h=5, n_panels_j=800, M_j=5)

@@ -93,16 +93,18 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(13))
+@pytest.mark.parametrize("variant", range(17))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
     LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
     are multiples of 8 with a partial group of m-tiles, the case in which the
-    XCD-aware variants (5, 6, 9, complex64 11, complex128 12) remap tiles;
-    variants 7 and 8 (and complex64 9-11, complex128 10-12) form complex
-    products from three real ones (Gauss); complex128 9, 10 and 12 stage
-    k-tiles two ahead (K = 33, 40, 128 cover one, two and an odd number of
-    k-tiles)."""
+    XCD-aware variants (5, 6, 9, complex64 11 and 14, complex128 12) remap
+    tiles; variants 7 and 8 (and complex64 9-11, complex128 10-12) form
+    complex products from three real ones (Gauss); complex128 9, 10 and 12
+    stage k-tiles two ahead (K = 33, 40, 128 cover one, two and an odd number
+    of k-tiles); complex64 13-16 run on the bf16 matrix cores with three-way
+    bf16 splits of the FP32 operands (32-deep k-tiles: K = 33 and 40 have a
+    partial second k-tile), within the FP32 tolerance."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
@@ -125,3 +127,133 @@ def test_autotune_report_and_results():
     rep = ctx.autotune(512, 4, 64, "mixed")
     assert "c64" in rep and "c128" in rep
     ctx.close()
+
+
+def _oz(L, ctx, T, X, V):
+    """qb_oz_residual through the harness: T [nb][N][N], X [nb][N][r] (numpy
+    row/column indexing), V [N][r]; returns R [nb][N][r]."""
+    nb, N, _ = T.shape
+    r = X.shape[2]
+    Tc = np.ascontiguousarray(T.transpose(0, 2, 1))  # column-major per node
+    Xc = np.ascontiguousarray(X.transpose(0, 2, 1))
+    Vc = np.ascontiguousarray(V.T)
+    Rc = np.empty_like(Xc)
+    P = lambda a: C.c_void_p(_c(a))
+    rc = L.qbt_oz_residual(C.c_void_p(ctx.value), N, r, nb, P(Tc), P(Xc), P(Vc), P(Rc))
+    assert rc == 0
+    return Rc.transpose(0, 2, 1)
+
+
+@pytest.mark.parametrize("N,r,nb", [(256, 8, 2), (200, 13, 3)])
+def test_ozaki_product_is_exact_on_integers(probe, N, r, nb):
+    """Integer-valued T and X whose row and column maxima lie in [2^52, 2^53)
+    (scale exponents 0): with V = 0, -R is the integer product T X rounded
+    once to FP64 (Garner's mixed-radix digits are exact; Horner's rule adds
+    at most 16 roundings).  Compared with Python integer arithmetic.  A wrong
+    lane layout of the int8 MFMA, a wrong modulus inverse or a wrong
+    residue sign would give errors of order the entries themselves."""
+    L, ctx = probe
+    rng = np.random.default_rng(N + r)
+    lim = 2 ** 52
+
+    def ints(nb, n, m):
+        """nb random complex integer matrices n x m (entries below 2^52 in
+        magnitude) with one real part per row of magnitude in [2^52, 2^53)."""
+        z = (rng.integers(-lim, lim, (nb, n, m)).astype(np.float64)
+             + 1j * rng.integers(-lim, lim, (nb, n, m)).astype(np.float64))
+        for bb in range(nb):
+            for i in range(n):
+                j = int(rng.integers(0, m))
+                sgn = 1.0 if rng.integers(0, 2) else -1.0
+                z[bb, i, j] = sgn * float(lim + int(rng.integers(0, lim))) + 1j * z[bb, i, j].imag
+        return z
+
+    T = ints(nb, N, N)
+    X = ints(nb, r, N).transpose(0, 2, 1)  # one large entry per column
+    V = np.zeros((N, r), complex)
+    R = _oz(L, ctx, T, X, V)
+    for b in range(nb):
+        Ti = np.vectorize(int, otypes=[object])
+        Tr, Tim = Ti(T[b].real), Ti(T[b].imag)
+        Xr, Xim = Ti(X[b].real), Ti(X[b].imag)
+        Pre = Tr.dot(Xr) - Tim.dot(Xim)
+        Pim = Tr.dot(Xim) + Tim.dot(Xr)
+        got = -R[b]
+        for P, g in ((Pre, got.real), (Pim, got.imag)):
+            ex = np.array([[float(v) for v in row] for row in P])
+            err = np.abs(g - ex)
+            assert np.all(err <= 2.0 ** -48 * np.abs(ex) + 1.0), float(np.max(err / np.abs(ex)))
+
+
+@pytest.mark.parametrize("N,r,nb", [(256, 8, 2), (333, 21, 2), (1024, 40, 1)])
+def test_ozaki_residual_matches_extended_precision(probe, N, r, nb):
+    """R = V - T X for complex T with rows of widely different scales and X
+    with columns of different scales (so the per-row and per-column
+    exponents differ), against the product in 80-bit extended precision:
+    every entry within 4 * 2^-53 * N * max|T_i.| * max|X_.c| (the truncation
+    of T and X to 53 significant bits per row / column) plus 2^-52 (|V| +
+    |T X|) (the FP64 roundings of the reconstructed product and of
+    V - T X), and no worse than 8x the error of the FP64 GEMM of NumPy on
+    the same bound."""
+    L, ctx = probe
+    rng = np.random.default_rng(7 * N + r)
+    T = (rng.standard_normal((nb, N, N)) + 1j * rng.standard_normal((nb, N, N)))
+    T *= np.exp(rng.uniform(-6, 6, (nb, N, 1)))
+    X = (rng.standard_normal((nb, N, r)) + 1j * rng.standard_normal((nb, N, r)))
+    X *= np.exp(rng.uniform(-4, 4, (nb, 1, r)))
+    V = rng.standard_normal((N, r)) + 1j * rng.standard_normal((N, r))
+    R = _oz(L, ctx, T, X, V)
+    for b in range(nb):
+        Tl, Xl = T[b].astype(np.clongdouble), X[b].astype(np.clongdouble)
+        ref = V.astype(np.clongdouble) - Tl @ Xl
+        tm = np.max(np.maximum(np.abs(T[b].real), np.abs(T[b].imag)), axis=1)[:, None]
+        xm = np.max(np.maximum(np.abs(X[b].real), np.abs(X[b].imag)), axis=0)[None, :]
+        TX = np.abs((Tl @ Xl).astype(np.complex128))
+        bound = 4 * 2.0 ** -53 * N * tm * xm + 2.0 ** -52 * (np.abs(V) + TX)
+        err = np.abs((R[b] - ref).astype(np.complex128))
+        print(N, r, "max err / bound", float(np.max(err / bound)))
+        assert np.all(err <= bound)
+        e64 = np.abs((V - T[b] @ X[b]) - ref.astype(np.complex128))
+        assert np.max(err / bound) <= max(8 * float(np.max(e64 / bound)), 0.05)
+
+
+@pytest.mark.parametrize("variant", [13, 14, 15, 16])
+def test_bf16_split_gemm_is_fp32_accurate(probe, variant):
+    """The complex64 GEMM on bf16 MFMA (three-way bf16 splits of every FP32
+    operand, six products per real product, FP32 accumulation) has the
+    error of the FP32-MFMA kernel on the same operands, within 2x; a
+    two-way split (or a dropped middle product) would be ~2^8 times worse."""
+    L, ctx = probe
+    c = C.c_void_p(ctx.value)
+    rng = np.random.default_rng(variant)
+    M, N, K, b = 515, 260, 700, 2
+    A = _rand(rng, (b, K, M), np.complex64)
+    B = _rand(rng, (b, N, K), np.complex64)
+    C0 = _rand(rng, (b, N, M), np.complex64)
+    ref = np.stack([(C0[i].T.astype(np.complex128) - A[i].T.astype(np.complex128)
+                     @ B[i].T.astype(np.complex128)).T for i in range(b)])
+    errs = {}
+    try:
+        for v in (0, variant):
+            L.qbt_force_variant(c, 0, v)
+            Cm = C0.copy()
+            assert L.qbt_gemm_sub_c64(c, M, N, K, C.c_void_p(_c(A)), C.c_void_p(_c(B)),
+                                      C.c_void_p(_c(Cm)), b) == 0
+            errs[v] = float(np.max(np.abs(Cm - ref)) / np.max(np.abs(ref)))
+    finally:
+        L.qbt_force_variant(c, 0, -1)
+    print(errs)
+    assert errs[variant] <= 2 * errs[0] + 1e-7
+
+
+def test_copy_kernel_reference_for_traffic_counters(probe):
+    """A plain copy kernel of known size (k_copy16: 1 GiB read + 1 GiB
+    written per launch, two launches): the reference against which the
+    FETCH_SIZE / WRITE_SIZE counter runs over the library's kernels are
+    scaled (the counters run over this test and the bench in the same
+    rocprofv3 pass).  Here only the bandwidth is sanity-checked."""
+    L, ctx = probe
+    L.qbt_copy_bench.restype = C.c_double
+    gbs = L.qbt_copy_bench(C.c_void_p(ctx.value), C.c_int64(1 << 30), 2)
+    print("copy GB/s", gbs)
+    assert gbs > 100.0

@@ -68,9 +68,10 @@ def test_admission_by_footprint():
 
 def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
     """A mixed-precision window whose footprint exceeds the whole budget runs
-    in FP64 (16 instead of 24 bytes per matrix entry) when that fits, with
-    the same eigenvalues; the per-request outcome of a stream includes the
-    rejection of a window that fits in neither."""
+    with the complex128-GEMM residual (no int8 residues of T: 24 instead of
+    56 bytes per matrix entry) when that fits, else in FP64 (16 bytes) when
+    that fits, with the same eigenvalues; the per-request outcome of a stream
+    includes the rejection of a window that fits in none."""
     from qbhip.beyn import footprint
     from qbhip.serve import Server, WindowRequest, summarize
     req = _reqs()[0]
@@ -80,8 +81,15 @@ def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
     srv0.close()
     assert prec == "mixed"
     fp64 = footprint(N, req.nq, "fp64", r)
-    assert fp64 < fp_mixed
-    srv = Server(slots=1, hbm_budget=(fp64 + fp_mixed) // 2)
+    fpres = footprint(N, req.nq, "mixed_fp64res", r)
+    assert fp64 < fpres < fp_mixed
+    srv = Server(slots=1, hbm_budget=(fpres + fp_mixed) // 2)
+    got = srv.submit(req).result()
+    srv.close()
+    assert got.stats["precision_used"] == "mixed_fp64res"
+    assert len(got.inside) == len(ref.inside)
+    assert np.max(np.abs(got.inside - ref.inside) / np.abs(ref.inside)) <= 1e-10
+    srv = Server(slots=1, hbm_budget=(fp64 + fpres) // 2)
     got = srv.submit(req).result()
     assert got.stats["precision_used"] == "fp64"
     assert len(got.inside) == len(ref.inside)
