@@ -93,7 +93,31 @@ def parse():
     ap.add_argument("--uneven-max-group", type=int, default=16)
     ap.add_argument("--group", action="store_true",
                     help="batch windows that share N into one window (qb_window_create_multi)")
+    ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (read once, written once): "
+                         "the calibration of HBM byte counters in rocprofv3 --pmc runs")
     return ap.parse_args()
+
+
+def copy_ref(mib: int):
+    """A plain device copy of a known size (an elementwise kernel that reads
+    `mib` MiB and writes `mib` MiB, launched twice), run before the workload
+    so a FETCH_SIZE / WRITE_SIZE counter run holds its own calibration."""
+    import torch
+    n = (mib << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"# copy reference: {mib} MiB read + {mib} MiB written, {2 * n * 4 / dt / 1e9:.0f} GB/s",
+          file=sys.stderr, flush=True)
+    del src, dst
+    torch.cuda.empty_cache()
 
 
 def run_serve(a):
@@ -655,6 +679,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding of contour nodes is a design note only)"}))
         sys.exit(2)
+    if a.copy_ref:
+        copy_ref(a.copy_ref)
     if a.serve:
         run_serve(a)
         return

@@ -88,6 +88,29 @@ SYM_GROUPS = {
 REAL_TOL = 1e-8
 
 
+# A request of tolerance tol solves T(z) X = V to a backward error of tol, so
+# the singular values of A0 carry noise of about tol sigma_max(A0); rank
+# directions below RANK_NOISE tol sigma_max are that noise (the reference's
+# adaptive threshold, sigma_max 1e-15, beyn.jl:454, is the same rule for its
+# FP64 solves).  With tol 1e-12 and sigma_max <= 1 this is the absolute
+# svd_tol 1e-10 unchanged.
+RANK_NOISE = 100.0
+
+# A rank that still saturates with GROW_LIMIT times the starting probes (at
+# least 64) is noise, not levels: the leak-margin Weyl count behind the
+# starting probes already holds every level the filter passes.
+GROW_LIMIT = 4
+
+
+def rank_threshold(Rb: np.ndarray, svd_tol: float, tol: float, S=None) -> float:
+    """Rank threshold of one moment block: max(svd_tol, RANK_NOISE tol
+    sigma_max(A0)); sigma_max(A0) = sigma_max(R) for A0 = Q R."""
+    if S is None:
+        S = np.linalg.svd(Rb, compute_uv=False)
+    smax = float(S[0]) if len(S) else 0.0
+    return max(svd_tol, RANK_NOISE * tol * smax)
+
+
 def real_roots(lam: np.ndarray) -> np.ndarray:
     """The candidates with |Im lam| <= REAL_TOL |lam|."""
     lam = np.asarray(lam)
@@ -402,8 +425,11 @@ class Server:
             # buffers are taken from the budget; when they do not fit, the
             # result is marked saturated instead.
             r_used, saturated, grow_rng = r, False, np.random.default_rng(req.seed + 7919)
-            while _saturated(Rm, min(q.svd_tol for q in reqs)):
+            while _saturated(Rm, min(q.svd_tol for q in reqs), req.tol):
                 r2 = min(2 * r_used, N)
+                if r2 > GROW_LIMIT * max(r, 64):
+                    saturated = True
+                    break
                 extra = footprint(N, req.nq * nirr * len(reqs), prec, r2) - fp
                 if r2 == r_used or not self._reserve(extra):
                     saturated = True
@@ -438,7 +464,9 @@ class Server:
             th = time.perf_counter()
             lams, rk = [], 0
             for e in range(nirr):
-                B, Ur, S, rk_e = B_from_qr(Rm[e * nwin + c], Cm[e * nwin + c], q.svd_tol)
+                Rb = Rm[e * nwin + c]
+                B, Ur, S, rk_e = B_from_qr(Rb, Cm[e * nwin + c],
+                                           rank_threshold(Rb, q.svd_tol, q.tol))
                 lams.append(np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128))
                 rk += rk_e
             lam = np.concatenate(lams)
@@ -470,15 +498,17 @@ class Server:
             self.lock.notify_all()
 
 
-def _saturated(Rm: np.ndarray, svd_tol: float) -> bool:
-    """True when some moment block has full rank r at the threshold: all r
-    singular values of its R (A0 = Q R) are >= svd_tol.  sigma_min(R) <=
-    min_i |R_ii| (R is triangular, its eigenvalues are its diagonal), so the
-    SVD is only taken when every diagonal entry clears the threshold."""
+def _saturated(Rm: np.ndarray, svd_tol: float, tol: float) -> bool:
+    """True when some moment block has full rank r at its threshold
+    (rank_threshold): all r singular values of its R (A0 = Q R) clear it.
+    sigma_min(R) <= min_i |R_ii| (R is triangular, its eigenvalues are its
+    diagonal), so the SVD is only taken when every diagonal entry clears the
+    absolute part of the threshold."""
     for Rb in Rm:
         if np.min(np.abs(np.diag(Rb))) < svd_tol:
             continue
-        if np.linalg.svd(Rb, compute_uv=False)[-1] >= svd_tol:
+        S = np.linalg.svd(Rb, compute_uv=False)
+        if S[-1] >= rank_threshold(Rb, svd_tol, tol, S):
             return True
     return False
 

@@ -129,17 +129,18 @@ def test_dlp_assembly_matches_oracle():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_fp64res"])
 def test_window_moments_match_host_solve(precision):
-    """LU + solves + moments on the device against LAPACK on the same T
-    (mixed: refinement residual on the int8 matrix cores; mixed_fp64res:
-    residual as a complex128 GEMM), and every node's full-column backward
-    error at exit below the refinement tolerance."""
+    """LU + solves + moments on the device against LAPACK on the same T (the
+    window's assembly: conjugate pairs from the envelope tables; mixed:
+    refinement residual on the int8 matrix cores; mixed_fp64res: residual as
+    a complex128 GEMM), and every node's full-column backward error at exit
+    below the refinement tolerance."""
     q = qb()
     b, pts = _pts("star", [320])
     solver = q.CFIE_kress(10.0, b)
     k0, R, nq = 30.0 + 0.0j, 0.4, 6
     zj, wj = O.contour(k0, R, nq)
     cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
-    T = q.assemble(solver, pts, zj, **cp)
+    T = q.assemble(solver, pts, zj, conj_pairs=True, **cp)
     V = q.probe_block(np.random.default_rng(5), 320, 16)
     A0r, A1r = O.beyn_moments(np.array(T), zj, wj, V)
     win = q.Window(solver, pts, k0, R, nq, precision=precision, **cp)
@@ -163,7 +164,7 @@ def test_refinement_falls_back_to_fp64_factors():
     k0, R, nq = 30.0 + 0.0j, 0.4, 6
     zj, wj = O.contour(k0, R, nq)
     cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
-    T = q.assemble(solver, pts, zj, **cp)
+    T = q.assemble(solver, pts, zj, conj_pairs=True, **cp)
     V = q.probe_block(np.random.default_rng(5), 320, 16)
     A0r, A1r = O.beyn_moments(np.array(T), zj, wj, V)
     w64 = q.Window(solver, pts, k0, R, nq, precision="fp64", **cp)
@@ -458,17 +459,21 @@ def test_conj_pair_assembly_matches_oracle(kind, Ns, k0, op, M, nq):
         c = comps[0]
         r = np.hypot(c.x - np.roll(c.x, 1), c.y - np.roll(c.y, 1))
         assert np.min(r) * np.min(np.abs(zj)) < 0.2  # the direct branch is reached
-    e = _relerr(T, Tref)
-    print(kind, op, M, k0, e.max(axis=0))
-    assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
+    # the envelope tables are closer to the exact Hankels than the uniform
+    # tables of degree M: against the oracle's table route the difference is
+    # that route's own error (4e-11 at M = 5, k0 = 60), plus 1e-12
     Tex = O.dlp_assemble_exact(comps[0], zj) if op == "dlp" else O.cfie_assemble(comps, zj,
                                                                                   exact=True)
     ex = _relerr(T, Tex)
     print("vs exact Hankels", ex.max(axis=0))
     assert ex[:, 0].max() < 1e-12 and ex[:, 1].max() < 1e-12
+    own = _relerr(Tref, Tex).max(axis=0)
+    e = _relerr(T, Tref)
+    print(kind, op, M, k0, e.max(axis=0), "oracle tables vs exact", own)
+    assert e[:, 0].max() < own[0] + 1e-12 and e[:, 1].max() < own[1] + 1e-11
     # the J-table kernel (k_assemble) gives the same matrices to the tables' tolerance
     P = q.assemble(solver, pts, zj, **cp)
-    assert _relerr(T, P)[:, 0].max() < 1e-12
+    assert _relerr(T, P)[:, 0].max() < own[0] + 1e-12
     # the uniform-table variant of the pair kernel
     import os
     os.environ["QB_ASM_TABLES"] = "1"

@@ -121,7 +121,9 @@ def test_reduced_derivatives_match_oracle(case):
     cp = dict(n_panels_h=3000, M_h=7, n_panels_j=1000, M_j=7)
     got = q.assemble_with_derivatives(solver, pts, zj, **cp)
     ocp = O.ChebParams(3000, 7, 1000, 7)
-    full = (O.cfie_assemble_deriv(comps, zj, ocp) if op == "cfie"
+    # the library's default is the reference's between-component sign
+    # (cheb-CFIE:1031-1032), the same in the reduced kernels
+    full = (O.cfie_assemble_deriv(comps, zj, ocp, reference_offblock_sign=True) if op == "cfie"
             else O.dlp_assemble_deriv(comps[0], zj, ocp))
     for g, f in zip(got, full):
         ref = np.stack([O.reduce_matrix(A, img, sc) for A in f])

@@ -56,3 +56,22 @@ def test_real_roots_filter():
     got = real_roots(lam)
     assert np.array_equal(got, lam[[0, 3, 4]])
     assert real_roots(np.zeros(0, complex)).size == 0
+
+
+def test_rank_threshold_follows_the_solve_tolerance():
+    """The rank threshold of a moment block is the absolute svd_tol for
+    tol 1e-12 at sigma_max <= 1 and rises to RANK_NOISE tol sigma_max for a
+    looser tolerance, so solve noise at 1e-8 does not count as rank: a block
+    with singular values 1, 1e-3 and a noise floor of 1e-7 has rank 2 at
+    tol 1e-8 and rank 4 (saturated) at tol 1e-12."""
+    from qbhip.serve import RANK_NOISE, _saturated, rank_threshold
+    rng = np.random.default_rng(3)
+    Q1, _ = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4)))
+    S = np.array([1.0, 1e-3, 2e-7, 1e-7])
+    Rb = np.triu(np.linalg.qr((Q1 * S) @ Q2)[1])
+    assert rank_threshold(Rb, 1e-10, 1e-12) == 1e-10
+    assert math.isclose(rank_threshold(Rb, 1e-10, 1e-8), RANK_NOISE * 1e-8, rel_tol=1e-9)
+    assert _saturated(Rb[None], 1e-10, 1e-12)
+    assert not _saturated(Rb[None], 1e-10, 1e-8)
+    assert not _saturated((Rb * 1e-4)[None], 1e-10, 1e-12)
