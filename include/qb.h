@@ -38,12 +38,13 @@ enum { QB_CFIE_KRESS = 1, QB_DLP_KRESS = 2 };
 /* precision of the Beyn LU/solves
  *  QB_FP64           complex128 LU and solves
  *  QB_MIXED          complex64 LU on MFMA, iterative refinement against an
- *                    FP64 residual formed exactly on the int8 matrix cores
- *                    from residues of T kept with the window (56 bytes per
- *                    matrix entry in all, 24 without the residues)
- *  QB_MIXED_FP64RES  the same refinement with the residual as a complex128
- *                    GEMM (24 bytes per matrix entry) */
-enum { QB_FP64 = 0, QB_MIXED = 1, QB_MIXED_FP64RES = 2 };
+ *                    FP64 residual computed as a complex128 GEMM on the FP64
+ *                    matrices kept with the window (24 bytes per matrix entry)
+ *  QB_MIXED_INT8RES  the same refinement with the FP64 residual formed
+ *                    exactly on the int8 matrix cores from residues of T kept
+ *                    with the window (56 bytes per matrix entry; N <= 65535,
+ *                    else as QB_MIXED) */
+enum { QB_FP64 = 0, QB_MIXED = 1, QB_MIXED_INT8RES = 2 };
 
 /* flags for qb_assemble / qb_assemble_deriv
  *  QB_OUT_DEVICE           out (and dout, ddout) are device pointers

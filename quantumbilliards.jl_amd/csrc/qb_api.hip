@@ -927,19 +927,21 @@ struct qb_window {
   qb_stats st{};
 };
 
-// The int8 (Ozaki) residual of the mixed-precision refinement: on unless
-// QB_RESID_FP64 is set (then the residual is a complex128 GEMM), for N that
-// its exponent range covers.
-static bool oz_enabled(int64_t N) { return N <= 65535 && getenv("QB_RESID_FP64") == nullptr; }
+// The int8 (Ozaki) residual of the mixed-precision refinement: taken by
+// QB_MIXED_INT8RES for N that its exponent range covers (else, and with
+// QB_MIXED, the residual is a complex128 GEMM on T64).
+static bool oz_enabled(int precision, int64_t N) {
+  return precision == QB_MIXED_INT8RES && N <= 65535;
+}
 // refinement nodes whose residual runs in one int8 launch (bounds the step buffers)
 #define QB_OZ_SLOTS 16
 
 static size_t win_bytes_core(int64_t N, int nq, int precision) {
   size_t NN = (size_t)N * N;
   size_t b = (size_t)nq * NN * sizeof(cd);
-  if (precision == QB_MIXED || precision == QB_MIXED_FP64RES) {
+  if (precision == QB_MIXED || precision == QB_MIXED_INT8RES) {
     b += (size_t)nq * NN * sizeof(cf);
-    if (precision == QB_MIXED && oz_enabled(N)) b += qb_oz_window_bytes(N, nq);
+    if (oz_enabled(precision, N)) b += qb_oz_window_bytes(N, nq);
   }
   b += 2 * (size_t)nq * N * sizeof(int) + (size_t)nq * sizeof(int);
   return b;
@@ -952,9 +954,9 @@ extern "C" int64_t qb_window_footprint(int64_t N, int32_t nq, int32_t precision,
   // per node in mixed mode, V, A0, A1, and the QR block [A0 | A1] that
   // qb_window_moments_qr keeps
   b += 2 * (size_t)nq * nv * sizeof(cd) + 5 * nv * sizeof(cd);
-  if (precision == QB_MIXED || precision == QB_MIXED_FP64RES) {
+  if (precision == QB_MIXED || precision == QB_MIXED_INT8RES) {
     b += (size_t)nq * nv * sizeof(cf);
-    if (precision == QB_MIXED && oz_enabled(N))
+    if (oz_enabled(precision, N))
       b += qb_oz_step_bytes(N, r, nq < QB_OZ_SLOTS ? nq : QB_OZ_SLOTS);
   }
   // Chebyshev tables, bounded by 1.2e5 panels of degree 7 per node
@@ -1020,13 +1022,13 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
   if (op == QB_DLP_KRESS && g->d.ncomp != 1)
     return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
   if (nq < 1 || nq > 4096) return qb_fail(ctx, QB_E_ARG, "nq must be in [1, 4096]");
-  if (precision != QB_FP64 && precision != QB_MIXED && precision != QB_MIXED_FP64RES)
-    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64, QB_MIXED or QB_MIXED_FP64RES");
+  if (precision != QB_FP64 && precision != QB_MIXED && precision != QB_MIXED_INT8RES)
+    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64, QB_MIXED or QB_MIXED_INT8RES");
   int rc = check_cheb(ctx, cheb);
   if (rc) return rc;
-  // the int8 residual of QB_MIXED, or the complex128 GEMM of QB_MIXED_FP64RES
-  const bool oz = precision == QB_MIXED && oz_enabled(qb_dim(g));
-  if (precision == QB_MIXED_FP64RES) precision = QB_MIXED;
+  // the int8 residual of QB_MIXED_INT8RES, or the complex128 GEMM of QB_MIXED
+  const bool oz = oz_enabled(precision, qb_dim(g));
+  if (precision == QB_MIXED_INT8RES) precision = QB_MIXED;
   (void)hipSetDevice(ctx->device);
   qb_window* w = new qb_window();
   w->ctx = ctx;
@@ -1700,9 +1702,9 @@ extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, in
   if (!ctx) return QB_E_ARG;
   if (N < 64 || N > 65535 || nq < 1 || r < 1 || r > N)
     return qb_fail(ctx, QB_E_ARG, "autotune needs 64 <= N <= 65535, nq >= 1, 1 <= r <= N");
-  if (precision != QB_FP64 && precision != QB_MIXED && precision != QB_MIXED_FP64RES)
-    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64, QB_MIXED or QB_MIXED_FP64RES");
-  if (precision == QB_MIXED_FP64RES) precision = QB_MIXED;
+  if (precision != QB_FP64 && precision != QB_MIXED && precision != QB_MIXED_INT8RES)
+    return qb_fail(ctx, QB_E_ARG, "precision must be QB_FP64, QB_MIXED or QB_MIXED_INT8RES");
+  if (precision == QB_MIXED_INT8RES) precision = QB_MIXED;
   (void)hipSetDevice(ctx->device);
   struct Shape {
     int M, N, K, ty;

@@ -20,6 +20,7 @@ const QB_CFIE_KRESS = Int32(1)
 const QB_DLP_KRESS = Int32(2)
 const QB_FP64 = Int32(0)
 const QB_MIXED = Int32(1)
+const QB_MIXED_INT8RES = Int32(2)
 
 struct ChebParams
     n_panels_h::Int32
@@ -211,7 +212,7 @@ function construct_B_matrix(solver, pts, N::Int, k0::Complex{Float64}, R::Float6
     w = Ref{Ptr{Cvoid}}(C_NULL)
     st = Ref(Stats())
     cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
-    prec = precision === :mixed ? QB_MIXED : QB_FP64
+    prec = precision === :mixed ? QB_MIXED : precision === :mixed_int8res ? QB_MIXED_INT8RES : QB_FP64
     try
         check(ccall((:qb_window_create, LIB), Cint,
                     (Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64, Float64, Int32,
@@ -263,7 +264,7 @@ function construct_B_matrices(solver, pts, N::Int, k0s::Vector{ComplexF64},
     w = Ref{Ptr{Cvoid}}(C_NULL)
     st = Ref(Stats())
     cp = Ref(ChebParams(n_panels_h, M_h, n_panels_j, M_j))
-    prec = precision === :mixed ? QB_MIXED : QB_FP64
+    prec = precision === :mixed ? QB_MIXED : precision === :mixed_int8res ? QB_MIXED_INT8RES : QB_FP64
     nw = length(k0s)
     k0v = collect(reinterpret(Float64, k0s))
     try

@@ -14,7 +14,7 @@ LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libqbhip.so"))
 
 QB_OK, QB_E_ARG, QB_E_HIP, QB_E_NOMEM, QB_E_STATE, QB_E_NUMERIC = 0, -1, -2, -3, -4, -5
 QB_CFIE_KRESS, QB_DLP_KRESS = 1, 2
-QB_FP64, QB_MIXED, QB_MIXED_FP64RES = 0, 1, 2
+QB_FP64, QB_MIXED, QB_MIXED_INT8RES = 0, 1, 2
 QB_OUT_DEVICE = 1
 QB_DERIV_REFERENCE_SIGN = 2
 QB_ASM_CONJ_PAIRS = 4

@@ -13,11 +13,12 @@ B = U_k^H A1 W_k Sigma_k^{-1} and eig(B) run on the host in FP64 (LAPACK
 through NumPy), as the reference runs them through LinearAlgebra.
 
 Extensions (keyword arguments the reference does not have):
-  precision="fp64"|"mixed"|"mixed_fp64res": complex128 LU, or complex64 LU on
-      MFMA with FP64 iterative refinement (refine_tol: normwise backward error
-      at which a node stops; nodes that stagnate are refactored in FP64); the
-      residual is formed exactly on the int8 matrix cores ("mixed") or as a
-      complex128 GEMM ("mixed_fp64res", less memory);
+  precision="fp64"|"mixed"|"mixed_int8res": complex128 LU, or complex64 LU
+      on MFMA with FP64 iterative refinement (refine_tol: normwise backward
+      error at which a node stops; nodes that stagnate are refactored in
+      FP64); the residual is a complex128 GEMM ("mixed") or is formed exactly
+      on the int8 matrix cores ("mixed_int8res", 32 more bytes per matrix
+      entry);
   grow_r: keep doubling r while the rank saturates (the reference retries
       once with 2r, beyn.jl:291-309, because of the unconditional break at
       :307); with grow_r=False the single retry is reproduced;
@@ -36,7 +37,7 @@ from . import _lib
 from .context import Context, default_context
 from .solvers import Geometry, boundary_matrix_size, boundary_matrix_size_for_solver
 
-_PREC = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED, "mixed_fp64res": _lib.QB_MIXED_FP64RES}
+_PREC = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED, "mixed_int8res": _lib.QB_MIXED_INT8RES}
 
 
 def probe_block(rng: np.random.Generator, N: int, r: int, pinned: bool = False) -> np.ndarray:

@@ -296,10 +296,11 @@ class Server:
         prec = req.precision
         nm = req.nq * len(irr)  # matrices of the window
         fp = footprint(N, nm, prec, r)
-        if fp > self.budget and prec == "mixed":
+        if fp > self.budget and prec != "fp64":
             # the int8 residual keeps residues of T (32 more bytes per matrix
-            # entry): without them, then in FP64, when that is what fits
-            for alt in ("mixed_fp64res", "fp64"):
+            # entry) and mixed precision keeps complex64 factors beside the
+            # FP64 matrices: without them, when that is what fits
+            for alt in (("mixed", "fp64") if prec == "mixed_int8res" else ("fp64",)):
                 fa = footprint(N, nm, alt, r)
                 if fa <= self.budget:
                     prec, fp = alt, fa

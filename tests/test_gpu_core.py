@@ -127,12 +127,12 @@ def test_dlp_assembly_matches_oracle():
     assert e[:, 0].max() < 1e-12 and e[:, 1].max() < 1e-11
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_fp64res"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_int8res"])
 def test_window_moments_match_host_solve(precision):
     """LU + solves + moments on the device against LAPACK on the same T (the
     window's assembly: conjugate pairs from the envelope tables; mixed:
-    refinement residual on the int8 matrix cores; mixed_fp64res: residual as
-    a complex128 GEMM), and every node's full-column backward error at exit
+    refinement residual as a complex128 GEMM; mixed_int8res: residual on the
+    int8 matrix cores), and every node's full-column backward error at exit
     below the refinement tolerance."""
     q = qb()
     b, pts = _pts("star", [320])

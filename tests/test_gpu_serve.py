@@ -67,26 +67,27 @@ def test_admission_by_footprint():
 
 
 def test_mixed_request_falls_back_to_fp64_when_only_that_fits():
-    """A mixed-precision window whose footprint exceeds the whole budget runs
+    """An int8-residual window whose footprint exceeds the whole budget runs
     with the complex128-GEMM residual (no int8 residues of T: 24 instead of
     56 bytes per matrix entry) when that fits, else in FP64 (16 bytes) when
     that fits, with the same eigenvalues; the per-request outcome of a stream
     includes the rejection of a window that fits in none."""
+    import dataclasses
     from qbhip.beyn import footprint
     from qbhip.serve import Server, WindowRequest, summarize
-    req = _reqs()[0]
+    req = dataclasses.replace(_reqs()[0], precision="mixed_int8res")
     srv0 = Server(slots=1)
-    b, Ns, N, r, fp_mixed, prec = srv0._prepare((req, None))
+    b, Ns, N, r, fp_int8, prec = srv0._prepare((req, None))
     ref = srv0.submit(req).result()
     srv0.close()
-    assert prec == "mixed"
+    assert prec == "mixed_int8res"
     fp64 = footprint(N, req.nq, "fp64", r)
-    fpres = footprint(N, req.nq, "mixed_fp64res", r)
-    assert fp64 < fpres < fp_mixed
-    srv = Server(slots=1, hbm_budget=(fpres + fp_mixed) // 2)
+    fpres = footprint(N, req.nq, "mixed", r)
+    assert fp64 < fpres < fp_int8
+    srv = Server(slots=1, hbm_budget=(fpres + fp_int8) // 2)
     got = srv.submit(req).result()
     srv.close()
-    assert got.stats["precision_used"] == "mixed_fp64res"
+    assert got.stats["precision_used"] == "mixed"
     assert len(got.inside) == len(ref.inside)
     assert np.max(np.abs(got.inside - ref.inside) / np.abs(ref.inside)) <= 1e-10
     srv = Server(slots=1, hbm_budget=(fp64 + fpres) // 2)
