@@ -221,6 +221,26 @@ int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z,
                 int64_t nz, const qb_cheb_params* cheb, double* out,
                 int32_t flags, double* rminmax);
 
+/* qb_assemble that allocates no device memory: for callers that cannot
+ * allocate (code that runs under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * z_dev (nz complex) and out_dev (the nz * irreps matrices of qb_assemble's
+ * layout) are device memory the caller owns; flags must be 0 (the J-table
+ * kernel; no conjugate pairing).  The Chebyshev tables, the only memory
+ * beyond the inputs and outputs, are built in a fixed device array of
+ * QB_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on
+ * until the process ends.  They take
+ *   nz * (n_panels_h * (M_h + 1) + n_panels_j * (M_j + 1)) * 32 bytes;
+ * a call that needs more is rejected with QB_E_ARG (assemble fewer
+ * wavenumbers per call).  Every call on one GPU shares that GPU's array and
+ * overwrites the tables it reads (nothing needs clearing): calls that overlap
+ * on one GPU, from different contexts or threads, race and must be
+ * serialised by the caller.  Complete on return, like qb_assemble. */
+#define QB_NOALLOC_SCRATCH_BYTES ((int64_t)128 << 20)
+int qb_assemble_noalloc(qb_ctx* ctx, const qb_geom* g, int32_t op, const double* z_dev,
+                        int64_t nz, const qb_cheb_params* cheb, double* out_dev,
+                        int32_t flags, double* rminmax);
+
 /* The first half of construct_B_matrix (beyn.jl:257-277): contour nodes
  * z_j = k0 + R e^{i 2pi j/nq}, assembly of T(z_j), LU of every node.  The
  * factors stay resident in the window so the moment solve can be repeated

@@ -8,6 +8,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "qb_internal.h"
@@ -123,6 +125,32 @@ void qb_prof_end(qb_ctx* ctx, int slot) {
   (void)hipEventRecord(ctx->prof_recs[slot].b, ctx->stream);
 }
 
+// ---------------------------------------------------------------------------
+// device array of qb_assemble_noalloc
+// ---------------------------------------------------------------------------
+// The Chebyshev tables of a no-allocation assembly live in this fixed array;
+// every GPU the library runs on holds its own copy until the process ends.
+// Host code never passes the array's name to a kernel (in host code it is a
+// stand-in, not a device address): qb_ctx_create looks its device address up
+// once per device (hipGetSymbolAddress, with the device current) and every
+// use goes through that address.
+static __device__ __align__(256) unsigned char scratch[QB_NOALLOC_SCRATCH_BYTES];
+#define QB_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[QB_MAX_DEVICES] = {nullptr};
+static hipError_t g_scratch_err[QB_MAX_DEVICES] = {hipSuccess};
+
+// With `device` current: the address of `scratch` on it, looked up once.
+static void scratch_setup(int device) {
+  if (device < 0 || device >= QB_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  if (e == hipSuccess) g_scratch_ptr[device] = p;
+}
+
 static double now_ms() {
   return std::chrono::duration<double, std::milli>(
              std::chrono::steady_clock::now().time_since_epoch())
@@ -151,6 +179,7 @@ int qb_ctx_create(int device, qb_ctx** out) {
     delete c;
     return QB_E_HIP;
   }
+  scratch_setup(device);
   *out = c;
   return QB_OK;
 }
@@ -723,6 +752,57 @@ extern "C" int qb_assemble(qb_ctx* ctx, const qb_geom* g, int32_t op, const doub
   if (!on_dev && od) qb_dev_free(ctx, od, obytes);
   qb_tables_free(ctx, &t);
   qb_dev_free(ctx, zd, nz * sizeof(cd));
+  return rc;
+}
+
+extern "C" int qb_assemble_noalloc(qb_ctx* ctx, const qb_geom* g, int32_t op,
+                                   const double* z_dev, int64_t nz, const qb_cheb_params* cheb,
+                                   double* out_dev, int32_t flags, double* rminmax) {
+  if (!ctx) return QB_E_ARG;
+  if (!g || !z_dev || !out_dev || nz < 0) return qb_fail(ctx, QB_E_ARG, "null argument");
+  if (g->ctx != ctx) return qb_fail(ctx, QB_E_ARG, "geometry belongs to another context");
+  if (op != QB_CFIE_KRESS && op != QB_DLP_KRESS) return qb_fail(ctx, QB_E_ARG, "unknown operator");
+  if (op == QB_DLP_KRESS && g->d.ncomp != 1)
+    return qb_fail(ctx, QB_E_ARG, "DLP_kress expects one boundary component");
+  if (flags != 0)
+    return qb_fail(ctx, QB_E_ARG,
+                   "qb_assemble_noalloc takes no flags (device in and out, J tables)");
+  int rc = check_cheb(ctx, cheb);
+  if (rc) return rc;
+  const size_t need = qb_tables_bytes(nz, *cheb, nullptr);
+  if (need > (size_t)QB_NOALLOC_SCRATCH_BYTES)
+    return qb_fail(ctx, QB_E_ARG,
+                   "qb_assemble_noalloc: the tables of " + std::to_string(nz) +
+                       " wavenumbers need " + std::to_string(need) +
+                       " bytes, more than the fixed device array of " +
+                       std::to_string((long long)QB_NOALLOC_SCRATCH_BYTES) +
+                       " bytes; assemble fewer wavenumbers per call");
+  if (ctx->device < 0 || ctx->device >= QB_MAX_DEVICES)
+    return qb_fail(ctx, QB_E_ARG, "device index out of range for the no-allocation array");
+  void* mem = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    mem = g_scratch_ptr[ctx->device];
+    if (!mem)
+      return qb_fail(ctx, QB_E_HIP,
+                     std::string("no device address for the no-allocation array: ") +
+                         hipGetErrorString(g_scratch_err[ctx->device]));
+  }
+  double rmin, rmax;
+  op_range(g, op, &rmin, &rmax);
+  if (rminmax) {
+    rminmax[0] = rmin;
+    rminmax[1] = rmax;
+  }
+  if (nz == 0) return QB_OK;
+  (void)hipSetDevice(ctx->device);
+  DevTables t;
+  rc = qb_tables_build_in(ctx, z_dev, nz, rmin, rmax, *cheb, mem,
+                          (size_t)QB_NOALLOC_SCRATCH_BYTES, &t);
+  if (!rc) rc = qb_assemble_dev(ctx, g, op, (const cd*)z_dev, nz, t, (cd*)out_dev, nullptr);
+  // complete on return, as qb_assemble is
+  const hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (!rc && e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
   return rc;
 }
 

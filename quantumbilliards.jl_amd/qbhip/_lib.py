@@ -30,7 +30,7 @@ EXPORTS = (
     "qb_geom_set_orbits", "qb_geom_dim", "qb_host_alloc", "qb_host_free",
     "qb_geom_set_orbits_multi", "qb_geom_irreps", "qb_window_basis_irrep",
     "qb_window_create_multi", "qb_window_blocks", "qb_window_matrix_cols",
-    "qb_ctx_kernel_issued",
+    "qb_ctx_kernel_issued", "qb_assemble_noalloc",
 )
 KERNEL_CLASSES = ("tables", "assemble", "gemm_c64", "gemm_c128", "panel", "trsm", "other",
                   "ozaki", "gemm_c64_bf16")
@@ -96,6 +96,7 @@ def lib():
         "qb_geom_irreps": (I32, [P]),
         "qb_window_basis_irrep": (C.c_int, [P, I32, P, I32, P]),
         "qb_assemble": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P]),
+        "qb_assemble_noalloc": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, I32, P]),
         "qb_assemble_deriv": (C.c_int, [P, P, I32, P, I64, C.POINTER(ChebParams), P, P, P, I32,
                                         P]),
         "qb_window_create": (C.c_int, [P, P, I32, D, D, D, I32, C.POINTER(ChebParams), I32, PP,
