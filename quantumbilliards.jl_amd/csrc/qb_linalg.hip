@@ -739,6 +739,19 @@ __device__ __forceinline__ T cmul_(T a, T b) {
   return T{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 
+// A[i] when ok, else 0, loaded component by component (a conditional
+// aggregate load can leave a per-thread temporary in memory, which the
+// compiler then places in LDS, 16 bytes per thread of the workgroup).
+template <typename T>
+__device__ __forceinline__ T ld_or0(const T* p, bool ok) {
+  T v{0, 0};
+  if (ok) {
+    v.re = p->re;
+    v.im = p->im;
+  }
+  return v;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, int64_t sA,
                                                 int n, int c0, int w, int* __restrict__ ipiv,
@@ -761,7 +774,7 @@ __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, 
     T* Ac = A + (int64_t)col * lda;
     // (1) U(c0 + k, col) = L11^{-1} A(c0.., col), k < c, on wave 0
     if (wv == 0) {
-      T a = (lane < c) ? Ac[c0 + lane] : T{0, 0};
+      T a = ld_or0(Ac + c0 + lane, lane < c);
       for (int k = 0; k < c; ++k) {
         T uk = T{__shfl(a.re, k), __shfl(a.im, k)};
         if (lane > k && lane < c) {
@@ -783,7 +796,7 @@ __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, 
       const int r2 = r + nt;
       const bool two = r2 < n;
       T a0 = Ac[r];
-      T a1 = two ? Ac[r2] : T{0, 0};
+      T a1 = ld_or0(Ac + r2, two);
       const T* Lr0 = L0 + r;
       int k = 0;
       for (; k + 4 <= c; k += 4) {
@@ -791,7 +804,7 @@ __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, 
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           l[q] = Lr0[(int64_t)(k + q) * lda];
-          m[q] = two ? Lr0[(int64_t)(k + q) * lda + nt] : T{0, 0};
+          m[q] = ld_or0(Lr0 + (int64_t)(k + q) * lda + nt, two);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -805,7 +818,7 @@ __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, 
       for (; k < c; ++k) {
         T uk = u[k];
         T l = Lr0[(int64_t)k * lda];
-        T m = two ? Lr0[(int64_t)k * lda + nt] : T{0, 0};
+        T m = ld_or0(Lr0 + (int64_t)k * lda + nt, two);
         a0.re -= l.re * uk.re - l.im * uk.im;
         a0.im -= l.re * uk.im + l.im * uk.re;
         a1.re -= m.re * uk.re - m.im * uk.im;
@@ -857,10 +870,17 @@ __global__ void __launch_bounds__(1024) k_getf2(T* __restrict__ A, int64_t lda, 
     // L part becomes row c of the top block
     for (int cc = tid; cc < w; cc += nt) {
       T* Acc = A + (int64_t)(c0 + cc) * lda;
-      T x = Acc[piv];
+      // component by component: a struct copy here keeps x in memory (16
+      // bytes of LDS per thread once the compiler promotes it)
+      T x;
+      x.re = Acc[piv].re;
+      x.im = Acc[piv].im;
       if (piv != col) {
-        Acc[piv] = Acc[col];
-        Acc[col] = x;
+        const auto yr = Acc[col].re, yi = Acc[col].im;
+        Acc[piv].re = yr;
+        Acc[piv].im = yi;
+        Acc[col].re = x.re;
+        Acc[col].im = x.im;
       }
       prow[cc] = x;
       if (cc < c) Lt[c][cc] = x;
@@ -891,10 +911,13 @@ __global__ void k_laswp(T* __restrict__ A, int64_t lda, int64_t sA, int cb, int 
   ipiv += b * sP;
   for (int s = k0; s < k1; ++s) {
     int p = ipiv[s];
-    if (p != s) {
-      T t = A[s];
-      A[s] = A[p];
-      A[p] = t;
+    if (p != s) {  // component by component (no per-thread temporary in memory)
+      const auto tr = A[s].re, ti = A[s].im;
+      const auto pr = A[p].re, pi = A[p].im;
+      A[s].re = pr;
+      A[s].im = pi;
+      A[p].re = tr;
+      A[p].im = ti;
     }
   }
 }
@@ -1068,20 +1091,38 @@ static int trsm_upper(qb_ctx* ctx, int n, int nrhs, const T* U, int64_t ldu, int
   return trsm_upper(ctx, n1, nrhs, U, ldu, sU, B, ldb, sB, batch);
 }
 
-// Recursive panel: factor columns [c0, c0+w) (rows c0..n-1) of every matrix.
+// Threads of a leaf workgroup that runs beside the trailing update.  Two
+// workgroups of the complex128 trailing-update tiles (70 KB of LDS, two waves
+// per SIMD at 128 VGPRs) leave a CU 20 KB of LDS and two waves per SIMD: the
+// complex128 leaf (18 KB, 94 VGPRs) fits there with 512 threads, and with
+// 1024 it waits for the whole trailing update to drain.  QB_PANEL_THREADS (a
+// multiple of 64 in [64, 1024], read once) overrides it.
+template <typename T>
+static int panel_threads_beside() {
+  static const int env = [] {
+    const char* e = getenv("QB_PANEL_THREADS");
+    long v = e ? strtol(e, nullptr, 10) : 0;
+    return (v >= 64 && v <= 1024 && v % 64 == 0) ? (int)v : 0;
+  }();
+  if (env) return env;
+  return sizeof(T) == sizeof(cd) ? 512 : 1024;
+}
+
+// Recursive panel: factor columns [c0, c0+w) (rows c0..n-1) of every matrix,
+// leaves with `threads` threads per workgroup.
 template <typename T>
 static int panel_rec(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int n, int c0, int w, int* ipiv,
-                     int64_t sP, int* info, int batch) {
+                     int64_t sP, int* info, int batch, int threads = 1024) {
   if (w <= 32) {
     int slot = qb_prof_begin(ctx, QB_K_PANEL, 8.0 * (double)(n - c0) * w * w / 2 * batch, 0);
-    hipLaunchKernelGGL(k_getf2<T>, dim3(batch), dim3(1024), 0, ctx->stream, A, lda, sA, n, c0, w,
-                       ipiv, sP, info);
+    hipLaunchKernelGGL(k_getf2<T>, dim3(batch), dim3(threads), 0, ctx->stream, A, lda, sA, n, c0,
+                       w, ipiv, sP, info);
     qb_prof_end(ctx, slot);
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
   }
   int w1 = split32(w);
-  int rc = panel_rec(ctx, A, lda, sA, n, c0, w1, ipiv, sP, info, batch);
+  int rc = panel_rec(ctx, A, lda, sA, n, c0, w1, ipiv, sP, info, batch, threads);
   if (rc) return rc;
   rc = laswp(ctx, A, lda, sA, c0 + w1, c0 + w, c0, c0 + w1, ipiv, sP, batch);
   if (rc) return rc;
@@ -1093,7 +1134,7 @@ static int panel_rec(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int n, int c0, 
                    A + c0 + (int64_t)(c0 + w1) * lda, lda, sA,
                    A + c0 + w1 + (int64_t)(c0 + w1) * lda, lda, sA, batch);
   if (rc) return rc;
-  rc = panel_rec(ctx, A, lda, sA, n, c0 + w1, w - w1, ipiv, sP, info, batch);
+  rc = panel_rec(ctx, A, lda, sA, n, c0 + w1, w - w1, ipiv, sP, info, batch, threads);
   if (rc) return rc;
   return laswp(ctx, A, lda, sA, c0, c0 + w1, c0 + w1, c0 + w, ipiv, sP, batch);
 }
@@ -1152,7 +1193,10 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
     if (rc) return rc;
     QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
     QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
-    rc = on_aux([&] { return panel_rec(ctx, A, lda, sA, n, k1, kb1, ipiv, sP, info, batch); });
+    rc = on_aux([&] {
+      return panel_rec(ctx, A, lda, sA, n, k1, kb1, ipiv, sP, info, batch,
+                       m > kb1 ? panel_threads_beside<T>() : 1024);
+    });
     if (rc) return rc;
     QB_HIP(ctx, hipEventRecord(ctx->ev_aux, aux));
     if (m > kb1) {
