@@ -73,6 +73,10 @@ struct qb_ctx {
   // the lookahead panel factorisation on the second stream.
   int gemm_pick_conc[2][QB_GEMM_CLASSES];
   bool gemm_conc = false;
+  // set while a complex128 LU panel runs beside the trailing update: its
+  // triangular-solve leaves and GEMMs take the variants that fit on a CU
+  // next to two trailing-update workgroups (no LDS, 19 KB of LDS)
+  bool panel_lean = false;
   // Node list (device array) for batched launches: when set, batch entry z
   // of a GEMM / TRSM / gather / node-wise helper works on matrix bix[z]
   // (strides unchanged), so a subset of a window's nodes runs in one launch.

@@ -549,22 +549,25 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 64, 32, 16, 2, 2, false, true>, launch_gemm<cd, 64, 64, 16, 2, 2, true, false, 2>,
     launch_gemm<cd, 64, 32, 16, 2, 2, false, true, 2>, launch_gemm<cd, 64, 64, 16, 2, 2, false, true, 1, 2>,
     launch_gemm<cd, 64, 64, 16, 2, 2, true, true, 2, 2>,
-    // no bf16-split counterpart for complex128: the slots repeat the defaults
-    // and the tuner skips them (kGemmAvail)
-    launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 64, 16, 2, 2>,
+    // 13: the lean tile of the LU panel beside the trailing update (19 KB of
+    // LDS; always taken through qb_ctx::panel_lean, and open to the tuner).  No bf16-split
+    // counterpart for complex128: the other slots repeat the default and the
+    // tuner skips them (kGemmAvail)
+    launch_gemm<cd, 32, 32, 8, 2, 2, false, true>, launch_gemm<cd, 128, 64, 16, 2, 2>,
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 64, 16, 2, 2>};
+#define QB_GEMM_LEAN_C128 13
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
      "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
-     "64x64x16/3m/xcd/pf2", "-", "-", "-", "-"}};
+     "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "-", "-", "-"}};
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
      true},
-    {true, true, true, true, true, true, true, true, true, true, true, true, true, false, false,
+    {true, true, true, true, true, true, true, true, true, true, true, true, true, true, false,
      false, false}};
 // Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
 static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
@@ -577,7 +580,7 @@ static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
 static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
      false, false, false},
-    {false, false, false, false, false, false, false, true, true, false, true, true, true, false,
+    {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
      false, false, false}};
 
 // Variants that pack 4 waves per SIMD (small tiles, low register and LDS
@@ -619,7 +622,10 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
   const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
   int v = (ctx->gemm_conc ? ctx->gemm_pick_conc : ctx->gemm_pick)[ty][qb_gemm_class(M, N, K)];
-  if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v]) v = gemm_default<T>(N);
+  if (ty == 1 && ctx->panel_lean)
+    v = QB_GEMM_LEAN_C128;
+  else if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v])
+    v = gemm_default<T>(N);
   const int kind = ty == 0 ? (kGemmBf6[ty][v] ? QB_K_GEMM_C64_BF16 : QB_K_GEMM_C64) : QB_K_GEMM_C128;
   int slot = qb_prof_begin(ctx, kind, 8.0 * M * N * K * (double)batch,
                            (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
@@ -1045,10 +1051,62 @@ static int split32(int n) {
 }
 
 // X = L^{-1} B with L the unit-lower n x n block at L (ld ldl).
+// Unit-lower triangular-solve leaf (n <= 32 rows) without LDS, for the LU
+// panel beside the trailing update: each lane holds its right-hand-side
+// column in registers and reads the triangle with wave-uniform loads.  Row i
+// subtracts L(i, j) x_j in increasing j, the order of k_trsm_leaf.
+template <typename T>
+__global__ void __launch_bounds__(64) k_trsm_leaf_lower_reg(int n, int nrhs, const T* __restrict__ L,
+                                                           int64_t ldl, int64_t sL,
+                                                           T* __restrict__ B, int64_t ldb,
+                                                           int64_t sB, const int* __restrict__ bix) {
+  typedef typename Scal<T>::S S;
+  const int64_t b = bix ? bix[blockIdx.y] : blockIdx.y;
+  L += b * sL;
+  B += b * sB;
+  const int col = blockIdx.x * 64 + threadIdx.x;
+  if (col >= nrhs) return;
+  T* Bc = B + (int64_t)col * ldb;
+  S xr[32], xi[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    xr[i] = i < n ? Bc[i].re : S(0);
+    xi[i] = i < n ? Bc[i].im : S(0);
+  }
+#pragma unroll
+  for (int j = 0; j < 31; ++j) {
+    if (j + 1 < n) {
+#pragma unroll
+      for (int i = j + 1; i < 32; ++i) {
+        if (i < n) {
+          const S lr = L[i + (int64_t)j * ldl].re, li = L[i + (int64_t)j * ldl].im;
+          xr[i] -= lr * xr[j] - li * xi[j];
+          xi[i] -= lr * xi[j] + li * xr[j];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 32; ++i)
+    if (i < n) {
+      Bc[i].re = xr[i];
+      Bc[i].im = xi[i];
+    }
+}
+
 template <typename T>
 static int trsm_lower(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int64_t sL, T* B,
                       int64_t ldb, int64_t sB, int batch) {
   if (n <= 0 || nrhs <= 0) return QB_OK;
+  if (n <= 32 && ctx->panel_lean) {
+    dim3 g((nrhs + 63) / 64, batch);
+    int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
+    hipLaunchKernelGGL((k_trsm_leaf_lower_reg<T>), g, dim3(64), 0, ctx->stream, n, nrhs, L, ldl,
+                       sL, B, ldb, sB, ctx->bix);
+    qb_prof_end(ctx, slot);
+    QB_HIP(ctx, hipGetLastError());
+    return QB_OK;
+  }
   if (n <= 32) {
     dim3 g((nrhs + 63) / 64, batch);
     int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
@@ -1106,6 +1164,16 @@ static int panel_threads_beside() {
   }();
   if (env) return env;
   return sizeof(T) == sizeof(cd) ? 512 : 1024;
+}
+
+// The lean kernels for a complex128 panel beside the trailing update
+// (qb_ctx::panel_lean); QB_PANEL_LEAN=0 (read once) turns them off.
+static bool panel_lean_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("QB_PANEL_LEAN");
+    return !(e && e[0] == '0');
+  }();
+  return on;
 }
 
 // Recursive panel: factor columns [c0, c0+w) (rows c0..n-1) of every matrix,
@@ -1194,8 +1262,11 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
     QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
     QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
     rc = on_aux([&] {
-      return panel_rec(ctx, A, lda, sA, n, k1, kb1, ipiv, sP, info, batch,
-                       m > kb1 ? panel_threads_beside<T>() : 1024);
+      ctx->panel_lean = sizeof(T) == sizeof(cd) && m > kb1 && panel_lean_enabled();
+      const int r = panel_rec(ctx, A, lda, sA, n, k1, kb1, ipiv, sP, info, batch,
+                              m > kb1 ? panel_threads_beside<T>() : 1024);
+      ctx->panel_lean = false;
+      return r;
     });
     if (rc) return rc;
     QB_HIP(ctx, hipEventRecord(ctx->ev_aux, aux));

@@ -104,7 +104,9 @@ def test_every_tile_variant(probe, variant):
     stage k-tiles two ahead (K = 33, 40, 128 cover one, two and an odd number
     of k-tiles); complex64 13-16 run on the bf16 matrix cores with three-way
     bf16 splits of the FP32 operands (32-deep k-tiles: K = 33 and 40 have a
-    partial second k-tile), within the FP32 tolerance."""
+    partial second k-tile), within the FP32 tolerance; complex128 13 is the
+    lean 32 x 32 tile with 8-deep k-tiles of the LU panel that runs beside
+    the trailing update (K = 33 and 40 end in partial k-tiles)."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
