@@ -96,6 +96,10 @@ def parse():
     ap.add_argument("--uneven-max-group", type=int, default=16)
     ap.add_argument("--group", action="store_true",
                     help="batch windows that share N into one window (qb_window_create_multi)")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay each window's LU from a captured hipGraph (qb_ctx_set_graphs)")
+    ap.add_argument("--no-kernel-stats", action="store_true",
+                    help="time without the library's per-launch HIP events (no roofline fields)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (read once, written once): "
                          "the calibration of HBM byte counters in rocprofv3 --pmc runs")
@@ -733,6 +737,9 @@ def main():
         c2.autotune(a.N, a.nq, r_used, "fp64")
         c2.autotune(a.N, a.nq, r_used, "mixed")
         lanes.append((c2, q.Geometry(pts, c2)))
+    if a.graphs:
+        for c, _ in lanes:
+            c.set_graphs(True)
     V = q.probe_block(np.random.default_rng(1), a.N, r_used, pinned=True)
     # both full-accuracy paths over the same windows: the complex128 path and
     # the complex64 LU refined to refine_tol; the faster one is the headline
@@ -740,7 +747,7 @@ def main():
     runs = {}
     for prec in paths:
         for c, _ in lanes:
-            c.set_profiling(True)
+            c.set_profiling(not a.no_kernel_stats)
         res, t_span = pipelined(q, ctx, solver, pts, geom, a, cheb, prec, V, a.steps, lanes=lanes)
         ks = lanes[0][0].kernel_stats()
         for c, _ in lanes:
