@@ -553,8 +553,8 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     // LDS; always taken through qb_ctx::panel_lean, and open to the tuner).  No bf16-split
     // counterpart for complex128: the other slots repeat the default and the
     // tuner skips them (kGemmAvail)
-    launch_gemm<cd, 32, 32, 8, 2, 2, false, true>, launch_gemm<cd, 128, 64, 16, 2, 2>,
-    launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 64, 16, 2, 2>};
+    launch_gemm<cd, 32, 32, 8, 2, 2, false, true>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 2, 3>,
+    launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 4>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 3>};
 #define QB_GEMM_LEAN_C128 13
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
@@ -562,13 +562,14 @@ static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
      "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
-     "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "-", "-", "-"}};
+     "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "64x64x8/3m/xcd/pf2/occ3", "64x64x8/3m/xcd/occ4",
+     "64x64x8/3m/xcd/occ3"}};
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
      true},
-    {true, true, true, true, true, true, true, true, true, true, true, true, true, true, false,
-     false, false}};
+    {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
+     true, true}};
 // Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
 static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
@@ -581,15 +582,17 @@ static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
      false, false, false},
     {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
-     false, false, false}};
+     true, true, true}};
 
-// Variants that pack 4 waves per SIMD (small tiles, low register and LDS
-// use): fastest alone, but they leave a concurrently running kernel (the
-// lookahead panel) few issue slots, so gemm_pick_conc excludes them.
+// Variants that pack 3-4 waves per SIMD (small tiles or k-tiles, low
+// register and LDS use): fastest alone, but they leave a concurrently
+// running kernel (the lookahead panel) no room on a CU, so gemm_pick_conc
+// excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, true, true, true, false,
      false, false, false},
-    {false}};
+    {false, false, false, false, false, false, false, false, false, false, false, false, false,
+     false, true, true, true}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);

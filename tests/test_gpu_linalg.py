@@ -106,7 +106,9 @@ def test_every_tile_variant(probe, variant):
     bf16 splits of the FP32 operands (32-deep k-tiles: K = 33 and 40 have a
     partial second k-tile), within the FP32 tolerance; complex128 13 is the
     lean 32 x 32 tile with 8-deep k-tiles of the LU panel that runs beside
-    the trailing update (K = 33 and 40 end in partial k-tiles)."""
+    the trailing update, and complex128 14-16 are 64 x 64 Gauss tiles with
+    8-deep k-tiles at 3-4 waves per SIMD (prefetch 2 for 14); K = 33 and 40
+    end in partial k-tiles."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
