@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=1.0, help="arrivals per second")
     ap.add_argument("--serve-kmax", type=float, default=1000.0)
     ap.add_argument("--serve-slots", type=int, default=2)
-    ap.add_argument("--serve-precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--serve-irreps-separate", dest="serve_irreps_batched",
                     action="store_false",
                     help="with --serve-symmetry: one job per irrep instead of one window")
@@ -132,12 +132,13 @@ def run_serve(a):
     ellipse (1, 0.6), the stadium (radius 1, half-length 1) and the star,
     k0 log-uniform in [50, kmax], R in [0.25, 0.5], tol 1e-8 or 1e-12, N from
     10 points per wavelength, 32 contour nodes, mixed precision; admission by
-    HBM footprint into `slots` device contexts.  One untimed warmup window
-    first.  Prints one JSON line with eigenvalues/s and p50/p99 latency."""
+    HBM footprint into `slots` device contexts.  Untimed warmup windows first
+    (one per slot, then one of the stream's kind).  Prints one JSON line with eigenvalues/s and p50/p99 latency."""
     from qbhip import serve as S
     reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax,
                             symmetry=a.serve_symmetry, precision=a.serve_precision)
     srv = S.Server(slots=a.serve_slots, irreps_batched=a.serve_irreps_batched)
+    srv.warm_up()
     srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32,
                                symmetry=a.serve_symmetry)).result()
     t0 = time.perf_counter()
@@ -230,6 +231,7 @@ def run_uneven(a):
     reqs = S.uneven_windows(a.uneven_windows, seed=4, precision=a.uneven_precision,
                             kmax=a.uneven_kmax)
     srv = S.Server(slots=a.uneven_slots, group=a.group, max_group=a.uneven_max_group)
+    srv.warm_up()
     srv.submit(S.WindowRequest(-1, "circle_hole", 100.0, 0.5, nq=32,
                                precision=a.uneven_precision,
                                N=S.hole_node_counts(100.5))).result()

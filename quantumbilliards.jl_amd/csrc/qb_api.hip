@@ -27,9 +27,13 @@ int qb_fail(qb_ctx* ctx, int code, const std::string& msg) {
 // by exact size (windows of one size allocate the same list of buffers), up
 // to cache_limit bytes; an allocation that fails releases the cache and
 // retries once.  `bytes` counts memory handed out plus cached.
+// Device memory is allocated and released in stream order on the context's
+// stream (hipMallocAsync / hipFreeAsync): hipFree waits for every kernel on
+// the device, so with several contexts on one GPU (the serving loop's slots)
+// a temporary freed by one context would wait for the other's long LU.
 static void cache_release(qb_ctx* ctx) {
   for (auto& e : ctx->cache) {
-    (void)hipFree(e.second);
+    (void)hipFreeAsync(e.second, ctx->stream);
     ctx->bytes -= (int64_t)e.first;
   }
   ctx->cache.clear();
@@ -47,18 +51,19 @@ int qb_dev_alloc(qb_ctx* ctx, void** p, size_t bytes) {
       return QB_OK;
     }
   }
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = hipMallocAsync(p, bytes, ctx->stream);
   if (e != hipSuccess && !ctx->cache.empty()) {
     (void)hipGetLastError();
-    (void)hipStreamSynchronize(ctx->stream);
     cache_release(ctx);
-    e = hipMalloc(p, bytes);
+    // the released blocks return to the pool once this stream reaches them
+    (void)hipStreamSynchronize(ctx->stream);
+    e = hipMallocAsync(p, bytes, ctx->stream);
   }
   if (e != hipSuccess) {
     *p = nullptr;
     (void)hipGetLastError();
     return qb_fail(ctx, QB_E_NOMEM,
-                   "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                   "hipMallocAsync of " + std::to_string(bytes) + " bytes failed: " +
                        hipGetErrorString(e));
   }
   ctx->bytes += (int64_t)bytes;
@@ -75,7 +80,7 @@ void qb_dev_free(qb_ctx* ctx, void* p, size_t bytes) {
     ctx->cache_bytes += (int64_t)bytes;
     return;
   }
-  (void)hipFree(p);
+  (void)hipFreeAsync(p, ctx->stream);
   ctx->bytes -= (int64_t)bytes;
 }
 
@@ -196,6 +201,7 @@ void qb_ctx_destroy(qb_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   qb_graphs_clear(ctx);
   cache_release(ctx);
+  (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ev_main) (void)hipEventDestroy(ctx->ev_main);
   if (ctx->ev_aux) (void)hipEventDestroy(ctx->ev_aux);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

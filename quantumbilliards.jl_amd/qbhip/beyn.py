@@ -206,14 +206,17 @@ def B_from_moments(A0: np.ndarray, A1: np.ndarray, svd_tol: float):
     return B, Uk, S, rk
 
 
-def B_from_qr(Rm: np.ndarray, Cm: np.ndarray, svd_tol: float):
+def B_from_qr(Rm: np.ndarray, Cm: np.ndarray, svd_tol: float, rel_tol: float = 0.0):
     """B from the device QR of the moments (A0 = Q Rm, Cm = (Q^H A1)[:r]):
     svd(Rm) = U_R S W^H has the singular values and W of svd(A0), and
     B = U_R[:, :k]^H Cm W[:, :k] S_k^{-1} equals U_k^H A1 W_k S_k^{-1}
-    (beyn.jl:286-321).  Returns (B, U_R[:, :k], S, rk); U_k = Q U_R[:, :k]
-    comes from Window.basis."""
+    (beyn.jl:286-321).  The rank k counts singular values >= svd_tol and
+    >= rel_tol sigma_max (rel_tol = 0: the reference's absolute threshold).
+    Returns (B, U_R[:, :k], S, rk); U_k = Q U_R[:, :k] comes from
+    Window.basis."""
     Ur, S, Wh = np.linalg.svd(Rm)
-    rk = int(np.count_nonzero(S >= svd_tol))
+    thr = max(svd_tol, rel_tol * float(S[0])) if len(S) else svd_tol
+    rk = int(np.count_nonzero(S >= thr))
     if rk == 0:
         return np.zeros((0, 0), np.complex128), Ur[:, :0], S, 0
     Wk = Wh.conj().T[:, :rk]

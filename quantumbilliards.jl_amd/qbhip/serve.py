@@ -7,14 +7,17 @@ streams), so windows admitted together overlap on the GPU.  A request is
 admitted when a slot is free and its HBM footprint (the N^2 matrices of all
 live contour nodes plus the N x r solve buffers, `qbhip.footprint`) fits in
 the budget next to the windows already running; requests that do not fit
-wait while smaller ones behind them may start (first fit).  A mixed-precision
-request larger than the whole budget runs with the complex128-GEMM residual
-(24 instead of 56 bytes per matrix entry: no int8 residues of T), or else in
-FP64 (16 bytes), when that footprint fits; one that fits in none is rejected
-with MemoryError.  The default budget is the device's free memory
+wait while smaller ones behind them may start (first fit).  Requests are
+FP64 by default (on MI355X the faster path at these probe counts); a request
+for the int8-residual mixed path larger than the whole budget runs with the
+complex128-GEMM residual (24 instead of 56 bytes per matrix entry: no int8
+residues of T), and a mixed one in FP64 (16 bytes), when that footprint fits;
+one that fits in none is rejected with MemoryError.  The default budget is the device's free memory
 at start-up less 12 GiB.  The host part of
 each window (thin SVD, B, eig) runs on a thread pool, overlapping the next
-windows' device work.  Latency is measured from arrival to result.
+windows' device work, with the BLAS threads shared out among the pool's
+workers.  Device memory is allocated in stream order on each slot's stream,
+so a slot's frees never wait for another slot's kernels.  Latency is measured from arrival to result.
 
 With group=True, when a slot frees up, the pending requests that share the
 admitted one's geometry and settings (shape, node counts, symmetry,
@@ -44,6 +47,7 @@ computation is the same construct_B_matrix + eigen!(B) as qbhip.beyn.
 from __future__ import annotations
 
 import math
+import os
 import threading
 import warnings
 import time
@@ -126,7 +130,7 @@ class WindowRequest:
     tol: float = 1e-12      # table tolerance and refinement backward error
     nq: int = 32
     ppw: float = 10.0       # points per wavelength: N = ppw k L / 2pi (rounded up, even)
-    precision: str = "mixed"
+    precision: str = "fp64"  # or "mixed" / "mixed_int8res" (complex64 LU + FP64 refinement)
     svd_tol: float = 1e-10
     seed: int = 0
     arrival: float = 0.0    # seconds after the stream starts
@@ -191,6 +195,17 @@ class Server:
         self.cheb_cache: Dict = {}       # (shape, N tuple, disks, nq, tol) -> cheb params
         self.prep_memo: Dict = {}        # id(job) -> (job, _prepare result)
         self.host_pool = ThreadPoolExecutor(max_workers=host_threads)
+        # host stages of several windows run at once: give each its share of
+        # the cores instead of letting every LAPACK call use all of them
+        # (restored in close)
+        self._blas_limit = None
+        try:
+            from threadpoolctl import threadpool_limits
+            ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else 4
+            self._blas_limit = threadpool_limits(limits=max(1, ncpu // host_threads),
+                                                 user_api="blas")
+        except Exception:
+            self._blas_limit = None
         self.workers: List[threading.Thread] = []
         self.closed = False
         self.dispatcher = threading.Thread(target=self._dispatch, daemon=True)
@@ -253,6 +268,15 @@ class Server:
                 out.append(e)
         return out
 
+    def warm_up(self):
+        """One small window (the ellipse (1, 0.6) at k0 = 60) in every slot at
+        once, so the first request of each slot does not pay for loading the
+        library's kernels into its device context."""
+        futs = [self.submit(WindowRequest(-1 - i, "ellipse", 60.0, 0.3, tol=1e-8, nq=32))
+                for i in range(len(self.slots))]
+        for f in futs:
+            f.result()
+
     def close(self):
         with self.lock:
             self.closed = True
@@ -261,6 +285,9 @@ class Server:
         for w in self.workers:
             w.join()
         self.host_pool.shutdown(wait=True)
+        if self._blas_limit is not None:
+            self._blas_limit.restore_original_limits()
+            self._blas_limit = None
         for g in self.geoms.values():
             g.close()
         for c in self.slots:
@@ -397,6 +424,7 @@ class Server:
             zj = np.concatenate([q.k0 + q.R * th for q in reqs])
             ck = (req.shape, Ns, tuple((round(q.k0, 3), round(q.R, 3)) for q in reqs), req.nq,
                   req.tol)
+            tp = time.perf_counter()
             if ck not in self.cheb_cache:
                 ttol = max(req.tol, 1e-13)
                 with warnings.catch_warnings():
@@ -406,16 +434,21 @@ class Server:
                 self.cheb_cache[ck] = (dict(n_panels_h=nh, M_h=mh, n_panels_j=nj, M_j=mj),
                                        bool(np.all(errs < ttol)))
             cheb, cheb_ok = self.cheb_cache[ck]
+            t_cheb = time.perf_counter() - tp
             # the probe block is drawn on the host pool while the window is
             # assembled and factored (the C calls release the interpreter)
             vf = self.host_pool.submit(probe_block, np.random.default_rng(req.seed), N, r)
+            tw = time.perf_counter()
             if len(reqs) == 1:
                 win = Window(solver, pts, req.k0, req.R, req.nq, precision=prec, geom=geom,
                              **cheb)
             else:
                 win = Window(solver, pts, [q.k0 for q in reqs], [q.R for q in reqs], req.nq,
                              precision=prec, geom=geom, **cheb)
+            t_create = time.perf_counter() - tw
+            tm = time.perf_counter()
             Rm, Cm = win.moments_qr(vf.result(), max(req.tol, 1e-13))
+            t_moments = time.perf_counter() - tm
             if win.nblocks == 1:
                 Rm, Cm = Rm[None], Cm[None]
             # Rank saturation: with rk == r every probe direction is used and
@@ -446,6 +479,8 @@ class Server:
             st["group"] = len(reqs)
             st["rank_saturated"] = saturated
             st["cheb_tol_met"] = cheb_ok
+            st["phases_s"] = {"setup": round(tw - t_admit - t_cheb, 4), "cheb": round(t_cheb, 4),
+                              "create": round(t_create, 4), "moments": round(t_moments, 4)}
         except Exception as e:  # release and report
             self._release(slot, fp)
             for _, fut, _ in entries:
@@ -465,9 +500,8 @@ class Server:
             th = time.perf_counter()
             lams, rk = [], 0
             for e in range(nirr):
-                Rb = Rm[e * nwin + c]
-                B, Ur, S, rk_e = B_from_qr(Rb, Cm[e * nwin + c],
-                                           rank_threshold(Rb, q.svd_tol, q.tol))
+                B, Ur, S, rk_e = B_from_qr(Rm[e * nwin + c], Cm[e * nwin + c], q.svd_tol,
+                                           RANK_NOISE * q.tol)
                 lams.append(np.linalg.eigvals(B) if B.size else np.zeros(0, np.complex128))
                 rk += rk_e
             lam = np.concatenate(lams)
@@ -575,7 +609,7 @@ def uneven_windows(n: int, seed: int = 0, kmin: float = 100.0, kmax: float = 800
 
 def poisson_stream(n: int, rate: float, seed: int = 0, shapes=("ellipse", "stadium", "star"),
                    kmin: float = 50.0, kmax: float = 1000.0, nq: int = 32,
-                   precision: str = "mixed", symmetry: bool = False) -> List[WindowRequest]:
+                   precision: str = "fp64", symmetry: bool = False) -> List[WindowRequest]:
     """Seeded request stream of the serving workload: Poisson arrivals, shapes
     drawn from `shapes`, k0 log-uniform in [kmin, kmax], R uniform in
     [0.25, 0.5], tol 1e-8 or 1e-12."""

@@ -188,7 +188,8 @@ def test_serving_stream_alone_equals_in_mix():
     for i, (shape, k0) in enumerate(spec):
         t += rng.exponential(0.25)
         reqs.append(S.WindowRequest(i, shape, k0, float(rng.uniform(0.25, 0.5)),
-                                    tol=(1e-8, 1e-12)[i % 2], nq=32, seed=300 + i, arrival=t))
+                                    tol=(1e-8, 1e-12)[i % 2], nq=32, seed=300 + i, arrival=t,
+                                    precision="mixed"))
     print([(rq.shape, rq.k0, round(rq.R, 3), rq.tol) for rq in reqs], flush=True)
     srv = S.Server(slots=1)
     alone = {rq.rid: srv.submit(rq).result() for rq in reqs}

@@ -12,11 +12,13 @@ pytestmark = pytest.mark.gpu
 def _reqs():
     from qbhip.serve import WindowRequest
     return [
-        WindowRequest(0, "circle", 20.0, 0.5, tol=1e-12, nq=24, ppw=13, seed=11),
-        WindowRequest(1, "ellipse", 24.0, 0.4, tol=1e-8, nq=24, ppw=12, seed=12),
+        WindowRequest(0, "circle", 20.0, 0.5, tol=1e-12, nq=24, ppw=13, seed=11, precision="mixed"),
+        WindowRequest(1, "ellipse", 24.0, 0.4, tol=1e-8, nq=24, ppw=12, seed=12, precision="mixed"),
         WindowRequest(2, "star", 18.0, 0.45, tol=1e-12, nq=24, ppw=12, seed=13, precision="fp64"),
-        WindowRequest(3, "stadium", 15.0, 0.3, tol=1e-8, nq=24, ppw=12, seed=14),
-        WindowRequest(4, "circle_hole", 16.0, 0.35, tol=1e-12, nq=24, ppw=12, seed=15),
+        WindowRequest(3, "stadium", 15.0, 0.3, tol=1e-8, nq=24, ppw=12, seed=14,
+                      precision="mixed"),
+        WindowRequest(4, "circle_hole", 16.0, 0.35, tol=1e-12, nq=24, ppw=12, seed=15,
+                      precision="mixed"),
         WindowRequest(5, "circle", 26.0, 0.25, tol=1e-12, nq=24, ppw=12, seed=16),
     ]
 

@@ -75,3 +75,23 @@ def test_rank_threshold_follows_the_solve_tolerance():
     assert _saturated(Rb[None], 1e-10, 1e-12)
     assert not _saturated(Rb[None], 1e-10, 1e-8)
     assert not _saturated((Rb * 1e-4)[None], 1e-10, 1e-12)
+
+
+def test_b_from_qr_relative_threshold_counts_the_rank_threshold():
+    """B_from_qr with rel_tol = RANK_NOISE tol keeps exactly the singular
+    values at or above rank_threshold: one SVD instead of two in the serving
+    loop's host stage."""
+    from qbhip.beyn import B_from_qr
+    from qbhip.serve import RANK_NOISE, rank_threshold
+    rng = np.random.default_rng(4)
+    Q1, _ = np.linalg.qr(rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6)))
+    S = np.array([2.0, 0.5, 1e-3, 3e-6, 1e-7, 1e-12])
+    Rb = np.triu(np.linalg.qr((Q1 * S) @ Q2)[1])
+    Cm = rng.standard_normal((6, 6)) + 1j * rng.standard_normal((6, 6))
+    for tol in (1e-12, 1e-8):
+        thr = rank_threshold(Rb, 1e-10, tol)
+        B, Ur, Sv, rk = B_from_qr(Rb, Cm, 1e-10, RANK_NOISE * tol)
+        assert rk == int(np.count_nonzero(Sv >= thr)) and B.shape == (rk, rk)
+    assert B_from_qr(Rb, Cm, 1e-10, RANK_NOISE * 1e-12)[3] == 5
+    assert B_from_qr(Rb, Cm, 1e-10, RANK_NOISE * 1e-8)[3] == 4
