@@ -42,7 +42,7 @@ TRAFFIC_PER_LAUNCH = {"c128": 1634.6e6, "c64": 486.7e6}
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=8192)
     ap.add_argument("--nq", type=int, default=48)
