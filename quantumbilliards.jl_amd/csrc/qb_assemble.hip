@@ -546,16 +546,20 @@ __global__ __launch_bounds__(64) void k_env_tables(const cd* __restrict__ zk, in
   }
 }
 
-template <int OP>
-__global__ void __launch_bounds__(256)
+// MW: columns of the mirror tile staged through LDS at a time (16, or 4 for
+// the variant that fits three workgroups per CU: 2 x 64 x 5 instead of
+// 2 x 64 x 17 entries); OCC: waves per SIMD the registers must allow.
+template <int OP, int MW, int OCC>
+__global__ void __launch_bounds__(256, OCC)
 k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env, int o0, int P,
                const int* __restrict__ pairs, int T, cd* __restrict__ out64,
                cf* __restrict__ out32) {
+  static_assert(MW == 4 || MW == 8 || MW == 16, "mirror width");
   extern __shared__ __align__(16) cd smem[];
   const int tsz = P * 2 * QB_ENV_NC;  // complex coefficients per node
   cd* Ta = smem;
   cd* Tb = smem + tsz;
-  cd (*mir)[64][17] = (cd (*)[64][17])(smem + 2 * tsz);  // [node][i_local][j_local - 16 h]
+  cd (*mir)[64][MW + 1] = (cd (*)[64][MW + 1])(smem + 2 * tsz);  // [node][i_local][j_local - MW h]
   const int64_t N = g.N;
   const int qa = pairs[2 * blockIdx.z], qb = pairs[2 * blockIdx.z + 1];
   const bool two = qa != qb;
@@ -580,10 +584,11 @@ k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env,
   const double xi = g.x[ic], yi = g.y[ic];
   const int ci = g.comp[ic];
   __syncthreads();
+  constexpr int CPS = MW / 4;  // column groups (4 columns each) per mirror stage
 #pragma unroll 1
-  for (int h = 0; h < 4; ++h) {
+  for (int h = 0; h < 64 / MW; ++h) {
 #pragma unroll 1
-    for (int c = 4 * h; c < 4 * h + 4; ++c) {
+    for (int c = CPS * h; c < CPS * h + CPS; ++c) {
       const int jl = jl0 + 4 * c;
       const int64_t j = (int64_t)Jt * 64 + jl;
       if (!iok || j >= N) continue;
@@ -662,19 +667,19 @@ k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env,
         if (two) out32[bb + e1] = to_cf(vb);
       }
       if (!diag_tile) {
-        mir[0][il][jl - 16 * h] = wa;
-        mir[1][il][jl - 16 * h] = wb;
+        mir[0][il][jl - MW * h] = wa;
+        mir[1][il][jl - MW * h] = wb;
       }
     }
     if (diag_tile) continue;  // uniform over the block
     __syncthreads();
-    // mirror rows j = Jt*64 + 16h + 0..15 of the 64 columns i: each group of
-    // 16 lanes writes one column's 16 contiguous rows
+    // mirror rows j = Jt*64 + MW h + 0..MW-1 of the 64 columns i: each group
+    // of MW lanes writes one column's MW contiguous rows
 #pragma unroll 1
-    for (int c = 0; c < 4; ++c) {
-      const int il2 = (threadIdx.x >> 4) + 16 * c;
-      const int jl2 = threadIdx.x & 15;
-      const int64_t ii = (int64_t)It * 64 + il2, jj = (int64_t)Jt * 64 + 16 * h + jl2;
+    for (int c = 0; c < MW / 4; ++c) {
+      const int il2 = (int)(threadIdx.x / MW) + (256 / MW) * c;
+      const int jl2 = (int)(threadIdx.x % MW);
+      const int64_t ii = (int64_t)It * 64 + il2, jj = (int64_t)Jt * 64 + MW * h + jl2;
       if (ii >= N || jj >= N) continue;
       const int64_t e2 = jj + ii * N;
       const cd xa = mir[0][il2][jl2];
@@ -689,6 +694,10 @@ k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env,
     __syncthreads();
   }
 }
+
+// Default variant of the envelope assembly per operator, chosen by measurement
+// (three workgroups per CU for DLP, whose registers fit without spilling).
+static bool env_occ3_default(int op) { return op == QB_DLP_KRESS; }
 
 int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
                         const cd* z_dev, int64_t nz, double rmin, double rmax, cd* out64,
@@ -729,21 +738,34 @@ int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
     const int64_t N = g->d.N;
     const int T = (int)((N + 63) / 64);
     const int64_t tpairs = (int64_t)T * (T + 1) / 2;
-    const size_t lds = (size_t)2 * P * 2 * QB_ENV_NC * sizeof(cd) + 2 * 64 * 17 * sizeof(cd);
+    // the three-workgroups-per-CU variant (4-column mirror stages) where its
+    // LDS fits three times in a CU (QB_ASM_ENV_OCC=2 or 3 forces a variant)
+    const size_t tab = (size_t)2 * P * 2 * QB_ENV_NC * sizeof(cd);
+    const size_t lds16 = tab + 2 * 64 * 17 * sizeof(cd), lds4 = tab + 2 * 64 * 5 * sizeof(cd);
+    bool occ3 = lds4 <= (size_t)(160 * 1024) / 3 && env_occ3_default(op);
+    if (const char* e = getenv("QB_ASM_ENV_OCC"))
+      occ3 = e[0] == '3' && lds4 <= (size_t)(160 * 1024) / 3;
+    const size_t lds = occ3 ? lds4 : lds16;
     const double ent = (double)N * N * nz;
     const dim3 grid((unsigned)tpairs, 1, (unsigned)npair);
     slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
                          ent * ((out64 ? sizeof(cd) : 0) + (out32 ? sizeof(cf) : 0)));
+    auto launch = [&](auto kern) {
+      (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds);
+      hipLaunchKernelGGL(kern, grid, dim3(256), lds, ctx->stream, g->d, z_dev, env, o0, P, pd, T,
+                         out64, out32);
+    };
     if (op == QB_CFIE_KRESS) {
-      (void)hipFuncSetAttribute((const void*)k_assemble_env<QB_CFIE_KRESS>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((k_assemble_env<QB_CFIE_KRESS>), grid, dim3(256), lds, ctx->stream,
-                         g->d, z_dev, env, o0, P, pd, T, out64, out32);
+      if (occ3)
+        launch(k_assemble_env<QB_CFIE_KRESS, 4, 3>);
+      else
+        launch(k_assemble_env<QB_CFIE_KRESS, 16, 1>);
     } else {
-      (void)hipFuncSetAttribute((const void*)k_assemble_env<QB_DLP_KRESS>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((k_assemble_env<QB_DLP_KRESS>), grid, dim3(256), lds, ctx->stream,
-                         g->d, z_dev, env, o0, P, pd, T, out64, out32);
+      if (occ3)
+        launch(k_assemble_env<QB_DLP_KRESS, 4, 3>);
+      else
+        launch(k_assemble_env<QB_DLP_KRESS, 16, 1>);
     }
     qb_prof_end(ctx, slot);
     e = hipGetLastError();
