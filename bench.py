@@ -97,7 +97,8 @@ def parse():
     ap.add_argument("--group", action="store_true",
                     help="batch windows that share N into one window (qb_window_create_multi)")
     ap.add_argument("--graphs", action="store_true",
-                    help="replay each window's LU from a captured hipGraph (qb_ctx_set_graphs)")
+                    help="replay each window's LU from a captured hipGraph (qb_ctx_set_graphs; "
+                         "only without the per-launch events, so with --no-kernel-stats)")
     ap.add_argument("--no-kernel-stats", action="store_true",
                     help="time without the library's per-launch HIP events (no roofline fields)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",

@@ -485,6 +485,35 @@ def test_conj_pair_assembly_matches_oracle(kind, Ns, k0, op, M, nq):
     assert et[:, 0].max() < 1e-12 and et[:, 1].max() < 1e-11
 
 
+@pytest.mark.parametrize("op", ["cfie", "dlp"])
+def test_envelope_assembly_variants_are_bitwise_equal(op):
+    """The envelope-table kernel's two variants (mirror tile staged 16 columns
+    at a time at two workgroups per CU, or 4 columns at three) compute the
+    same entries with the same arithmetic: QB_ASM_ENV_OCC=2 and =3 give
+    bitwise the same matrices, on the star (CFIE) and the ellipse (DLP), with
+    a partial last tile (N not a multiple of 64)."""
+    import os
+    q = qb()
+    if op == "dlp":
+        b = q.make_full_ellipse(1.0, 0.6)
+        solver = q.DLP_kress(10.0, b)
+        pts = q.evaluate_points(solver, b, 1.0, N=[330])
+    else:
+        b, pts = _pts("star", [330])
+        solver = q.CFIE_kress(10.0, b)
+    zj, _ = O.contour(40.0, 0.5, 6)
+    cp = dict(n_panels_h=3000, M_h=7, n_panels_j=1000, M_j=7)
+    out = {}
+    try:
+        for occ in ("2", "3"):
+            os.environ["QB_ASM_ENV_OCC"] = occ
+            out[occ] = q.assemble(solver, pts, zj, conj_pairs=True, **cp)
+    finally:
+        del os.environ["QB_ASM_ENV_OCC"]
+    assert np.array_equal(out["2"], out["3"])
+    assert np.array_equal(q.assemble(solver, pts, zj, conj_pairs=True, **cp), out["2"])
+
+
 def test_conj_pair_assembly_rejects_unpaired_nodes():
     """QB_ASM_CONJ_PAIRS with a node whose conjugate is missing is an argument
     error (the library does not fall back silently)."""
