@@ -5,6 +5,13 @@
 template <typename T>
 int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t sA, const T* B,
              int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch);
+// gemm_sub with the given variant of the operand type's tile table (for a
+// shape outside the tuned classes); an unknown variant takes the default tile.
+template <typename T>
+int gemm_sub_v(qb_ctx* ctx, int v, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
+               const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch);
+// complex128 variant 13: the 32 x 32 x 8 Gauss-3M tile with 19 KB of LDS
+#define QB_GEMM_LEAN_C128 13
 template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
                      int* info, int batch);

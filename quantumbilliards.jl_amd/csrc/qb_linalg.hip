@@ -555,7 +555,7 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     // tuner skips them (kGemmAvail)
     launch_gemm<cd, 32, 32, 8, 2, 2, false, true>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 2, 3>,
     launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 4>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 3>};
-#define QB_GEMM_LEAN_C128 13
+static_assert(QB_GEMM_LEAN_C128 == 13, "the lean complex128 tile is variant 13");
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
@@ -625,10 +625,16 @@ int gemm_sub(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda, int64_t 
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
   const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
   int v = (ctx->gemm_conc ? ctx->gemm_pick_conc : ctx->gemm_pick)[ty][qb_gemm_class(M, N, K)];
-  if (ty == 1 && ctx->panel_lean)
-    v = QB_GEMM_LEAN_C128;
-  else if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v])
-    v = gemm_default<T>(N);
+  if (ty == 1 && ctx->panel_lean) v = QB_GEMM_LEAN_C128;
+  return gemm_sub_v<T>(ctx, v, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
+}
+
+template <typename T>
+int gemm_sub_v(qb_ctx* ctx, int v, int M, int N, int K, const T* A, int64_t lda, int64_t sA,
+               const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
+  if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
+  const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
+  if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v]) v = gemm_default<T>(N);
   const int kind = ty == 0 ? (kGemmBf6[ty][v] ? QB_K_GEMM_C64_BF16 : QB_K_GEMM_C64) : QB_K_GEMM_C128;
   int slot = qb_prof_begin(ctx, kind, 8.0 * M * N * K * (double)batch,
                            (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
@@ -1570,4 +1576,6 @@ template int qb_gemm_tune<cd>(qb_ctx*, int, int, int, int, double*);
 template int gemm_sub<cf>(qb_ctx*, int, int, int, const cf*, int64_t, int64_t, const cf*, int64_t,
                           int64_t, cf*, int64_t, int64_t, int);
 template int gemm_sub<cd>(qb_ctx*, int, int, int, const cd*, int64_t, int64_t, const cd*, int64_t,
+                          int64_t, cd*, int64_t, int64_t, int);
+template int gemm_sub_v<cd>(qb_ctx*, int, int, int, int, const cd*, int64_t, int64_t, const cd*, int64_t,
                           int64_t, cd*, int64_t, int64_t, int);

@@ -411,6 +411,33 @@ def test_moments_qr_on_device(precision):
     assert np.max(np.abs(P1 - P2)) < 1e-8
 
 
+@pytest.mark.parametrize("r", [16, 40, 96])
+def test_moments_qr_blocked_matches_lapack(r):
+    """The device QR of the moment block at widths that take the unblocked
+    sweep (r <= 32) and the compact-WY sweep (panels of 32 reflectors, the
+    last one partial): R equals LAPACK's zgeqrf R of the same A0 (same
+    Householder sign convention, beta = -sign(Re alpha) |x|) to rounding,
+    Q is orthonormal, Q R = A0 and C = Q^H A1."""
+    q = qb()
+    b, pts = _pts("star", [320])
+    solver = q.CFIE_kress(10.0, b)
+    cp = dict(n_panels_h=3000, M_h=5, n_panels_j=1000, M_j=5)
+    V = q.probe_block(np.random.default_rng(7), 320, r)
+    win = q.Window(solver, pts, 30.0 + 0j, 0.4, 8, precision="fp64", **cp)
+    A0, A1 = win.moments(V, 1e-13)
+    Rm, Cm = win.moments_qr(V, 1e-13)
+    Q = win.basis(np.eye(r, dtype=complex))
+    win.close()
+    s0 = np.max(np.abs(A0))
+    R_np = np.linalg.qr(A0, mode="r")
+    assert np.max(np.abs(np.tril(Rm, -1))) == 0.0
+    assert np.all(np.abs(np.diag(Rm).imag) == 0.0)
+    assert np.max(np.abs(Rm - R_np)) < 1e-12 * s0
+    assert np.max(np.abs(Q.conj().T @ Q - np.eye(r))) < 1e-13
+    assert np.max(np.abs(Q @ Rm - A0)) < 1e-12 * s0
+    assert np.max(np.abs(Q.conj().T @ A1 - Cm)) < 1e-12 * np.max(np.abs(A1))
+
+
 @pytest.mark.parametrize("kind,Ns,k0,op,M,nq", [
     ("star", [512], 60.0, "cfie", 5, 8),
     ("star", [512], 60.0, "cfie", 7, 8),
