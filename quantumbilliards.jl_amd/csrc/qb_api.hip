@@ -1581,6 +1581,22 @@ extern "C" int qb_window_moments(qb_window* w, int32_t r, const double* V, doubl
 // A0 and A1 stay on the device in w->qr ([A0 | A1], N x 2r); A0 = Q R by
 // Householder reflections, Q^H applied to A1; R (r x r upper triangular) and
 // C = (Q^H A1)[:r, :] are returned, the reflectors kept for qb_window_basis.
+// R (upper triangle of each block's A0 half, zeros below) and C (the first r
+// rows of its A1 half) of every moment block, packed as [R_0 .. R_b | C_0 ..
+// C_b], r x r each (ld r), so they leave the device in two contiguous copies
+// instead of one row transfer per column.  Grid (ceil(r*r/256), blocks, 2).
+__global__ void k_pack_rc(const cd* __restrict__ qr, int64_t N, int r, cd* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t rr = (int64_t)r * r;
+  if (e >= rr) return;
+  const int ie = blockIdx.y, half = blockIdx.z;
+  const int i = (int)(e % r), c = (int)(e / r);
+  const cd* src = qr + ((int64_t)2 * ie + half) * N * r;
+  cd v = src[i + (int64_t)c * N];
+  if (half == 0 && i > c) v = mk(0.0, 0.0);
+  out[((int64_t)half * gridDim.y + ie) * rr + e] = v;
+}
+
 extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, double* R,
                                     double* Cm, double refine_tol, qb_stats* stats) {
   if (!w) return QB_E_ARG;
@@ -1614,26 +1630,22 @@ extern "C" int qb_window_moments_qr(qb_window* w, int32_t r, const double* V, do
   // one QR per block on its [A0_b | A1_b], all blocks in the same launches
   rc = qb_house_qr(ctx, w->qr, N, r, 2 * r, w->qr_tau, nirr, (int64_t)2 * N * r, (int64_t)r);
   if (rc) return rc;
-  for (int ie = 0; ie < nirr; ++ie) {
-    const cd* blk = w->qr + (size_t)2 * ie * N * r;
-    const cd* Cd = blk + (size_t)N * r;
-    double* Re = R + (size_t)2 * ie * r * r;
-    double* Ce = Cm + (size_t)2 * ie * r * r;
-    hipError_t e = hipMemcpy2DAsync(Re, r * sizeof(cd), blk, N * sizeof(cd), r * sizeof(cd), r,
-                                    hipMemcpyDeviceToHost, ctx->stream);
+  {
+    const size_t pbytes = (size_t)2 * nirr * r * r * sizeof(cd);
+    cd* pk = nullptr;
+    rc = qb_dev_alloc(ctx, (void**)&pk, pbytes);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_pack_rc, dim3((unsigned)(((int64_t)r * r + 255) / 256), nirr, 2),
+                       dim3(256), 0, ctx->stream, w->qr, N, r, pk);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess)
-      e = hipMemcpy2DAsync(Ce, r * sizeof(cd), Cd, N * sizeof(cd), r * sizeof(cd), r,
-                           hipMemcpyDeviceToHost, ctx->stream);
+      e = hipMemcpyAsync(R, pk, pbytes / 2, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(Cm, pk + (size_t)nirr * r * r, pbytes / 2, hipMemcpyDeviceToHost,
+                         ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    qb_dev_free(ctx, pk, pbytes);
     if (e != hipSuccess) return qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
-  }
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess)
-    return qb_fail(ctx, QB_E_HIP, "moment QR failed");
-  // R: zero the reflectors stored below the diagonal
-  for (int ie = 0; ie < nirr; ++ie) {
-    double* Re = R + (size_t)2 * ie * r * r;
-    for (int c = 0; c < r; ++c)
-      for (int i = c + 1; i < r; ++i)
-        Re[2 * ((size_t)c * r + i)] = Re[2 * ((size_t)c * r + i) + 1] = 0;
   }
   w->qr_r = r;
   if (stats) *stats = w->st;
@@ -1661,13 +1673,21 @@ extern "C" int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* 
   const int64_t N = w->N;
   const int r = w->qr_r;
   const size_t bytes = (size_t)N * k * sizeof(cd);
-  cd* Y = nullptr;
+  const size_t ubytes = (size_t)r * k * sizeof(cd);
+  cd *Y = nullptr, *Ud = nullptr;
   int rc = qb_dev_alloc(ctx, (void**)&Y, bytes);
-  if (rc) return rc;
+  if (!rc) rc = qb_dev_alloc(ctx, (void**)&Ud, ubytes);
+  if (rc) {
+    qb_dev_free(ctx, Y, bytes);
+    return rc;
+  }
+  // Ur goes up in one contiguous copy and is spread into the first r rows of
+  // Y on the device (a host-to-device 2D copy moves one row at a time)
   hipError_t e = hipMemsetAsync(Y, 0, bytes, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(Ud, Ur, ubytes, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess)
-    e = hipMemcpy2DAsync(Y, N * sizeof(cd), Ur, r * sizeof(cd), r * sizeof(cd), k,
-                         hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpy2DAsync(Y, N * sizeof(cd), Ud, r * sizeof(cd), r * sizeof(cd), k,
+                         hipMemcpyDeviceToDevice, ctx->stream);
   if (e != hipSuccess) rc = qb_fail(ctx, QB_E_HIP, hipGetErrorString(e));
   if (!rc)
     rc = qb_house_apply_q(ctx, w->qr + (size_t)2 * irrep * N * r, N, r,
@@ -1676,6 +1696,7 @@ extern "C" int qb_window_basis_irrep(qb_window* w, int32_t irrep, const double* 
               hipStreamSynchronize(ctx->stream) != hipSuccess))
     rc = qb_fail(ctx, QB_E_HIP, "basis download failed");
   (void)hipStreamSynchronize(ctx->stream);
+  qb_dev_free(ctx, Ud, ubytes);
   qb_dev_free(ctx, Y, bytes);
   return rc;
 }
