@@ -301,6 +301,168 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_sub(int M, int N, int K, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// complex128 C[b] -= A[b] * B[b] with the operands streamed from global
+// memory straight into the MFMA operand registers: no LDS and no barrier.
+// Each wave owns a WM x WN tile of C (WGM x WGN waves per workgroup share
+// their A rows and B columns through L1) and keeps the fragments of the next
+// D k-steps (4-deep each) in flight in a register ring.  Lane l holds
+// A[m0 + 16i + l % 16][k + l / 16] and B[k + l / 16][n0 + 16j + l % 16]:
+// 16 lanes read 256 contiguous bytes of an A column, 4 lanes 64 contiguous
+// bytes of a B column.  Gauss's three products as in k_gemm_sub (P1 = Ar Br,
+// P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi)), accumulated as +AB and subtracted in
+// the epilogue, the same k order and MFMA as k_gemm_sub, so the two give the
+// same bits.
+// ---------------------------------------------------------------------------
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+typedef double qb_d2 __attribute__((ext_vector_type(2)));
+
+template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD>
+__global__ void __launch_bounds__(64 * WGM * WGN, OCC)
+    k_gemm_reg(int M, int N, int K, const cd* __restrict__ A, int64_t lda, int64_t sA,
+               const cd* __restrict__ B, int64_t ldb, int64_t sB, cd* __restrict__ C,
+               int64_t ldc, int64_t sC, const int* __restrict__ bix) {
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int BM = WM * WGM, BN = WN * WGN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  int mt = blockIdx.x, ntile = blockIdx.y;
+  if constexpr (XCD) {
+    // the XCD-aware tile order of k_gemm_sub
+    const unsigned gx = gridDim.x, gy = gridDim.y, ntiles = gx * gy;
+    if ((ntiles & 7u) == 0) {
+      const unsigned L = blockIdx.x + gx * blockIdx.y;
+      const unsigned t = (L & 7u) * (ntiles >> 3) + (L >> 3);
+      const unsigned grp = t / (8u * gy), first = grp * 8u;
+      const unsigned gm = gx - first < 8u ? gx - first : 8u;
+      const unsigned loc = t - first * gy;
+      mt = (int)(first + loc % gm);
+      ntile = (int)(loc / gm);
+    }
+  }
+  const int m0 = mt * BM + (wave % WGM) * WM, n0 = ntile * BN + (wave / WGM) * WN;
+  if (m0 >= M || n0 >= N) return;  // the wave has no rows or columns (no barrier follows)
+  const int64_t b = bix ? bix[blockIdx.z] : blockIdx.z;
+  A += b * sA;
+  B += b * sB;
+  C += b * sC;
+  // rows, columns and k outside the operands are read at a clamped address
+  // (so every load is in bounds and unconditional) and zeroed when used
+  const cd* Ap[FM];
+  const cd* Bp[FN];
+  bool aok[FM], bok[FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+    const int row = m0 + 16 * i + li;
+    aok[i] = row < M;
+    Ap[i] = A + (aok[i] ? row : M - 1);
+  }
+#pragma unroll
+  for (int j = 0; j < FN; ++j) {
+    const int col = n0 + 16 * j + li;
+    bok[j] = col < N;
+    Bp[j] = B + (int64_t)(bok[j] ? col : N - 1) * ldb;
+  }
+  const int ns = (K + 3) >> 2;  // k-steps of 4
+  qb_d2 ra[D][FM], rb[D][FN];   // (re, im) of the lane's A and B elements, as loaded
+  auto load = [&](auto slot, int s) {
+    constexpr int d = decltype(slot)::value;
+    const int k = 4 * s + lk < K ? 4 * s + lk : K - 1;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) ra[d][i] = *(const qb_d2*)(Ap[i] + (int64_t)k * lda);
+#pragma unroll
+    for (int j = 0; j < FN; ++j) rb[d][j] = *(const qb_d2*)(Bp[j] + k);
+  };
+  doublex4 p1[FM][FN], p2[FM][FN], p3[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      p1[i][j] = doublex4{0, 0, 0, 0};
+      p2[i][j] = doublex4{0, 0, 0, 0};
+      p3[i][j] = doublex4{0, 0, 0, 0};
+    }
+  // k-step s: multiply ring slot s % D, masking rows, columns and k outside
+  // the operands
+  auto mult = [&](auto slot, int s) {
+    constexpr int d = decltype(slot)::value;
+    const bool kok = 4 * s + lk < K;
+    double xr[FM], xi[FM], yr[FN], yi[FN], as[FM], bs[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+      const bool ok = aok[i] && kok;
+      xr[i] = ok ? ra[d][i].x : 0.0;
+      xi[i] = ok ? ra[d][i].y : 0.0;
+      as[i] = xr[i] + xi[i];
+    }
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const bool ok = bok[j] && kok;
+      yr[j] = ok ? rb[d][j].x : 0.0;
+      yi[j] = ok ? rb[d][j].y : 0.0;
+      bs[j] = yr[j] + yi[j];
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        p1[i][j] = mfma16(xr[i], yr[j], p1[i][j]);
+        p2[i][j] = mfma16(xi[i], yi[j], p2[i][j]);
+        p3[i][j] = mfma16(as[i], bs[j], p3[i][j]);
+      }
+  };
+  // the ring is refilled without conditions (clamped addresses are always
+  // valid), so the loop body is straight-line code and the compiler waits
+  // only for the slot it is about to multiply
+  static_for<0, D>([&](auto slot) { load(slot, decltype(slot)::value); });
+  int s0 = 0;
+  for (; s0 + D <= ns; s0 += D) {
+    static_for<0, D>([&](auto slot) {
+      mult(slot, s0 + decltype(slot)::value);
+      load(slot, s0 + decltype(slot)::value + D);
+    });
+  }
+  static_for<0, D>([&](auto slot) {
+    if (s0 + decltype(slot)::value < ns) mult(slot, s0 + decltype(slot)::value);
+  });
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + 16 * j + li;
+      if (gn >= N) continue;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + 16 * i + 4 * v + lk;
+        if (gm < M) {
+          cd* p = C + gm + (int64_t)gn * ldc;
+          cd c = *p;
+          const double q1 = p1[i][j][v], q2 = p2[i][j][v];
+          c.re -= q1 - q2;
+          c.im -= p3[i][j][v] - q1 - q2;
+          *p = c;
+        }
+      }
+    }
+}
+
+template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD>
+static void launch_gemm_reg(qb_ctx* ctx, int M, int N, int K, const cd* A, int64_t lda,
+                            int64_t sA, const cd* B, int64_t ldb, int64_t sB, cd* C,
+                            int64_t ldc, int64_t sC, int batch) {
+  constexpr int BM = WM * WGM, BN = WN * WGN;
+  dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+  hipLaunchKernelGGL((k_gemm_reg<WM, WN, WGM, WGN, D, OCC, XCD>), g, dim3(64 * WGM * WGN), 0,
+                     ctx->stream, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
+}
+
 // OCC: waves per SIMD the register allocation must allow (__launch_bounds__).
 template <typename T, int BM, int BN, int BK, int WGM, int WGN, bool XCD = false,
           bool G3 = false, int PF = 1, int OCC = (PF == 2 ? 2 : 1)>
@@ -540,7 +702,10 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 64, 64, 16, 2, 2, false, true>, launch_gemm<cf, 64, 64, 16, 2, 2, true, true>,
     launch_gemm<cf, 64, 64, 16, 2, 2>,         launch_gemm_bf6<128, 128, 2, 2>,
     launch_gemm_bf6<128, 128, 2, 2, true>,     launch_gemm_bf6<128, 64, 2, 2>,
-    launch_gemm_bf6<64, 64, 2, 2>};
+    launch_gemm_bf6<64, 64, 2, 2>,
+    // 17-19: complex128 only (the slots repeat the default; kGemmAvail)
+    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>,
+    launch_gemm<cf, 128, 128, 16, 2, 2>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
@@ -554,35 +719,42 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     // counterpart for complex128: the other slots repeat the default and the
     // tuner skips them (kGemmAvail)
     launch_gemm<cd, 32, 32, 8, 2, 2, false, true>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 2, 3>,
-    launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 4>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 3>};
+    launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 4>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 3>,
+    // 17-19: operands streamed into registers, no LDS (k_gemm_reg).  Wave
+    // tiles of 32 x 64 and 64 x 32 at one wave per SIMD were slower: the
+    // compiler shuttles their accumulators between AGPRs and VGPRs
+    launch_gemm_reg<32, 32, 2, 2, 8, 2, true>, launch_gemm_reg<32, 32, 4, 1, 6, 2, true>,
+    launch_gemm_reg<32, 32, 2, 2, 6, 2, true>};
 static_assert(QB_GEMM_LEAN_C128 == 13, "the lean complex128 tile is variant 13");
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
-     "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6"},
+     "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6", "-", "-",
+     "-"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
      "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "64x64x8/3m/xcd/pf2/occ3", "64x64x8/3m/xcd/occ4",
-     "64x64x8/3m/xcd/occ3"}};
+     "64x64x8/3m/xcd/occ3", "64x64x4/3m/xcd/reg/d8", "128x32x4/3m/xcd/reg/d6",
+     "64x64x4/3m/xcd/reg/d6"}};
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true},
+     true, false, false, false},
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true, true}};
+     true, true, true, true, true}};
 // Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
 static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
-     true, true, true, true},
+     true, true, true, true, false, false, false},
     {false}};
 
 // Variants built on Gauss's three-product form (3 real MFMA products per
 // complex block product instead of 4).
 static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
-     false, false, false},
+     false, false, false, false, false, false},
     {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
-     true, true, true}};
+     true, true, true, true, true, true}};
 
 // Variants that pack 3-4 waves per SIMD (small tiles or k-tiles, low
 // register and LDS use): fastest alone, but they leave a concurrently
@@ -590,9 +762,9 @@ static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
 // excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, true, true, true, false,
-     false, false, false},
+     false, false, false, false, false, false},
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
-     false, true, true, true}};
+     false, true, true, true, false, false, false}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);

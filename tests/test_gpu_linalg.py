@@ -93,7 +93,7 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(17))
+@pytest.mark.parametrize("variant", range(20))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
     LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
@@ -108,7 +108,11 @@ def test_every_tile_variant(probe, variant):
     lean 32 x 32 tile with 8-deep k-tiles of the LU panel that runs beside
     the trailing update, and complex128 14-16 are 64 x 64 Gauss tiles with
     8-deep k-tiles at 3-4 waves per SIMD (prefetch 2 for 14); K = 33 and 40
-    end in partial k-tiles."""
+    end in partial k-tiles; complex128 17-19 stream the operands from global
+    memory straight into the MFMA registers (no LDS, no barrier) with 8 or 6
+    k-steps of 4 in flight (64 x 64 and 128 x 32 workgroup tiles of 32 x 32
+    wave tiles), so K = 33 ends in a partial step and K = 40 in a partial
+    ring."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
@@ -123,6 +127,32 @@ def test_every_tile_variant(probe, variant):
     finally:
         for ty in (0, 1):
             L.qbt_force_variant(c, ty, -1)
+
+
+@pytest.mark.parametrize("reg_variant", [17, 18, 19])
+def test_register_tiles_give_the_bits_of_the_lds_tile(probe, reg_variant):
+    """The register-streamed complex128 tiles (17-19) multiply the same
+    operands in the same k order with the same MFMA and Gauss form as the
+    64 x 64 LDS tile (16), so C - A B agrees bit for bit (edge tiles in M, N
+    and a partial k-step included)."""
+    L, ctx = probe
+    c = C.c_void_p(ctx.value)
+    rng = np.random.default_rng(11)
+    M, N, K, b = 300, 129, 131, 2
+    A = _rand(rng, (b, K, M), np.complex128)
+    B = _rand(rng, (b, N, K), np.complex128)
+    C0 = _rand(rng, (b, N, M), np.complex128)
+    out = []
+    try:
+        for v in (16, reg_variant):
+            L.qbt_force_variant(c, 1, v)
+            Cm = C0.copy()
+            assert L.qbt_gemm_sub_c128(c, M, N, K, C.c_void_p(_c(A)), C.c_void_p(_c(B)),
+                                       C.c_void_p(_c(Cm)), b) == 0
+            out.append(Cm)
+    finally:
+        L.qbt_force_variant(c, 1, -1)
+    assert np.array_equal(out[0], out[1])
 
 
 def test_autotune_report_and_results():
