@@ -391,20 +391,23 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
     }
   // k-step s: multiply ring slot s % D, masking rows, columns and k outside
   // the operands
-  auto mult = [&](auto slot, int s) {
+  // (MASK: rows, columns or k of the wave's tile fall outside the operands;
+  // interior tiles with K a multiple of 4 multiply the loads as they are)
+  auto mult = [&](auto slot, int s, auto mask) {
     constexpr int d = decltype(slot)::value;
+    constexpr bool MASK = decltype(mask)::value;
     const bool kok = 4 * s + lk < K;
     double xr[FM], xi[FM], yr[FN], yi[FN], as[FM], bs[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-      const bool ok = aok[i] && kok;
+      const bool ok = !MASK || (aok[i] && kok);
       xr[i] = ok ? ra[d][i].x : 0.0;
       xi[i] = ok ? ra[d][i].y : 0.0;
       as[i] = xr[i] + xi[i];
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const bool ok = bok[j] && kok;
+      const bool ok = !MASK || (bok[j] && kok);
       yr[j] = ok ? rb[d][j].x : 0.0;
       yi[j] = ok ? rb[d][j].y : 0.0;
       bs[j] = yr[j] + yi[j];
@@ -423,14 +426,21 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
   // only for the slot it is about to multiply
   static_for<0, D>([&](auto slot) { load(slot, decltype(slot)::value); });
   int s0 = 0;
-  for (; s0 + D <= ns; s0 += D) {
-    static_for<0, D>([&](auto slot) {
-      mult(slot, s0 + decltype(slot)::value);
-      load(slot, s0 + decltype(slot)::value + D);
-    });
-  }
+  auto rounds = [&](auto mask) {
+    for (; s0 + D <= ns; s0 += D) {
+      static_for<0, D>([&](auto slot) {
+        mult(slot, s0 + decltype(slot)::value, mask);
+        load(slot, s0 + decltype(slot)::value + D);
+      });
+    }
+  };
+  const bool interior = m0 + WM <= M && n0 + WN <= N && (K & 3) == 0;  // uniform per wave
+  if (interior)
+    rounds(std::false_type{});
+  else
+    rounds(std::true_type{});
   static_for<0, D>([&](auto slot) {
-    if (s0 + decltype(slot)::value < ns) mult(slot, s0 + decltype(slot)::value);
+    if (s0 + decltype(slot)::value < ns) mult(slot, s0 + decltype(slot)::value, std::true_type{});
   });
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -723,7 +733,7 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     // 17-19: operands streamed into registers, no LDS (k_gemm_reg).  Wave
     // tiles of 32 x 64 and 64 x 32 at one wave per SIMD were slower: the
     // compiler shuttles their accumulators between AGPRs and VGPRs
-    launch_gemm_reg<32, 32, 2, 2, 8, 2, true>, launch_gemm_reg<32, 32, 4, 1, 6, 2, true>,
+    launch_gemm_reg<32, 32, 4, 1, 7, 2, true>, launch_gemm_reg<32, 32, 4, 1, 8, 2, true>,
     launch_gemm_reg<32, 32, 2, 2, 6, 2, true>};
 static_assert(QB_GEMM_LEAN_C128 == 13, "the lean complex128 tile is variant 13");
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
@@ -734,7 +744,7 @@ static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
      "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "64x64x8/3m/xcd/pf2/occ3", "64x64x8/3m/xcd/occ4",
-     "64x64x8/3m/xcd/occ3", "64x64x4/3m/xcd/reg/d8", "128x32x4/3m/xcd/reg/d6",
+     "64x64x8/3m/xcd/occ3", "128x32x4/3m/xcd/reg/d7", "128x32x4/3m/xcd/reg/d8",
      "64x64x4/3m/xcd/reg/d6"}};
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {

@@ -109,10 +109,10 @@ def test_every_tile_variant(probe, variant):
     the trailing update, and complex128 14-16 are 64 x 64 Gauss tiles with
     8-deep k-tiles at 3-4 waves per SIMD (prefetch 2 for 14); K = 33 and 40
     end in partial k-tiles; complex128 17-19 stream the operands from global
-    memory straight into the MFMA registers (no LDS, no barrier) with 8 or 6
-    k-steps of 4 in flight (64 x 64 and 128 x 32 workgroup tiles of 32 x 32
+    memory straight into the MFMA registers (no LDS, no barrier) with 7, 8 or
+    6 k-steps of 4 in flight (128 x 32 and 64 x 64 workgroup tiles of 32 x 32
     wave tiles), so K = 33 ends in a partial step and K = 40 in a partial
-    ring."""
+    ring; 1300 x 1000 x 40 has interior waves, which skip the masks."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
