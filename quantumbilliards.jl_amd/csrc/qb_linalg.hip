@@ -784,11 +784,15 @@ int qb_gemm_class(int M, int N, int K) {
 }
 
 template <typename T>
-static int gemm_default(int N) {
+static int gemm_default(int M, int N, int K) {
+  // the tile a shape gets when the autotuner has not timed its class (the
+  // serving loop's windows of many sizes): for complex128 the autotuner's
+  // usual picks
+  (void)M;
   if constexpr (sizeof(T) == sizeof(cf))
     return N <= 64 ? 1 : 0;
   else
-    return N <= 32 ? 1 : 0;
+    return (N <= 32 || K <= 32) ? QB_GEMM_LEAN_C128 : K <= 64 ? 16 : 17;  // 128x32 register tile
 }
 
 template <typename T>
@@ -816,7 +820,7 @@ int gemm_sub_v(qb_ctx* ctx, int v, int M, int N, int K, const T* A, int64_t lda,
                const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch) {
   if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) return QB_OK;
   const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
-  if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v]) v = gemm_default<T>(N);
+  if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[ty][v]) v = gemm_default<T>(M, N, K);
   const int kind = ty == 0 ? (kGemmBf6[ty][v] ? QB_K_GEMM_C64_BF16 : QB_K_GEMM_C64) : QB_K_GEMM_C128;
   int slot = qb_prof_begin(ctx, kind, 8.0 * M * N * K * (double)batch,
                            (double)sizeof(T) * ((double)M * K + (double)K * N + 2.0 * M * N) * batch);
@@ -847,7 +851,7 @@ int qb_gemm_tune(qb_ctx* ctx, int M, int N, int K, int batch, double* best_tflop
   int rc = qb_dev_alloc(ctx, (void**)&A, (size_t)M * K * b * sizeof(T));
   if (!rc) rc = qb_dev_alloc(ctx, (void**)&B, (size_t)K * N * b * sizeof(T));
   if (!rc) rc = qb_dev_alloc(ctx, (void**)&C, (size_t)M * N * b * sizeof(T));
-  int best = gemm_default<T>(N), best_conc = best;
+  int best = gemm_default<T>(M, N, K), best_conc = best;
   double best_ms = 1e300, best_conc_ms = 1e300;
   if (!rc) {
     int64_t na = (int64_t)M * K * b * sizeof(T) / 8, nb = (int64_t)K * N * b * sizeof(T) / 8,
