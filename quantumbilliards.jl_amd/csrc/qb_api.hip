@@ -1825,6 +1825,7 @@ extern "C" int qb_ctx_autotune(qb_ctx* ctx, int64_t N, int32_t nq, int32_t r, in
   for (int d : {1, 2, 4, 8, 16})
     shapes.push_back({n / d - (d == 1 ? nb : 0), n / d - (d == 1 ? nb : 0), nb, lu_ty});
   for (int d : {2, 8}) shapes.push_back({128, n / d, 128, lu_ty});
+  shapes.push_back({64, n - nb, 64, lu_ty});  // the block-row TRSM's lowest updates
   for (int d : {2, 8, 32})
     for (int w = 128; w >= 32; w /= 2) shapes.push_back({n / d, w, w, lu_ty});
   // triangular solves with r right-hand sides (recursive halves)

@@ -713,9 +713,9 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 64, 64, 16, 2, 2>,         launch_gemm_bf6<128, 128, 2, 2>,
     launch_gemm_bf6<128, 128, 2, 2, true>,     launch_gemm_bf6<128, 64, 2, 2>,
     launch_gemm_bf6<64, 64, 2, 2>,
-    // 17-19: complex128 only (the slots repeat the default; kGemmAvail)
+    // 17-20: complex128 only (the slots repeat the default; kGemmAvail)
     launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>,
-    launch_gemm<cf, 128, 128, 16, 2, 2>};
+    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
@@ -730,41 +730,43 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     // tuner skips them (kGemmAvail)
     launch_gemm<cd, 32, 32, 8, 2, 2, false, true>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 2, 3>,
     launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 4>, launch_gemm<cd, 64, 64, 8, 2, 2, true, true, 1, 3>,
-    // 17-19: operands streamed into registers, no LDS (k_gemm_reg).  Wave
+    // 17-20: operands streamed into registers, no LDS (k_gemm_reg).  Wave
     // tiles of 32 x 64 and 64 x 32 at one wave per SIMD were slower: the
     // compiler shuttles their accumulators between AGPRs and VGPRs
     launch_gemm_reg<32, 32, 4, 1, 7, 2, true>, launch_gemm_reg<32, 32, 4, 1, 8, 2, true>,
-    launch_gemm_reg<32, 32, 2, 2, 6, 2, true>};
+    launch_gemm_reg<32, 32, 2, 2, 6, 2, true>,
+    // 20: 32 x 128 workgroup tile for updates of few rows (the block-row TRSM)
+    launch_gemm_reg<32, 32, 1, 4, 7, 2, true>};
 static_assert(QB_GEMM_LEAN_C128 == 13, "the lean complex128 tile is variant 13");
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
      "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6", "-", "-",
-     "-"},
+     "-", "-"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
      "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "64x64x8/3m/xcd/pf2/occ3", "64x64x8/3m/xcd/occ4",
      "64x64x8/3m/xcd/occ3", "128x32x4/3m/xcd/reg/d7", "128x32x4/3m/xcd/reg/d8",
-     "64x64x4/3m/xcd/reg/d6"}};
+     "64x64x4/3m/xcd/reg/d6", "32x128x4/3m/xcd/reg/d7"}};
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true, false, false, false},
+     true, false, false, false, false},
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true, true, true, true, true}};
+     true, true, true, true, true, true}};
 // Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
 static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
-     true, true, true, true, false, false, false},
+     true, true, true, true, false, false, false, false},
     {false}};
 
 // Variants built on Gauss's three-product form (3 real MFMA products per
 // complex block product instead of 4).
 static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
-     false, false, false, false, false, false},
+     false, false, false, false, false, false, false},
     {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
-     true, true, true, true, true, true}};
+     true, true, true, true, true, true, true}};
 
 // Variants that pack 3-4 waves per SIMD (small tiles or k-tiles, low
 // register and LDS use): fastest alone, but they leave a concurrently
@@ -772,9 +774,9 @@ static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
 // excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, true, true, true, false,
-     false, false, false, false, false, false},
+     false, false, false, false, false, false, false},
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
-     false, true, true, true, false, false, false}};
+     false, true, true, true, false, false, false, false}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);

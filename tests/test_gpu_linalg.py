@@ -93,7 +93,7 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(20))
+@pytest.mark.parametrize("variant", range(21))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
     LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
@@ -110,7 +110,7 @@ def test_every_tile_variant(probe, variant):
     8-deep k-tiles at 3-4 waves per SIMD (prefetch 2 for 14); K = 33 and 40
     end in partial k-tiles; complex128 17-19 stream the operands from global
     memory straight into the MFMA registers (no LDS, no barrier) with 7, 8 or
-    6 k-steps of 4 in flight (128 x 32 and 64 x 64 workgroup tiles of 32 x 32
+    6 k-steps of 4 in flight (128 x 32, 64 x 64 and 32 x 128 (20) workgroup tiles of 32 x 32
     wave tiles), so K = 33 ends in a partial step and K = 40 in a partial
     ring; 1300 x 1000 x 40 has interior waves, which skip the masks."""
     L, ctx = probe
@@ -129,9 +129,9 @@ def test_every_tile_variant(probe, variant):
             L.qbt_force_variant(c, ty, -1)
 
 
-@pytest.mark.parametrize("reg_variant", [17, 18, 19])
+@pytest.mark.parametrize("reg_variant", [17, 18, 19, 20])
 def test_register_tiles_give_the_bits_of_the_lds_tile(probe, reg_variant):
-    """The register-streamed complex128 tiles (17-19) multiply the same
+    """The register-streamed complex128 tiles (17-20) multiply the same
     operands in the same k order with the same MFMA and Gauss form as the
     64 x 64 LDS tile (16), so C - A B agrees bit for bit (edge tiles in M, N
     and a partial k-step included)."""
