@@ -92,6 +92,9 @@ def parse():
     ap.add_argument("--uneven-kmax", type=float, default=800.0)
     ap.add_argument("--uneven-precision", default="fp64",
                     help="fp64 (default: faster on this workload) or mixed")
+    ap.add_argument("--tune-n", type=int, default=0, metavar="N",
+                    help="serving and uneven runs: time the GEMM tiles of each slot for the shape "
+                         "classes of an N-point window at warm-up (0: untuned defaults)")
     ap.add_argument("--uneven-slots", type=int, default=2)
     ap.add_argument("--uneven-max-group", type=int, default=16)
     ap.add_argument("--group", action="store_true",
@@ -139,7 +142,7 @@ def run_serve(a):
     reqs = S.poisson_stream(a.serve_requests, rate=a.serve_rate, seed=5, kmax=a.serve_kmax,
                             symmetry=a.serve_symmetry, precision=a.serve_precision)
     srv = S.Server(slots=a.serve_slots, irreps_batched=a.serve_irreps_batched)
-    srv.warm_up()
+    srv.warm_up(tune_N=a.tune_n, precisions=(a.serve_precision,))
     srv.submit(S.WindowRequest(-1, "ellipse", 60.0, 0.3, tol=1e-8, nq=32,
                                symmetry=a.serve_symmetry)).result()
     t0 = time.perf_counter()
@@ -232,7 +235,7 @@ def run_uneven(a):
     reqs = S.uneven_windows(a.uneven_windows, seed=4, precision=a.uneven_precision,
                             kmax=a.uneven_kmax)
     srv = S.Server(slots=a.uneven_slots, group=a.group, max_group=a.uneven_max_group)
-    srv.warm_up()
+    srv.warm_up(tune_N=a.tune_n, precisions=(a.uneven_precision,))
     srv.submit(S.WindowRequest(-1, "circle_hole", 100.0, 0.5, nq=32,
                                precision=a.uneven_precision,
                                N=S.hole_node_counts(100.5))).result()

@@ -59,7 +59,8 @@ class Context:
         """Pick GEMM tile variants for the shapes of an (N, nq, r) window
         (qb_ctx_autotune); returns the report."""
         buf = C.create_string_buffer(8192)
-        prec = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED}[precision]
+        prec = {"fp64": _lib.QB_FP64, "mixed": _lib.QB_MIXED,
+                "mixed_int8res": _lib.QB_MIXED_INT8RES}[precision]
         _lib.check(self.ptr, _lib.lib().qb_ctx_autotune(self.ptr, int(N), int(nq), int(r), prec,
                                                         buf, len(buf)))
         return buf.value.decode()

@@ -268,10 +268,18 @@ class Server:
                 out.append(e)
         return out
 
-    def warm_up(self):
+    def warm_up(self, tune_N: int = 0, tune_r: int = 0, precisions=("fp64",)):
         """One small window (the ellipse (1, 0.6) at k0 = 60) in every slot at
         once, so the first request of each slot does not pay for loading the
-        library's kernels into its device context."""
+        library's kernels into its device context.  With tune_N, each slot's
+        context also times the GEMM tiles for the shape classes of a window of
+        that size (qb_ctx_autotune with 32 nodes and tune_r probes, default
+        N/10); the picks then serve windows of every size in those classes."""
+        if tune_N:
+            r = tune_r or max(64, tune_N // 10)
+            for c in self.slots:
+                for p in precisions:
+                    c.autotune(int(tune_N), 32, int(r), p)
         futs = [self.submit(WindowRequest(-1 - i, "ellipse", 60.0, 0.3, tol=1e-8, nq=32))
                 for i in range(len(self.slots))]
         for f in futs:
