@@ -9,7 +9,7 @@
 #include "../../include/qb.h"
 
 #define QB_MAX_COMP 8
-#define QB_GEMM_VARIANTS 21
+#define QB_GEMM_VARIANTS 23
 #define QB_GEMM_CLASSES 45
 
 // Boundary geometry resident on the device.  Node arrays are concatenated

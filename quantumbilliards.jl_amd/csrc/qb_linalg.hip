@@ -324,7 +324,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 typedef double qb_d2 __attribute__((ext_vector_type(2)));
 
-template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD>
+template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD, int GRP = 8>
 __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
     k_gemm_reg(int M, int N, int K, const cd* __restrict__ A, int64_t lda, int64_t sA,
                const cd* __restrict__ B, int64_t ldb, int64_t sB, cd* __restrict__ C,
@@ -335,13 +335,13 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
   const int li = lane & 15, lk = lane >> 4;
   int mt = blockIdx.x, ntile = blockIdx.y;
   if constexpr (XCD) {
-    // the XCD-aware tile order of k_gemm_sub
+    // the XCD-aware tile order of k_gemm_sub, in groups of GRP m-tiles
     const unsigned gx = gridDim.x, gy = gridDim.y, ntiles = gx * gy;
     if ((ntiles & 7u) == 0) {
       const unsigned L = blockIdx.x + gx * blockIdx.y;
       const unsigned t = (L & 7u) * (ntiles >> 3) + (L >> 3);
-      const unsigned grp = t / (8u * gy), first = grp * 8u;
-      const unsigned gm = gx - first < 8u ? gx - first : 8u;
+      const unsigned grp = t / ((unsigned)GRP * gy), first = grp * (unsigned)GRP;
+      const unsigned gm = gx - first < (unsigned)GRP ? gx - first : (unsigned)GRP;
       const unsigned loc = t - first * gy;
       mt = (int)(first + loc % gm);
       ntile = (int)(loc / gm);
@@ -463,13 +463,13 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
     }
 }
 
-template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD>
+template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD, int GRP = 8>
 static void launch_gemm_reg(qb_ctx* ctx, int M, int N, int K, const cd* A, int64_t lda,
                             int64_t sA, const cd* B, int64_t ldb, int64_t sB, cd* C,
                             int64_t ldc, int64_t sC, int batch) {
   constexpr int BM = WM * WGM, BN = WN * WGN;
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-  hipLaunchKernelGGL((k_gemm_reg<WM, WN, WGM, WGN, D, OCC, XCD>), g, dim3(64 * WGM * WGN), 0,
+  hipLaunchKernelGGL((k_gemm_reg<WM, WN, WGM, WGN, D, OCC, XCD, GRP>), g, dim3(64 * WGM * WGN), 0,
                      ctx->stream, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
@@ -713,7 +713,8 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 64, 64, 16, 2, 2>,         launch_gemm_bf6<128, 128, 2, 2>,
     launch_gemm_bf6<128, 128, 2, 2, true>,     launch_gemm_bf6<128, 64, 2, 2>,
     launch_gemm_bf6<64, 64, 2, 2>,
-    // 17-20: complex128 only (the slots repeat the default; kGemmAvail)
+    // 17-22: complex128 only (the slots repeat the default; kGemmAvail)
+    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>,
     launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>,
     launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
@@ -736,37 +737,40 @@ static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm_reg<32, 32, 4, 1, 7, 2, true>, launch_gemm_reg<32, 32, 4, 1, 8, 2, true>,
     launch_gemm_reg<32, 32, 2, 2, 6, 2, true>,
     // 20: 32 x 128 workgroup tile for updates of few rows (the block-row TRSM)
-    launch_gemm_reg<32, 32, 1, 4, 7, 2, true>};
+    launch_gemm_reg<32, 32, 1, 4, 7, 2, true>,
+    // 21, 22: the 128 x 32 tile in XCD groups of 2 and 4 m-tiles (less of A in L2)
+    launch_gemm_reg<32, 32, 4, 1, 7, 2, true, 2>, launch_gemm_reg<32, 32, 4, 1, 7, 2, true, 4>};
 static_assert(QB_GEMM_LEAN_C128 == 13, "the lean complex128 tile is variant 13");
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
      "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6", "-", "-",
-     "-", "-"},
+     "-", "-", "-", "-"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
      "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "64x64x8/3m/xcd/pf2/occ3", "64x64x8/3m/xcd/occ4",
      "64x64x8/3m/xcd/occ3", "128x32x4/3m/xcd/reg/d7", "128x32x4/3m/xcd/reg/d8",
-     "64x64x4/3m/xcd/reg/d6", "32x128x4/3m/xcd/reg/d7"}};
+     "64x64x4/3m/xcd/reg/d6", "32x128x4/3m/xcd/reg/d7", "128x32x4/3m/xcd2/reg/d7",
+     "128x32x4/3m/xcd4/reg/d7"}};
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true, false, false, false, false},
+     true, false, false, false, false, false, false},
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true, true, true, true, true, true}};
+     true, true, true, true, true, true, true, true}};
 // Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
 static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
-     true, true, true, true, false, false, false, false},
+     true, true, true, true, false, false, false, false, false, false},
     {false}};
 
 // Variants built on Gauss's three-product form (3 real MFMA products per
 // complex block product instead of 4).
 static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
-     false, false, false, false, false, false, false},
+     false, false, false, false, false, false, false, false, false},
     {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
-     true, true, true, true, true, true, true}};
+     true, true, true, true, true, true, true, true, true}};
 
 // Variants that pack 3-4 waves per SIMD (small tiles or k-tiles, low
 // register and LDS use): fastest alone, but they leave a concurrently
@@ -774,9 +778,9 @@ static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
 // excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, true, true, true, false,
-     false, false, false, false, false, false, false},
+     false, false, false, false, false, false, false, false, false},
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
-     false, true, true, true, false, false, false, false}};
+     false, true, true, true, false, false, false, false, false, false}};
 
 int qb_gemm_class(int M, int N, int K) {
   int cm = M < 1024 ? 0 : (M < 4096 ? 1 : 2);
@@ -794,7 +798,7 @@ static int gemm_default(int M, int N, int K) {
   if constexpr (sizeof(T) == sizeof(cf))
     return N <= 64 ? 1 : 0;
   else
-    return (N <= 32 || K <= 32) ? QB_GEMM_LEAN_C128 : K <= 64 ? 16 : 17;  // 128x32 register tile
+    return (N <= 32 || K <= 32) ? QB_GEMM_LEAN_C128 : K <= 64 ? 16 : 21;  // 128x32 register tile
 }
 
 template <typename T>

@@ -93,7 +93,7 @@ def test_getrf_and_solve(probe, dt, tol, n, b):
         assert err < tol * 10
 
 
-@pytest.mark.parametrize("variant", range(21))
+@pytest.mark.parametrize("variant", range(23))
 def test_every_tile_variant(probe, variant):
     """Each MFMA tile variant the autotuner may pick computes the same GEMM,
     LU and triangular solves.  The 1300 x 1000 shape gives tile counts that
@@ -129,9 +129,9 @@ def test_every_tile_variant(probe, variant):
             L.qbt_force_variant(c, ty, -1)
 
 
-@pytest.mark.parametrize("reg_variant", [17, 18, 19, 20])
+@pytest.mark.parametrize("reg_variant", [17, 18, 19, 20, 21, 22])
 def test_register_tiles_give_the_bits_of_the_lds_tile(probe, reg_variant):
-    """The register-streamed complex128 tiles (17-20) multiply the same
+    """The register-streamed complex128 tiles (17-22) multiply the same
     operands in the same k order with the same MFMA and Gauss form as the
     64 x 64 LDS tile (16), so C - A B agrees bit for bit (edge tiles in M, N
     and a partial k-step included)."""
