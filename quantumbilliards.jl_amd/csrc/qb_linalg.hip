@@ -323,12 +323,26 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 typedef double qb_d2 __attribute__((ext_vector_type(2)));
+typedef float qb_f2 __attribute__((ext_vector_type(2)));
+template <typename T>
+struct RegVec;
+template <>
+struct RegVec<cd> {
+  typedef qb_d2 V2;
+};
+template <>
+struct RegVec<cf> {
+  typedef qb_f2 V2;
+};
 
-template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD, int GRP = 8>
+template <typename T, int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD, int GRP = 8>
 __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
-    k_gemm_reg(int M, int N, int K, const cd* __restrict__ A, int64_t lda, int64_t sA,
-               const cd* __restrict__ B, int64_t ldb, int64_t sB, cd* __restrict__ C,
+    k_gemm_reg(int M, int N, int K, const T* __restrict__ A, int64_t lda, int64_t sA,
+               const T* __restrict__ B, int64_t ldb, int64_t sB, T* __restrict__ C,
                int64_t ldc, int64_t sC, const int* __restrict__ bix) {
+  typedef typename Scal<T>::S S;
+  typedef typename Scal<T>::V4 V4;
+  typedef typename RegVec<T>::V2 V2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int BM = WM * WGM, BN = WN * WGN;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -355,8 +369,8 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
   C += b * sC;
   // rows, columns and k outside the operands are read at a clamped address
   // (so every load is in bounds and unconditional) and zeroed when used
-  const cd* Ap[FM];
-  const cd* Bp[FN];
+  const T* Ap[FM];
+  const T* Bp[FN];
   bool aok[FM], bok[FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i) {
@@ -371,23 +385,23 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
     Bp[j] = B + (int64_t)(bok[j] ? col : N - 1) * ldb;
   }
   const int ns = (K + 3) >> 2;  // k-steps of 4
-  qb_d2 ra[D][FM], rb[D][FN];   // (re, im) of the lane's A and B elements, as loaded
+  V2 ra[D][FM], rb[D][FN];  // (re, im) of the lane's A and B elements, as loaded
   auto load = [&](auto slot, int s) {
     constexpr int d = decltype(slot)::value;
     const int k = 4 * s + lk < K ? 4 * s + lk : K - 1;
 #pragma unroll
-    for (int i = 0; i < FM; ++i) ra[d][i] = *(const qb_d2*)(Ap[i] + (int64_t)k * lda);
+    for (int i = 0; i < FM; ++i) ra[d][i] = *(const V2*)(Ap[i] + (int64_t)k * lda);
 #pragma unroll
-    for (int j = 0; j < FN; ++j) rb[d][j] = *(const qb_d2*)(Bp[j] + k);
+    for (int j = 0; j < FN; ++j) rb[d][j] = *(const V2*)(Bp[j] + k);
   };
-  doublex4 p1[FM][FN], p2[FM][FN], p3[FM][FN];
+  V4 p1[FM][FN], p2[FM][FN], p3[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      p1[i][j] = doublex4{0, 0, 0, 0};
-      p2[i][j] = doublex4{0, 0, 0, 0};
-      p3[i][j] = doublex4{0, 0, 0, 0};
+      p1[i][j] = V4{0, 0, 0, 0};
+      p2[i][j] = V4{0, 0, 0, 0};
+      p3[i][j] = V4{0, 0, 0, 0};
     }
   // k-step s: multiply ring slot s % D, masking rows, columns and k outside
   // the operands
@@ -397,19 +411,19 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
     constexpr int d = decltype(slot)::value;
     constexpr bool MASK = decltype(mask)::value;
     const bool kok = 4 * s + lk < K;
-    double xr[FM], xi[FM], yr[FN], yi[FN], as[FM], bs[FN];
+    S xr[FM], xi[FM], yr[FN], yi[FN], as[FM], bs[FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
       const bool ok = !MASK || (aok[i] && kok);
-      xr[i] = ok ? ra[d][i].x : 0.0;
-      xi[i] = ok ? ra[d][i].y : 0.0;
+      xr[i] = ok ? ra[d][i].x : S(0);
+      xi[i] = ok ? ra[d][i].y : S(0);
       as[i] = xr[i] + xi[i];
     }
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
       const bool ok = !MASK || (bok[j] && kok);
-      yr[j] = ok ? rb[d][j].x : 0.0;
-      yi[j] = ok ? rb[d][j].y : 0.0;
+      yr[j] = ok ? rb[d][j].x : S(0);
+      yi[j] = ok ? rb[d][j].y : S(0);
       bs[j] = yr[j] + yi[j];
     }
 #pragma unroll
@@ -450,11 +464,12 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
       if (gn >= N) continue;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int gm = m0 + 16 * i + 4 * v + lk;
+        // result register v of a lane: row 4v + lane/16 (f64), 4(lane/16) + v (f32)
+        const int gm = m0 + 16 * i + (sizeof(S) == 8 ? 4 * v + lk : 4 * lk + v);
         if (gm < M) {
-          cd* p = C + gm + (int64_t)gn * ldc;
-          cd c = *p;
-          const double q1 = p1[i][j][v], q2 = p2[i][j][v];
+          T* p = C + gm + (int64_t)gn * ldc;
+          T c = *p;
+          const S q1 = p1[i][j][v], q2 = p2[i][j][v];
           c.re -= q1 - q2;
           c.im -= p3[i][j][v] - q1 - q2;
           *p = c;
@@ -463,13 +478,13 @@ __global__ void __launch_bounds__(64 * WGM * WGN, OCC)
     }
 }
 
-template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD, int GRP = 8>
-static void launch_gemm_reg(qb_ctx* ctx, int M, int N, int K, const cd* A, int64_t lda,
-                            int64_t sA, const cd* B, int64_t ldb, int64_t sB, cd* C,
+template <int WM, int WN, int WGM, int WGN, int D, int OCC, bool XCD, int GRP = 8, typename T = cd>
+static void launch_gemm_reg(qb_ctx* ctx, int M, int N, int K, const T* A, int64_t lda,
+                            int64_t sA, const T* B, int64_t ldb, int64_t sB, T* C,
                             int64_t ldc, int64_t sC, int batch) {
   constexpr int BM = WM * WGM, BN = WN * WGN;
   dim3 g((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-  hipLaunchKernelGGL((k_gemm_reg<WM, WN, WGM, WGN, D, OCC, XCD, GRP>), g, dim3(64 * WGM * WGN), 0,
+  hipLaunchKernelGGL((k_gemm_reg<T, WM, WN, WGM, WGN, D, OCC, XCD, GRP>), g, dim3(64 * WGM * WGN), 0,
                      ctx->stream, M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, ctx->bix);
 }
 
@@ -713,10 +728,10 @@ static const gemm_fn_c64 kGemmC64[QB_GEMM_VARIANTS] = {
     launch_gemm<cf, 64, 64, 16, 2, 2>,         launch_gemm_bf6<128, 128, 2, 2>,
     launch_gemm_bf6<128, 128, 2, 2, true>,     launch_gemm_bf6<128, 64, 2, 2>,
     launch_gemm_bf6<64, 64, 2, 2>,
-    // 17-22: complex128 only (the slots repeat the default; kGemmAvail)
-    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>,
-    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>,
-    launch_gemm<cf, 128, 128, 16, 2, 2>, launch_gemm<cf, 128, 128, 16, 2, 2>};
+    // 17-22: operands streamed into registers, no LDS (k_gemm_reg)
+    launch_gemm_reg<32, 32, 4, 1, 8, 2, true, 2, cf>, launch_gemm_reg<32, 64, 2, 2, 6, 2, true, 2, cf>,
+    launch_gemm_reg<32, 32, 2, 2, 8, 2, true, 2, cf>, launch_gemm_reg<64, 32, 2, 2, 6, 2, true, 2, cf>,
+    launch_gemm_reg<32, 32, 4, 1, 8, 3, true, 2, cf>, launch_gemm_reg<32, 32, 4, 1, 12, 2, true, 2, cf>};
 static const gemm_fn_c128 kGemmC128[QB_GEMM_VARIANTS] = {
     launch_gemm<cd, 128, 64, 16, 2, 2>,       launch_gemm<cd, 128, 32, 16, 4, 1>,
     launch_gemm<cd, 64, 64, 16, 2, 2>,        launch_gemm<cd, 64, 128, 16, 2, 2>,
@@ -744,8 +759,9 @@ static_assert(QB_GEMM_LEAN_C128 == 13, "the lean complex128 tile is variant 13")
 static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
     {"128x128x16", "256x64x16", "64x256x16", "128x64x16", "128x128x32", "128x64x16/xcd",
      "128x128x16/xcd", "128x64x16/3m", "128x128x16/3m", "128x64x16/3m/xcd", "64x64x16/3m", "64x64x16/3m/xcd", "64x64x16",
-     "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6", "-", "-",
-     "-", "-", "-", "-"},
+     "128x128x32/bf16x6", "128x128x32/bf16x6/xcd", "128x64x32/bf16x6", "64x64x32/bf16x6",
+     "128x32x4/3m/xcd2/reg/d8", "64x128x4/3m/xcd2/reg/d6", "64x64x4/3m/xcd2/reg/d8",
+     "128x64x4/3m/xcd2/reg/d6", "128x32x4/3m/xcd2/reg/d8/occ3", "128x32x4/3m/xcd2/reg/d12"},
     {"128x64x16", "128x32x16", "64x64x16", "64x128x16", "256x32x16", "64x64x16/xcd",
      "128x64x16/xcd", "64x64x16/3m", "64x32x16/3m", "64x64x16/xcd/pf2", "64x32x16/3m/pf2", "64x64x16/3m/o2",
      "64x64x16/3m/xcd/pf2", "32x32x8/3m/lean", "64x64x8/3m/xcd/pf2/occ3", "64x64x8/3m/xcd/occ4",
@@ -755,7 +771,7 @@ static const char* kGemmNames[2][QB_GEMM_VARIANTS] = {
 // Variants that exist for the type (the tuner times only these).
 static const bool kGemmAvail[2][QB_GEMM_VARIANTS] = {
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
-     true, false, false, false, false, false, false},
+     true, true, true, true, true, true, true},
     {true, true, true, true, true, true, true, true, true, true, true, true, true, true, true,
      true, true, true, true, true, true, true, true}};
 // Variants on the bf16 matrix cores (three-way bf16 splits of FP32 operands).
@@ -768,7 +784,7 @@ static const bool kGemmBf6[2][QB_GEMM_VARIANTS] = {
 // complex block product instead of 4).
 static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, true, true, true, false, false,
-     false, false, false, false, false, false, false, false, false},
+     false, false, false, true, true, true, true, true, true},
     {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
      true, true, true, true, true, true, true, true, true}};
 
@@ -778,7 +794,7 @@ static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
 // excludes them.
 static const bool kGemmPacks[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, false, false, false, true, true, true, false,
-     false, false, false, false, false, false, false, false, false},
+     false, false, false, false, false, false, false, true, false},
     {false, false, false, false, false, false, false, false, false, false, false, false, false,
      false, true, true, true, false, false, false, false, false, false}};
 
@@ -792,11 +808,11 @@ int qb_gemm_class(int M, int N, int K) {
 template <typename T>
 static int gemm_default(int M, int N, int K) {
   // the tile a shape gets when the autotuner has not timed its class (the
-  // serving loop's windows of many sizes): for complex128 the autotuner's
-  // usual picks
+  // serving loop's windows of many sizes): tiles the autotuner picks for
+  // such classes, and for complex64 beside the LU panel (no LDS)
   (void)M;
   if constexpr (sizeof(T) == sizeof(cf))
-    return N <= 64 ? 1 : 0;
+    return N <= 64 ? 1 : K <= 64 ? 0 : 20;  // 128x64 register tile
   else
     return (N <= 32 || K <= 32) ? QB_GEMM_LEAN_C128 : K <= 64 ? 16 : 21;  // 128x32 register tile
 }

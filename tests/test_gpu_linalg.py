@@ -112,7 +112,8 @@ def test_every_tile_variant(probe, variant):
     memory straight into the MFMA registers (no LDS, no barrier) with 7, 8 or
     6 k-steps of 4 in flight (128 x 32, 64 x 64 and 32 x 128 (20) workgroup tiles of 32 x 32
     wave tiles), so K = 33 ends in a partial step and K = 40 in a partial
-    ring; 1300 x 1000 x 40 has interior waves, which skip the masks."""
+    ring; 1300 x 1000 x 40 has interior waves, which skip the masks;
+    complex64 17-22 are the same register-streamed tiles on FP32 MFMA."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
     try:
@@ -129,29 +130,31 @@ def test_every_tile_variant(probe, variant):
             L.qbt_force_variant(c, ty, -1)
 
 
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
 @pytest.mark.parametrize("reg_variant", [17, 18, 19, 20, 21, 22])
-def test_register_tiles_give_the_bits_of_the_lds_tile(probe, reg_variant):
-    """The register-streamed complex128 tiles (17-22) multiply the same
+def test_register_tiles_give_the_bits_of_the_lds_tile(probe, reg_variant, dt):
+    """The register-streamed tiles (17-22 of each type) multiply the same
     operands in the same k order with the same MFMA and Gauss form as the
-    64 x 64 LDS tile (16), so C - A B agrees bit for bit (edge tiles in M, N
-    and a partial k-step included)."""
+    64 x 64 Gauss LDS tile (complex128 16, complex64 10), so C - A B agrees bit
+    for bit (edge tiles in M, N and a partial k-step included)."""
     L, ctx = probe
     c = C.c_void_p(ctx.value)
+    ty, lds = (1, 16) if dt == np.complex128 else (0, 10)
+    f = L.qbt_gemm_sub_c128 if dt == np.complex128 else L.qbt_gemm_sub_c64
     rng = np.random.default_rng(11)
     M, N, K, b = 300, 129, 131, 2
-    A = _rand(rng, (b, K, M), np.complex128)
-    B = _rand(rng, (b, N, K), np.complex128)
-    C0 = _rand(rng, (b, N, M), np.complex128)
+    A = _rand(rng, (b, K, M), dt)
+    B = _rand(rng, (b, N, K), dt)
+    C0 = _rand(rng, (b, N, M), dt)
     out = []
     try:
-        for v in (16, reg_variant):
-            L.qbt_force_variant(c, 1, v)
+        for v in (lds, reg_variant):
+            L.qbt_force_variant(c, ty, v)
             Cm = C0.copy()
-            assert L.qbt_gemm_sub_c128(c, M, N, K, C.c_void_p(_c(A)), C.c_void_p(_c(B)),
-                                       C.c_void_p(_c(Cm)), b) == 0
+            assert f(c, M, N, K, C.c_void_p(_c(A)), C.c_void_p(_c(B)), C.c_void_p(_c(Cm)), b) == 0
             out.append(Cm)
     finally:
-        L.qbt_force_variant(c, 1, -1)
+        L.qbt_force_variant(c, ty, -1)
     assert np.array_equal(out[0], out[1])
 
 
