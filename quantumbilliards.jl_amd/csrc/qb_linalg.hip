@@ -808,11 +808,12 @@ int qb_gemm_class(int M, int N, int K) {
 template <typename T>
 static int gemm_default(int M, int N, int K) {
   // the tile a shape gets when the autotuner has not timed its class (the
-  // serving loop's windows of many sizes): tiles the autotuner picks for
-  // such classes, and for complex64 beside the LU panel (no LDS)
+  // serving loop's windows of many sizes): for complex128 the autotuner's
+  // usual picks; complex64 keeps four-product tiles (the three-product form
+  // loses about a bit more of FP32 accuracy, which the refinement pays for)
   (void)M;
   if constexpr (sizeof(T) == sizeof(cf))
-    return N <= 64 ? 1 : K <= 64 ? 0 : 20;  // 128x64 register tile
+    return N <= 64 ? 1 : 0;
   else
     return (N <= 32 || K <= 32) ? QB_GEMM_LEAN_C128 : K <= 64 ? 16 : 21;  // 128x32 register tile
 }
