@@ -302,17 +302,19 @@ __global__ void __launch_bounds__(256, OCC) k_gemm_sub(int M, int N, int K, cons
 }
 
 // ---------------------------------------------------------------------------
-// complex128 C[b] -= A[b] * B[b] with the operands streamed from global
-// memory straight into the MFMA operand registers: no LDS and no barrier.
-// Each wave owns a WM x WN tile of C (WGM x WGN waves per workgroup share
-// their A rows and B columns through L1) and keeps the fragments of the next
-// D k-steps (4-deep each) in flight in a register ring.  Lane l holds
-// A[m0 + 16i + l % 16][k + l / 16] and B[k + l / 16][n0 + 16j + l % 16]:
-// 16 lanes read 256 contiguous bytes of an A column, 4 lanes 64 contiguous
-// bytes of a B column.  Gauss's three products as in k_gemm_sub (P1 = Ar Br,
-// P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi)), accumulated as +AB and subtracted in
-// the epilogue, the same k order and MFMA as k_gemm_sub, so the two give the
-// same bits.
+// C[b] -= A[b] * B[b] (complex128 on FP64 MFMA, complex64 on FP32 MFMA) with
+// the operands streamed from global memory straight into the MFMA operand
+// registers: no LDS and no barrier.  Each wave owns a WM x WN tile of C
+// (WGM x WGN waves per workgroup share their A rows and B columns through
+// L1) and keeps the fragments of the next D k-steps (4-deep each) in flight
+// in a register ring.  Lane l holds A[m0 + 16i + l % 16][k + l / 16] and
+// B[k + l / 16][n0 + 16j + l % 16]: 16 lanes read one contiguous run of an A
+// column, 4 lanes one of a B column.  Gauss's three products as in
+// k_gemm_sub (P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi)), accumulated
+// as +AB and subtracted in the epilogue, the same k order and MFMA as
+// k_gemm_sub, so the two give the same bits.  XCD: the tile order of
+// k_gemm_sub, with GRP m-tiles per group (2-4 keep the in-flight A and B
+// panels of an XCD within its L2).
 // ---------------------------------------------------------------------------
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
