@@ -790,6 +790,13 @@ static const bool kGemmG3[2][QB_GEMM_VARIANTS] = {
     {false, false, false, false, false, false, false, true, true, false, true, true, true, true,
      true, true, true, true, true, true, true, true, true}};
 
+// Register-streamed variants (k_gemm_reg: no LDS, no barriers).
+static const bool kGemmReg[2][QB_GEMM_VARIANTS] = {
+    {false, false, false, false, false, false, false, false, false, false, false, false, false,
+     false, false, false, false, true, true, true, true, true, true},
+    {false, false, false, false, false, false, false, false, false, false, false, false, false,
+     false, false, false, false, true, true, true, true, true, true}};
+
 // Variants that pack 3-4 waves per SIMD (small tiles or k-tiles, low
 // register and LDS use): fastest alone, but they leave a concurrently
 // running kernel (the lookahead panel) no room on a CU, so gemm_pick_conc
@@ -1387,13 +1394,27 @@ static int panel_threads_beside() {
 }
 
 // The lean kernels for a complex128 panel beside the trailing update
-// (qb_ctx::panel_lean); QB_PANEL_LEAN=0 (read once) turns them off.
-static bool panel_lean_enabled() {
-  static const bool on = [] {
+// (qb_ctx::panel_lean).  They leave LDS to the trailing tile, which pays only
+// when that tile holds LDS: beside a register-streamed tile (no LDS) the
+// ordinary panel kernels are faster (1009 ms against 1022 ms of LU on the
+// 16-node N = 8192 window).  QB_PANEL_LEAN (read once): 0 never, 1 always,
+// unset: when the trailing update of the step takes an LDS tile.
+static int panel_lean_mode() {
+  static const int mode = [] {
     const char* e = getenv("QB_PANEL_LEAN");
-    return !(e && e[0] == '0');
+    return !e || !e[0] ? -1 : e[0] == '0' ? 0 : 1;
   }();
-  return on;
+  return mode;
+}
+
+template <typename T>
+static bool panel_lean_beside(const qb_ctx* ctx, int M, int N, int K) {
+  if (sizeof(T) != sizeof(cd)) return false;
+  const int mode = panel_lean_mode();
+  if (mode >= 0) return mode == 1;
+  int v = ctx->gemm_pick_conc[1][qb_gemm_class(M, N, K)];
+  if (v < 0 || v >= QB_GEMM_VARIANTS || !kGemmAvail[1][v]) v = gemm_default<T>(M, N, K);
+  return !kGemmReg[1][v];
 }
 
 // Recursive panel: factor columns [c0, c0+w) (rows c0..n-1) of every matrix,
@@ -1482,7 +1503,7 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
     QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
     QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
     rc = on_aux([&] {
-      ctx->panel_lean = sizeof(T) == sizeof(cd) && m > kb1 && panel_lean_enabled();
+      ctx->panel_lean = m > kb1 && panel_lean_beside<T>(ctx, m, m - kb1, kb);
       const int r = panel_rec(ctx, A, lda, sA, n, k1, kb1, ipiv, sP, info, batch,
                               m > kb1 ? panel_threads_beside<T>() : 1024);
       ctx->panel_lean = false;
