@@ -1397,7 +1397,7 @@ static int panel_threads_beside() {
 // (qb_ctx::panel_lean).  They leave LDS to the trailing tile, which pays only
 // when that tile holds LDS: beside a register-streamed tile (no LDS) the
 // ordinary panel kernels are faster (1009 ms against 1022 ms of LU on the
-// 16-node N = 8192 window).  QB_PANEL_LEAN (read once): 0 never, 1 always,
+// 48 matrices of N = 8192).  QB_PANEL_LEAN (read once): 0 never, 1 always,
 // unset: when the trailing update of the step takes an LDS tile.
 static int panel_lean_mode() {
   static const int mode = [] {
