@@ -24,6 +24,10 @@ int qb_getrf_graph(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv,
 void qb_graphs_clear(qb_ctx* ctx);
 // Triangular solves only (the row permutation is applied separately, fused
 // into the gather that fills B).
+// Triangular-solve leaf kernel for every later launch of the process: 1 the
+// one-wave LDS leaf (k_trsm_leaf), 0 the register leaf (k_trsm_leaf_reg), -1
+// back to the default (QB_TRSM_LEAF, else the register leaf).
+void qb_set_trsm_leaf_lds(int mode);
 template <typename T>
 int qb_trsm_lu_batched(qb_ctx* ctx, const T* LU, int n, int64_t lda, int64_t sA, T* B, int nrhs,
                        int64_t ldb, int64_t sB, int batch);

@@ -1258,6 +1258,124 @@ __global__ void __launch_bounds__(64) k_trsm_leaf(int n, int nrhs, const T* __re
   }
 }
 
+// The same leaf with W waves per workgroup (W x 64 right-hand-side columns)
+// and each lane's column held in registers (up to 256 VGPRs: two waves per
+// SIMD): the triangle is staged once per workgroup and read by broadcast, and
+// each wave moves its columns between global memory and registers through a
+// 16-column LDS stage (coalesced: 32 lanes per column, four rounds).  For
+// complex128, 17 + 8.4 W KB of LDS instead of 51 KB per wave, so a CU holds 8
+// waves (W = 2) or 6 (W = 1) instead of 3.  Every row takes the operations of
+// k_trsm_leaf in the same order.
+template <typename T, bool UNIT_LOWER, int W>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2)))
+k_trsm_leaf_reg(int n, int nrhs, const T* __restrict__ L, int64_t ldl, int64_t sL,
+                T* __restrict__ B, int64_t ldb, int64_t sB, const int* __restrict__ bix) {
+  __shared__ T Ls[32][33];
+  __shared__ T St[W][16][33];
+  const int64_t b = bix ? bix[blockIdx.y] : blockIdx.y;
+  L += b * sL;
+  B += b * sB;
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int c0 = blockIdx.x * (64 * W) + w * 64;  // first column of this wave
+  for (int e = tid; e < 32 * 32; e += 64 * W) {
+    const int i = e % 32, j = e / 32;
+    Ls[i][j] = (i < n && j < n) ? L[i + (int64_t)j * ldl] : T{0, 0};
+  }
+  T x[32];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int cc = 2 * t + (lane >> 5), i = lane & 31, col = c0 + 16 * q + cc;
+      St[w][cc][i] = (i < n && col < nrhs) ? B[i + (int64_t)col * ldb] : T{0, 0};
+    }
+    __syncthreads();
+    if ((lane >> 4) == q) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) x[i] = St[w][lane & 15][i];
+    }
+    __syncthreads();
+  }
+  if (!UNIT_LOWER && tid < 32) {
+    T d = tid < n ? Ls[tid][tid] : T{1, 0};
+    auto den = d.re * d.re + d.im * d.im;
+    Ls[tid][tid] = T{d.re / den, -d.im / den};
+  }
+  __syncthreads();
+  if (UNIT_LOWER) {
+#pragma unroll
+    for (int ib = 0; ib < 32; ib += 8) {
+      T v[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) v[a] = x[ib + a];
+#pragma unroll
+      for (int j = 0; j < ib; ++j) {
+        const T xj = x[j];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const T l = Ls[ib + a][j];
+          v[a].re -= l.re * xj.re - l.im * xj.im;
+          v[a].im -= l.re * xj.im + l.im * xj.re;
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int c = 0; c < a; ++c) {
+          const T l = Ls[ib + a][ib + c];
+          v[a].re -= l.re * v[c].re - l.im * v[c].im;
+          v[a].im -= l.re * v[c].im + l.im * v[c].re;
+        }
+#pragma unroll
+      for (int a = 0; a < 8; ++a) x[ib + a] = v[a];
+    }
+  } else {
+#pragma unroll
+    for (int ib = 24; ib >= 0; ib -= 8) {
+      T v[8];
+#pragma unroll
+      for (int a = 0; a < 8; ++a) v[a] = x[ib + a];
+#pragma unroll
+      for (int j = ib + 8; j < 32; ++j) {
+        const T xj = x[j];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          const T u = Ls[ib + a][j];
+          v[a].re -= u.re * xj.re - u.im * xj.im;
+          v[a].im -= u.re * xj.im + u.im * xj.re;
+        }
+      }
+#pragma unroll
+      for (int a = 7; a >= 0; --a) {
+#pragma unroll
+        for (int c = a + 1; c < 8; ++c) {
+          const T u = Ls[ib + a][ib + c];
+          v[a].re -= u.re * v[c].re - u.im * v[c].im;
+          v[a].im -= u.re * v[c].im + u.im * v[c].re;
+        }
+        const T d = Ls[ib + a][ib + a];
+        v[a] = T{v[a].re * d.re - v[a].im * d.im, v[a].re * d.im + v[a].im * d.re};
+      }
+#pragma unroll
+      for (int a = 0; a < 8; ++a) x[ib + a] = v[a];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((lane >> 4) == q) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) St[w][lane & 15][i] = x[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int cc = 2 * t + (lane >> 5), i = lane & 31, col = c0 + 16 * q + cc;
+      if (i < n && col < nrhs) B[i + (int64_t)col * ldb] = St[w][cc][i];
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host orchestration
 // ---------------------------------------------------------------------------
@@ -1275,6 +1393,44 @@ static int laswp(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int
 static int split32(int n) {
   int h = (n / 2 + 31) / 32 * 32;
   return h >= n ? n / 2 : h;
+}
+
+// Leaf launch (n <= 32): k_trsm_leaf_reg with two waves per workgroup when
+// the launch has at least 8 waves per CU to give (the LU's block-row solves),
+// else the one-wave LDS leaf k_trsm_leaf, which is the faster of the two at
+// low occupancy (the moment solves: 576 waves, 270 against 274 ms per window).
+// QB_TRSM_LEAF=lds (read once) or qb_set_trsm_leaf_lds(1) takes the LDS leaf
+// everywhere; qb_set_trsm_leaf_lds(0) the register leaf everywhere (one wave
+// per workgroup for the small launches).
+static int g_trsm_leaf_lds = -1;
+void qb_set_trsm_leaf_lds(int mode) { g_trsm_leaf_lds = mode; }
+static int trsm_leaf_mode() {
+  static const int env = [] {
+    const char* e = getenv("QB_TRSM_LEAF");
+    return e && e[0] == 'l' ? 1 : -1;
+  }();
+  return g_trsm_leaf_lds >= 0 ? g_trsm_leaf_lds : env;
+}
+
+template <typename T, bool LOWER>
+static int trsm_leaf(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int64_t sL, T* B,
+                     int64_t ldb, int64_t sB, int batch) {
+  int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
+  const int64_t waves = (int64_t)((nrhs + 63) / 64) * batch;
+  const int mode = trsm_leaf_mode();
+  if (mode == 1 || (mode < 0 && waves < 8 * 256)) {
+    hipLaunchKernelGGL((k_trsm_leaf<T, LOWER>), dim3((nrhs + 63) / 64, batch), dim3(64), 0,
+                       ctx->stream, n, nrhs, L, ldl, sL, B, ldb, sB, ctx->bix);
+  } else if (waves >= 8 * 256) {
+    hipLaunchKernelGGL((k_trsm_leaf_reg<T, LOWER, 2>), dim3((nrhs + 127) / 128, batch),
+                       dim3(128), 0, ctx->stream, n, nrhs, L, ldl, sL, B, ldb, sB, ctx->bix);
+  } else {
+    hipLaunchKernelGGL((k_trsm_leaf_reg<T, LOWER, 1>), dim3((nrhs + 63) / 64, batch), dim3(64),
+                       0, ctx->stream, n, nrhs, L, ldl, sL, B, ldb, sB, ctx->bix);
+  }
+  qb_prof_end(ctx, slot);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
 }
 
 // X = L^{-1} B with L the unit-lower n x n block at L (ld ldl).
@@ -1334,15 +1490,7 @@ static int trsm_lower(qb_ctx* ctx, int n, int nrhs, const T* L, int64_t ldl, int
     QB_HIP(ctx, hipGetLastError());
     return QB_OK;
   }
-  if (n <= 32) {
-    dim3 g((nrhs + 63) / 64, batch);
-    int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
-    hipLaunchKernelGGL((k_trsm_leaf<T, true>), g, dim3(64), 0, ctx->stream, n, nrhs, L, ldl, sL,
-                       B, ldb, sB, ctx->bix);
-    qb_prof_end(ctx, slot);
-    QB_HIP(ctx, hipGetLastError());
-    return QB_OK;
-  }
+  if (n <= 32) return trsm_leaf<T, true>(ctx, n, nrhs, L, ldl, sL, B, ldb, sB, batch);
   int n1 = split32(n);
   int rc = trsm_lower(ctx, n1, nrhs, L, ldl, sL, B, ldb, sB, batch);
   if (rc) return rc;
@@ -1357,15 +1505,7 @@ template <typename T>
 static int trsm_upper(qb_ctx* ctx, int n, int nrhs, const T* U, int64_t ldu, int64_t sU, T* B,
                       int64_t ldb, int64_t sB, int batch) {
   if (n <= 0 || nrhs <= 0) return QB_OK;
-  if (n <= 32) {
-    dim3 g((nrhs + 63) / 64, batch);
-    int slot = qb_prof_begin(ctx, QB_K_TRSM, 4.0 * n * n * (double)nrhs * batch, 0);
-    hipLaunchKernelGGL((k_trsm_leaf<T, false>), g, dim3(64), 0, ctx->stream, n, nrhs, U, ldu,
-                       sU, B, ldb, sB, ctx->bix);
-    qb_prof_end(ctx, slot);
-    QB_HIP(ctx, hipGetLastError());
-    return QB_OK;
-  }
+  if (n <= 32) return trsm_leaf<T, false>(ctx, n, nrhs, U, ldu, sU, B, ldb, sB, batch);
   int n1 = split32(n);
   int rc = trsm_upper(ctx, n - n1, nrhs, U + n1 + (int64_t)n1 * ldu, ldu, sU, B + n1, ldb, sB,
                       batch);
@@ -1464,6 +1604,15 @@ int qb_lu_nb(int n) {
   return n >= 4096 ? 512 : 256;
 }
 
+// QB_LASWP_AUX=0 (read once) keeps every row interchange on the main stream.
+static bool laswp_left_aux() {
+  static const bool on = [] {
+    const char* e = getenv("QB_LASWP_AUX");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
                      int* info, int batch) {
@@ -1482,9 +1631,20 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
   };
   int rc = panel_rec(ctx, A, lda, sA, n, 0, n < NB ? n : NB, ipiv, sP, info, batch);
   if (rc) return rc;
+  // The interchanges of step s in the columns left of its panel (the L part
+  // of earlier panels) run on the auxiliary stream, ahead of the next panel:
+  // nothing on the main stream touches those columns any more once the
+  // previous trailing update (which reads the last panel's L) is done.
+  const bool left_aux = laswp_left_aux();
   for (int k0 = 0; k0 < n; k0 += NB) {
     int kb = n - k0 < NB ? n - k0 : NB;
-    rc = laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch);
+    if (left_aux && k0 > 0) {
+      QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
+      QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
+      rc = on_aux([&] { return laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch); });
+    } else {
+      rc = laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch);
+    }
     if (rc) return rc;
     rc = laswp(ctx, A, lda, sA, k0 + kb, n, k0, k0 + kb, ipiv, sP, batch);
     if (rc) return rc;
@@ -1518,6 +1678,10 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
       ctx->gemm_conc = false;
       if (rc) return rc;
     }
+    QB_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_aux, 0));
+  }
+  if (left_aux && n > NB) {  // the last step's interchanges
+    QB_HIP(ctx, hipEventRecord(ctx->ev_aux, aux));
     QB_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_aux, 0));
   }
   return QB_OK;
@@ -1592,9 +1756,45 @@ __global__ void k_perm_from_ipiv(const int* __restrict__ ipiv, int* __restrict__
   }
 }
 
+// The same with one workgroup per matrix and both arrays in LDS (8 n bytes):
+// the interchanges stay sequential, but on LDS instead of global memory.
+__global__ void __launch_bounds__(256) k_perm_from_ipiv_lds(const int* __restrict__ ipiv,
+                                                            int* __restrict__ perm, int n) {
+  extern __shared__ int sm_perm[];
+  int* pm = sm_perm;
+  int* ip = sm_perm + n;
+  const int64_t b = blockIdx.x;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    pm[i] = i;
+    ip[i] = ipiv[b * n + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int s = 0; s < n; ++s) {
+      const int p = ip[s];
+      if (p != s) {
+        const int t = pm[s];
+        pm[s] = pm[p];
+        pm[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) perm[b * n + i] = pm[i];
+}
+
 int qb_perm_from_ipiv(qb_ctx* ctx, const int* ipiv, int* perm, int n, int batch) {
-  hipLaunchKernelGGL(k_perm_from_ipiv, dim3((batch + 63) / 64), dim3(64), 0, ctx->stream, ipiv,
-                     perm, n, batch);
+  const size_t lds = (size_t)8 * n;
+  if (n > 0 && lds <= (size_t)96 * 1024) {
+    if (lds > (size_t)64 * 1024)
+      (void)hipFuncSetAttribute((const void*)k_perm_from_ipiv_lds,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_perm_from_ipiv_lds, dim3(batch), dim3(256), lds, ctx->stream, ipiv, perm,
+                       n);
+  } else {
+    hipLaunchKernelGGL(k_perm_from_ipiv, dim3((batch + 63) / 64), dim3(64), 0, ctx->stream, ipiv,
+                       perm, n, batch);
+  }
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }

@@ -70,6 +70,7 @@ int qbt_trsm_c128(qb_ctx* c, int n, const cd* LU, cd* B, int nrhs, int b) {
 int qbt_trsm_c64(qb_ctx* c, int n, const cf* LU, cf* B, int nrhs, int b) {
   return run_trsm<cf>(c, n, LU, B, nrhs, b);
 }
+void qbt_set_trsm_leaf_lds(int mode) { qb_set_trsm_leaf_lds(mode); }
 }
 
 // Timing of gemm_sub at one shape: buffers filled on the device, `iters`

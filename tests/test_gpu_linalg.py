@@ -131,6 +131,50 @@ def test_every_tile_variant(probe, variant):
 
 
 @pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_register_leaf_gives_the_bits_of_the_lds_leaf(probe, dt):
+    """The triangular-solve leaf with each lane's column in registers
+    (k_trsm_leaf_reg, one and two waves per workgroup) does the operations of
+    the one-wave LDS leaf in the same order, so the LU factors and both
+    triangular solves agree bit for bit.  n = 80 gives leaves of 32, 32 and 16
+    rows; 32 matrices x 4100 right-hand sides take the two-wave form, with a
+    partial last workgroup."""
+    L, ctx = probe
+    c = C.c_void_p(ctx.value)
+    rng = np.random.default_rng(5)
+    getrf = L.qbt_getrf_c128 if dt == np.complex128 else L.qbt_getrf_c64
+    trsm = L.qbt_trsm_c128 if dt == np.complex128 else L.qbt_trsm_c64
+    n, b, nr = 80, 32, 4100
+    T = _rand(rng, (b, n, n), dt) + 4 * np.eye(n, dtype=dt)[None]
+    B0 = _rand(rng, (b, nr, n), dt)
+    T2 = _rand(rng, (2, 300, 300), dt)
+    out = []
+    try:
+        for mode in (1, 0):
+            L.qbt_set_trsm_leaf_lds(mode)
+            A = T.copy()
+            ipiv = np.zeros((b, n), dtype=np.int32)
+            assert getrf(c, n, C.c_void_p(_c(A)), C.c_void_p(_c(ipiv)), b) == 0
+            X = B0.copy()
+            assert trsm(c, n, C.c_void_p(_c(A)), C.c_void_p(_c(X)), nr, b) == 0
+            A2 = T2.copy()
+            ip2 = np.zeros((2, 300), dtype=np.int32)
+            assert getrf(c, 300, C.c_void_p(_c(A2)), C.c_void_p(_c(ip2)), 2) == 0
+            out.append((A, ipiv, X, A2, ip2))
+    finally:
+        L.qbt_set_trsm_leaf_lds(-1)
+    for u, v in zip(out[0], out[1]):
+        assert np.array_equal(u, v)
+    # and the solve is a solve
+    Lm = np.tril(out[1][0][0].T, -1) + np.eye(n)
+    Um = np.triu(out[1][0][0].T)
+    perm_rhs = B0[0].T.astype(np.complex128)
+    ref = sla.solve_triangular(Um, sla.solve_triangular(Lm, perm_rhs, lower=True,
+                                                        unit_diagonal=True))
+    tol = 1e-12 if dt == np.complex128 else 1e-4
+    assert np.max(np.abs(out[1][2][0].T - ref)) / np.max(np.abs(ref)) < tol
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
 @pytest.mark.parametrize("reg_variant", [17, 18, 19, 20, 21, 22])
 def test_register_tiles_give_the_bits_of_the_lds_tile(probe, reg_variant, dt):
     """The register-streamed tiles (17-22 of each type) multiply the same
