@@ -1155,6 +1155,56 @@ __global__ void k_laswp(T* __restrict__ A, int64_t lda, int64_t sA, int cb, int 
   }
 }
 
+// The interchanges ipiv[k0..k1) of columns [cb, ce) when every row they touch
+// lies in [k0, n) with n - k0 small (the last steps of the LU): each
+// workgroup turns the interchanges into a permutation of those R = n - k0
+// rows (in LDS, one thread), then moves TC columns at a time through LDS,
+// loading and storing along the rows (coalesced) instead of one lane per
+// column.  Same result as k_laswp.
+template <typename T>
+__global__ void __launch_bounds__(256) k_laswp_tail(T* __restrict__ A, int64_t lda, int64_t sA,
+                                                    int cb, int ce, int k0, int k1, int n,
+                                                    const int* __restrict__ ipiv, int64_t sP,
+                                                    int cpw, int tc) {
+  extern __shared__ __align__(16) unsigned char sm_tail[];
+  const int R = n - k0, ns = k1 - k0;
+  T* tile = (T*)sm_tail;                                   // [tc][R]
+  int* perm = (int*)(sm_tail + (size_t)tc * R * sizeof(T));  // [R]
+  int* ip = perm + R;                                      // [ns]
+  const int64_t b = blockIdx.y;
+  A += b * sA;
+  ipiv += b * sP;
+  for (int r = threadIdx.x; r < R; r += 256) perm[r] = r;
+  for (int i = threadIdx.x; i < ns; i += 256) ip[i] = ipiv[k0 + i] - k0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < ns; ++i) {
+      const int j = ip[i];
+      if (j != i) {
+        const int t = perm[i];
+        perm[i] = perm[j];
+        perm[j] = t;
+      }
+    }
+  }
+  __syncthreads();
+  const int c_begin = cb + blockIdx.x * cpw;
+  const int c_end = min(ce, c_begin + cpw);
+  for (int c0 = c_begin; c0 < c_end; c0 += tc) {
+    const int nc = min(tc, c_end - c0);
+    for (int e = threadIdx.x; e < nc * R; e += 256) {
+      const int c = e / R, r = e - c * R;
+      tile[c * R + r] = A[(int64_t)(c0 + c) * lda + k0 + r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc * R; e += 256) {
+      const int c = e / R, r = e - c * R;
+      A[(int64_t)(c0 + c) * lda + k0 + r] = tile[c * R + perm[r]];
+    }
+    __syncthreads();
+  }
+}
+
 // Triangular-solve leaf, n <= 32 rows: X = L^{-1} B (unit lower) or
 // X = U^{-1} B (upper).  One wave per 64 right-hand-side columns: the 32 x 64
 // tile of B is staged through LDS with coalesced loads/stores (each column's
@@ -1379,6 +1429,15 @@ k_trsm_leaf_reg(int n, int nrhs, const T* __restrict__ L, int64_t ldl, int64_t s
 // ---------------------------------------------------------------------------
 // Host orchestration
 // ---------------------------------------------------------------------------
+// QB_LASWP_TAIL=0 (read once): the last steps' interchanges one lane per column.
+static bool laswp_tail_off() {
+  static const bool off = [] {
+    const char* e = getenv("QB_LASWP_TAIL");
+    return e && e[0] == '0';
+  }();
+  return off;
+}
+
 template <typename T>
 static int laswp(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int k0, int k1,
                  const int* ipiv, int64_t sP, int batch) {
@@ -1386,6 +1445,24 @@ static int laswp(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int
   dim3 g((ce - cb + 127) / 128, batch);
   hipLaunchKernelGGL(k_laswp<T>, g, dim3(128), 0, ctx->stream, A, lda, sA, cb, ce, k0, k1, ipiv,
                      sP);
+  QB_HIP(ctx, hipGetLastError());
+  return QB_OK;
+}
+
+// laswp for interchanges whose rows all lie in [k0, n): the LDS block kernel
+// when n - k0 <= 2048, else k_laswp.
+template <typename T>
+static int laswp_rows(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int k0, int k1,
+                      int n, const int* ipiv, int64_t sP, int batch) {
+  if (ce <= cb || k1 <= k0) return QB_OK;
+  const int R = n - k0;
+  if (R > 2048 || laswp_tail_off()) return laswp(ctx, A, lda, sA, cb, ce, k0, k1, ipiv, sP, batch);
+  const int tc = (int)((size_t)48 * 1024 / ((size_t)R * sizeof(T)));
+  const int cpw = 256;
+  const size_t lds = (size_t)tc * R * sizeof(T) + (size_t)(R + (k1 - k0)) * sizeof(int);
+  dim3 g((ce - cb + cpw - 1) / cpw, batch);
+  hipLaunchKernelGGL(k_laswp_tail<T>, g, dim3(256), lds, ctx->stream, A, lda, sA, cb, ce, k0, k1,
+                     n, ipiv, sP, cpw, tc);
   QB_HIP(ctx, hipGetLastError());
   return QB_OK;
 }
@@ -1641,12 +1718,14 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
     if (left_aux && k0 > 0) {
       QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
       QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
-      rc = on_aux([&] { return laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch); });
+      rc = on_aux([&] {
+        return laswp_rows(ctx, A, lda, sA, 0, k0, k0, k0 + kb, n, ipiv, sP, batch);
+      });
     } else {
-      rc = laswp(ctx, A, lda, sA, 0, k0, k0, k0 + kb, ipiv, sP, batch);
+      rc = laswp_rows(ctx, A, lda, sA, 0, k0, k0, k0 + kb, n, ipiv, sP, batch);
     }
     if (rc) return rc;
-    rc = laswp(ctx, A, lda, sA, k0 + kb, n, k0, k0 + kb, ipiv, sP, batch);
+    rc = laswp_rows(ctx, A, lda, sA, k0 + kb, n, k0, k0 + kb, n, ipiv, sP, batch);
     if (rc) return rc;
     if (k0 + kb >= n) break;
     const int k1 = k0 + kb, m = n - k1;
