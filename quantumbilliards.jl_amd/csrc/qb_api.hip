@@ -178,7 +178,8 @@ int qb_ctx_create(int device, qb_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&c->stream_aux, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->ev_main, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_aux, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->ev_aux, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_first, hipEventDisableTiming) != hipSuccess) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream_aux) (void)hipStreamDestroy(c->stream_aux);
     delete c;
@@ -204,6 +205,7 @@ void qb_ctx_destroy(qb_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->ev_main) (void)hipEventDestroy(ctx->ev_main);
   if (ctx->ev_aux) (void)hipEventDestroy(ctx->ev_aux);
+  if (ctx->ev_first) (void)hipEventDestroy(ctx->ev_first);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -1135,6 +1137,7 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
   const int nz = nc * nwin;  // wavenumbers assembled; nq from here on is the batch nz x nirr
   nq = w->nq;
   double t0 = now_ms(), t1, t2;
+  bool first_done = false;  // first LU panel factored during the assembly
   DevTables tab;
   std::vector<cd> wbuf(2 * nz);
   double rmin, rmax;
@@ -1186,8 +1189,25 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
     cf* o32 = precision == QB_MIXED ? w->LU32 : nullptr;
     const bool no_conj = getenv("QB_ASM_NO_CONJ") != nullptr;
     int crc = 1;
+    // The first LU panel (columns [0, NB) of every node, complete after the
+    // assembly's first launch) is factored on the second stream while the
+    // rest of the matrices assemble; QB_FIRST_PANEL_AHEAD=0 turns this off.
+    const char* fpa = getenv("QB_FIRST_PANEL_AHEAD");
+    const bool ahead = !(fpa && fpa[0] == '0') && !ctx->graphs;
+    const int nb = qb_lu_nb((int)N);
+    auto first_panel = [&]() -> int {
+      const int r = precision == QB_FP64
+                        ? qb_getrf_first_panel<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, w->ipiv,
+                                                   N, w->info, nq)
+                        : qb_getrf_first_panel<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN,
+                                                   w->ipiv, N, w->info, nq);
+      if (r == QB_OK) first_done = true;
+      return r;
+    };
     if (!no_conj && !getenv("QB_ASM_TABLES"))
-      crc = qb_assemble_env_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, rmin, rmax, w->T64, o32);
+      crc = qb_assemble_env_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, rmin, rmax, w->T64, o32,
+                                ahead ? nb : 0,
+                                ahead ? std::function<int()>(first_panel) : std::function<int()>());
     if (crc == 1) {
       QB_TRY(qb_tables_build(ctx, (const double*)w->z_dev, nz, rmin, rmax, *cheb, true, &tab));
       crc = no_conj ? 1 : qb_assemble_conj_dev(ctx, g, op, w->zj.data(), w->z_dev, nz, tab,
@@ -1211,10 +1231,12 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
   t1 = now_ms();
   // LU of every node (beyn.jl:270-277)
   if (precision == QB_FP64) {
-    QB_TRY(qb_getrf_graph<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
+    QB_TRY(qb_getrf_graph<cd>(ctx, w->T64, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq,
+                              first_done));
   } else {
     QB_TRY(qb_node_rownorm(ctx, w->T64, N, nq, w->tnorm));
-    QB_TRY(qb_getrf_graph<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq));
+    QB_TRY(qb_getrf_graph<cf>(ctx, w->LU32, (int)N, N, (int64_t)NN, w->ipiv, N, w->info, nq,
+                              first_done));
   }
   QB_TRY(qb_perm_from_ipiv(ctx, w->ipiv, w->perm, (int)N, nq));
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {

@@ -13,6 +13,7 @@
 #include "qb_bessel.h"
 #include "qb_cheb.h"
 #include "qb_internal.h"
+#include <functional>
 
 #define QB_INV_TWO_PI 0.15915494309189533577
 #define QB_EULER_OVER_PI (QB_EULER_GAMMA / QB_PI)
@@ -552,7 +553,7 @@ __global__ __launch_bounds__(64) void k_env_tables(const cd* __restrict__ zk, in
 template <int OP, int MW, int OCC>
 __global__ void __launch_bounds__(256, OCC)
 k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env, int o0, int P,
-               const int* __restrict__ pairs, int T, cd* __restrict__ out64,
+               const int* __restrict__ pairs, int T, int J0, int first, cd* __restrict__ out64,
                cf* __restrict__ out32) {
   static_assert(MW == 4 || MW == 8 || MW == 16, "mirror width");
   extern __shared__ __align__(16) cd smem[];
@@ -567,13 +568,30 @@ k_assemble_env(DevGeom g, const cd* __restrict__ zk, const cd* __restrict__ env,
     Ta[e] = env[(int64_t)qa * tsz + e];
     Tb[e] = env[(int64_t)qb * tsz + e];
   }
+  // tile pair (It, Jt), It >= Jt: first = 1 walks the pairs with Jt < J0
+  // column by column (the first J0 tile columns, and by the mirror the first
+  // J0 tile rows, complete); first = 0 the pairs with Jt >= J0 diagonal by
+  // diagonal (J0 = 0: every pair)
   int64_t p = blockIdx.x;
-  int d = 0;
-  while (p >= T - d) {
-    p -= T - d;
-    ++d;
+  int It, Jt;
+  if (first) {
+    int j = 0;
+    while (p >= T - j) {
+      p -= T - j;
+      ++j;
+    }
+    Jt = j;
+    It = j + (int)p;
+  } else {
+    const int Tr = T - J0;
+    int d = 0;
+    while (p >= Tr - d) {
+      p -= Tr - d;
+      ++d;
+    }
+    Jt = J0 + (int)p;
+    It = Jt + d;
   }
-  const int Jt = (int)p, It = Jt + d;
   const cd ka = zk[qa], kb = zk[qb];
   const int il = threadIdx.x & 63, jl0 = threadIdx.x >> 6;
   const int64_t i = (int64_t)It * 64 + il;
@@ -701,7 +719,7 @@ static bool env_occ3_default(int op) { return op == QB_DLP_KRESS; }
 
 int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
                         const cd* z_dev, int64_t nz, double rmin, double rmax, cd* out64,
-                        cf* out32) {
+                        cf* out32, int cols_first, const std::function<int()>& between) {
   // usable: plain geometry and every node's conjugate among the nodes;
   // returns 1 when not usable
   if (g->sym.m > 0) return 1;
@@ -738,6 +756,12 @@ int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
     const int64_t N = g->d.N;
     const int T = (int)((N + 63) / 64);
     const int64_t tpairs = (int64_t)T * (T + 1) / 2;
+    // the first cols_first columns (whole 64-column tiles) in a launch of
+    // their own, then `between` (enqueued while the rest assembles)
+    const int J0 = between && cols_first > 0 && cols_first % 64 == 0 && cols_first / 64 < T
+                       ? cols_first / 64
+                       : 0;
+    const int64_t pairs_first = (int64_t)J0 * T - (int64_t)J0 * (J0 - 1) / 2;
     // the three-workgroups-per-CU variant (4-column mirror stages) where its
     // LDS fits three times in a CU (QB_ASM_ENV_OCC=2 or 3 forces a variant)
     const size_t tab = (size_t)2 * P * 2 * QB_ENV_NC * sizeof(cd);
@@ -747,28 +771,44 @@ int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
       occ3 = e[0] == '3' && lds4 <= (size_t)(160 * 1024) / 3;
     const size_t lds = occ3 ? lds4 : lds16;
     const double ent = (double)N * N * nz;
-    const dim3 grid((unsigned)tpairs, 1, (unsigned)npair);
     slot = qb_prof_begin(ctx, QB_K_ASSEMBLE, 0.0,
                          ent * ((out64 ? sizeof(cd) : 0) + (out32 ? sizeof(cf) : 0)));
-    auto launch = [&](auto kern) {
+    auto launch = [&](auto kern, int64_t npairs, int first) {
       (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds);
-      hipLaunchKernelGGL(kern, grid, dim3(256), lds, ctx->stream, g->d, z_dev, env, o0, P, pd, T,
-                         out64, out32);
+      hipLaunchKernelGGL(kern, dim3((unsigned)npairs, 1, (unsigned)npair), dim3(256), lds,
+                         ctx->stream, g->d, z_dev, env, o0, P, pd, T, J0, first, out64, out32);
     };
-    if (op == QB_CFIE_KRESS) {
-      if (occ3)
-        launch(k_assemble_env<QB_CFIE_KRESS, 4, 3>);
-      else
-        launch(k_assemble_env<QB_CFIE_KRESS, 16, 1>);
+    auto launch_all = [&](int64_t npairs, int first) {
+      if (op == QB_CFIE_KRESS) {
+        if (occ3)
+          launch(k_assemble_env<QB_CFIE_KRESS, 4, 3>, npairs, first);
+        else
+          launch(k_assemble_env<QB_CFIE_KRESS, 16, 1>, npairs, first);
+      } else {
+        if (occ3)
+          launch(k_assemble_env<QB_DLP_KRESS, 4, 3>, npairs, first);
+        else
+          launch(k_assemble_env<QB_DLP_KRESS, 16, 1>, npairs, first);
+      }
+    };
+    int brc = 0;
+    if (J0 > 0) {
+      launch_all(pairs_first, 1);
+      e = hipGetLastError();
+      if (e == hipSuccess) brc = between();
+      if (e == hipSuccess && brc == 0) launch_all(tpairs - pairs_first, 0);
     } else {
-      if (occ3)
-        launch(k_assemble_env<QB_DLP_KRESS, 4, 3>);
-      else
-        launch(k_assemble_env<QB_DLP_KRESS, 16, 1>);
+      launch_all(tpairs, 0);
     }
     qb_prof_end(ctx, slot);
-    e = hipGetLastError();
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess && brc != 0) {
+      (void)hipStreamSynchronize(ctx->stream);
+      qb_dev_free(ctx, pd, pr.size() * sizeof(int));
+      qb_dev_free(ctx, env, ebytes);
+      return brc;
+    }
   }
   // the tables and the pair list must outlive the launch
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -65,6 +66,7 @@ struct qb_ctx {
   hipStream_t stream = nullptr;      // stream every launch of this context uses
   hipStream_t stream_aux = nullptr;  // second stream for overlapped work (LU lookahead)
   hipEvent_t ev_main = nullptr, ev_aux = nullptr;  // cross-stream ordering
+  hipEvent_t ev_first = nullptr;  // end of a first LU panel factored ahead (qb_getrf_first_panel)
   std::string err;
   int64_t bytes = 0;  // device bytes held by objects of this context
   int gemm_pick[2][QB_GEMM_CLASSES];  // autotuned tile variant per shape class (-1: default)
@@ -199,10 +201,16 @@ int qb_assemble_conj_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host
                          cf* out32);
 // The same matrices from envelope tables of H_n e^{-ikr} held in LDS (no
 // uniform tables needed; rmin, rmax: the distance range of the geometry);
-// returns 1, having done nothing, when the nodes do not pair up.
+// returns 1, having done nothing, when the nodes do not pair up.  With
+// `between` set and cols_first a multiple of 64 below N, columns [0,
+// cols_first) of every matrix (and rows [0, cols_first)) are assembled by a
+// first launch, then between() is called (to enqueue work on those columns
+// on another stream) and the rest is assembled; the rest never writes those
+// columns.
 int qb_assemble_env_dev(qb_ctx* ctx, const qb_geom* g, int op, const cd* z_host,
                         const cd* z_dev, int64_t nz, double rmin, double rmax, cd* out64,
-                        cf* out32);
+                        cf* out32, int cols_first = 0,
+                        const std::function<int()>& between = std::function<int()>());
 // Pairs (a, b) of wavenumbers with z_b = conj(z_a) (a real one with itself),
 // flattened; false when some wavenumber has no partner.
 bool qb_conj_pairs(const cd* z_host, int64_t nz, std::vector<int>* pr);

@@ -12,22 +12,35 @@ int gemm_sub_v(qb_ctx* ctx, int v, int M, int N, int K, const T* A, int64_t lda,
                const T* B, int64_t ldb, int64_t sB, T* C, int64_t ldc, int64_t sC, int batch);
 // complex128 variant 13: the 32 x 32 x 8 Gauss-3M tile with 19 KB of LDS
 #define QB_GEMM_LEAN_C128 13
+// first_done: the first panel was already factored by qb_getrf_first_panel
+// (the call then starts by waiting for it).
 template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
-                     int* info, int batch);
+                     int* info, int batch, bool first_done = false);
+// The first panel of qb_getrf_batched (columns [0, min(n, qb_lu_nb(n)))) on
+// the context's second stream, after the work already enqueued on the main
+// stream; pass first_done to the qb_getrf_batched / qb_getrf_graph call that
+// factors the rest.  Work on other columns may run on the main stream
+// meanwhile.
+template <typename T>
+int qb_getrf_first_panel(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv,
+                         int64_t sP, int* info, int batch);
+int qb_lu_nb(int n);
 // qb_getrf_batched through a cached hipGraph when ctx->graphs is set and
 // launch accounting is off: the first call with given buffers captures the
 // launch sequence (both streams) and instantiates it, later calls replay it.
+// With first_done the call is not captured.
 template <typename T>
 int qb_getrf_graph(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
-                   int* info, int batch);
+                   int* info, int batch, bool first_done = false);
 void qb_graphs_clear(qb_ctx* ctx);
-// Triangular solves only (the row permutation is applied separately, fused
-// into the gather that fills B).
 // Triangular-solve leaf kernel for every later launch of the process: 1 the
 // one-wave LDS leaf (k_trsm_leaf), 0 the register leaf (k_trsm_leaf_reg), -1
-// back to the default (QB_TRSM_LEAF, else the register leaf).
+// back to the default (QB_TRSM_LEAF=lds, else the register leaf for launches
+// of at least 8 waves per CU and the LDS leaf below).
 void qb_set_trsm_leaf_lds(int mode);
+// Triangular solves only (the row permutation is applied separately, fused
+// into the gather that fills B).
 template <typename T>
 int qb_trsm_lu_batched(qb_ctx* ctx, const T* LU, int n, int64_t lda, int64_t sA, T* B, int nrhs,
                        int64_t ldb, int64_t sB, int batch);

@@ -1691,8 +1691,23 @@ static bool laswp_left_aux() {
 }
 
 template <typename T>
+int qb_getrf_first_panel(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv,
+                         int64_t sP, int* info, int batch) {
+  const int NB = qb_lu_nb(n);
+  hipStream_t main = ctx->stream, aux = ctx->stream_aux;
+  QB_HIP(ctx, hipEventRecord(ctx->ev_main, main));
+  QB_HIP(ctx, hipStreamWaitEvent(aux, ctx->ev_main, 0));
+  ctx->stream = aux;
+  const int rc = panel_rec(ctx, A, lda, sA, n, 0, n < NB ? n : NB, ipiv, sP, info, batch);
+  ctx->stream = main;
+  if (rc) return rc;
+  QB_HIP(ctx, hipEventRecord(ctx->ev_first, aux));
+  return QB_OK;
+}
+
+template <typename T>
 int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
-                     int* info, int batch) {
+                     int* info, int batch, bool first_done) {
   // Lookahead: after the block row of step s is solved, the columns of panel
   // s+1 are updated first; panel s+1 is then factored on the auxiliary
   // stream while the rest of the trailing update runs on the main stream.
@@ -1706,7 +1721,11 @@ int qb_getrf_batched(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipi
     ctx->stream = main;
     return rc;
   };
-  int rc = panel_rec(ctx, A, lda, sA, n, 0, n < NB ? n : NB, ipiv, sP, info, batch);
+  int rc = QB_OK;
+  if (first_done)
+    QB_HIP(ctx, hipStreamWaitEvent(main, ctx->ev_first, 0));
+  else
+    rc = panel_rec(ctx, A, lda, sA, n, 0, n < NB ? n : NB, ipiv, sP, info, batch);
   if (rc) return rc;
   // The interchanges of step s in the columns left of its panel (the L part
   // of earlier panels) run on the auxiliary stream, ahead of the next panel:
@@ -1773,9 +1792,9 @@ void qb_graphs_clear(qb_ctx* ctx) {
 
 template <typename T>
 int qb_getrf_graph(qb_ctx* ctx, T* A, int n, int64_t lda, int64_t sA, int* ipiv, int64_t sP,
-                   int* info, int batch) {
-  if (!ctx->graphs || ctx->prof || ctx->bix)
-    return qb_getrf_batched<T>(ctx, A, n, lda, sA, ipiv, sP, info, batch);
+                   int* info, int batch, bool first_done) {
+  if (!ctx->graphs || ctx->prof || ctx->bix || first_done)
+    return qb_getrf_batched<T>(ctx, A, n, lda, sA, ipiv, sP, info, batch, first_done);
   const int ty = sizeof(T) == sizeof(cf) ? 0 : 1;
   for (auto& g : ctx->graph_cache)
     if (g.ty == ty && g.A == (const void*)A && g.n == n && g.lda == lda && g.sA == sA &&
@@ -2074,10 +2093,18 @@ int qb_node_rownorm(qb_ctx* ctx, const cd* A, int64_t n, int nb, double* out_dev
 }
 
 // explicit instantiations
-template int qb_getrf_batched<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int);
-template int qb_getrf_batched<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int);
-template int qb_getrf_graph<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int);
-template int qb_getrf_graph<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int);
+template int qb_getrf_batched<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int,
+                                  bool);
+template int qb_getrf_batched<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int,
+                                  bool);
+template int qb_getrf_first_panel<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*,
+                                      int);
+template int qb_getrf_first_panel<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*,
+                                      int);
+template int qb_getrf_graph<cf>(qb_ctx*, cf*, int, int64_t, int64_t, int*, int64_t, int*, int,
+                                bool);
+template int qb_getrf_graph<cd>(qb_ctx*, cd*, int, int64_t, int64_t, int*, int64_t, int*, int,
+                                bool);
 template int qb_trsm_lu_batched<cf>(qb_ctx*, const cf*, int, int64_t, int64_t, cf*, int, int64_t,
                                     int64_t, int);
 template int qb_trsm_lu_batched<cd>(qb_ctx*, const cd*, int, int64_t, int64_t, cd*, int, int64_t,
