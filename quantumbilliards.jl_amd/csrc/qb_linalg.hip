@@ -1177,29 +1177,37 @@ __global__ void __launch_bounds__(256) k_laswp_tail(T* __restrict__ A, int64_t l
   for (int r = threadIdx.x; r < R; r += 256) perm[r] = r;
   for (int i = threadIdx.x; i < ns; i += 256) ip[i] = ipiv[k0 + i] - k0;
   __syncthreads();
+  __shared__ int any_moved;
   if (threadIdx.x == 0) {
+    int moved = 0;
     for (int i = 0; i < ns; ++i) {
       const int j = ip[i];
       if (j != i) {
         const int t = perm[i];
         perm[i] = perm[j];
         perm[j] = t;
+        moved = 1;
       }
     }
+    any_moved = moved;
   }
   __syncthreads();
+  if (!any_moved) return;  // uniform: no interchange in this step
+  // only the rows that move are loaded and stored (the permutation maps the
+  // moved rows onto themselves)
   const int c_begin = cb + blockIdx.x * cpw;
   const int c_end = min(ce, c_begin + cpw);
   for (int c0 = c_begin; c0 < c_end; c0 += tc) {
     const int nc = min(tc, c_end - c0);
     for (int e = threadIdx.x; e < nc * R; e += 256) {
       const int c = e / R, r = e - c * R;
-      tile[c * R + r] = A[(int64_t)(c0 + c) * lda + k0 + r];
+      if (perm[r] != r) tile[c * R + r] = A[(int64_t)(c0 + c) * lda + k0 + r];
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nc * R; e += 256) {
       const int c = e / R, r = e - c * R;
-      A[(int64_t)(c0 + c) * lda + k0 + r] = tile[c * R + perm[r]];
+      const int q = perm[r];
+      if (q != r) A[(int64_t)(c0 + c) * lda + k0 + r] = tile[c * R + q];
     }
     __syncthreads();
   }
@@ -1450,13 +1458,17 @@ static int laswp(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int
 }
 
 // laswp for interchanges whose rows all lie in [k0, n): the LDS block kernel
-// when n - k0 <= 2048, else k_laswp.
+// when n - k0 <= 2048 and n >= 4096, else k_laswp.  Measured: the last steps
+// at n = 8192 (48 matrices) gain 5 ms per window; at n = 1640 (240 matrices,
+// mostly diagonal pivots, which k_laswp skips at no cost) every step would
+// take the kernel and the C_5 windows lost 7%.
 template <typename T>
 static int laswp_rows(qb_ctx* ctx, T* A, int64_t lda, int64_t sA, int cb, int ce, int k0, int k1,
                       int n, const int* ipiv, int64_t sP, int batch) {
   if (ce <= cb || k1 <= k0) return QB_OK;
   const int R = n - k0;
-  if (R > 2048 || laswp_tail_off()) return laswp(ctx, A, lda, sA, cb, ce, k0, k1, ipiv, sP, batch);
+  if (R > 2048 || n < 4096 || laswp_tail_off())
+    return laswp(ctx, A, lda, sA, cb, ce, k0, k1, ipiv, sP, batch);
   const int tc = (int)((size_t)48 * 1024 / ((size_t)R * sizeof(T)));
   const int cpw = 256;
   const size_t lds = (size_t)tc * R * sizeof(T) + (size_t)(R + (k1 - k0)) * sizeof(int);
