@@ -36,7 +36,7 @@ PEAK = {"c64": 157.3, "c128": 78.6, "i8": 5000.0}  # dense MFMA TFLOP/s (fp32, f
 # HBM bytes per GEMM launch over the windows of the default workload, from
 # FETCH_SIZE / WRITE_SIZE counter runs scaled by a copy kernel of known size
 # (profiles/beyn_window_star_N8192_pmc_traffic.txt)
-TRAFFIC_PER_LAUNCH = {"c128": 1439.9e6, "c64": 488.6e6}
+TRAFFIC_PER_LAUNCH = {"c128": 1664.7e6, "c64": 467.3e6}
 
 
 def parse():
