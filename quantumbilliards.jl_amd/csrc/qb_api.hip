@@ -1255,7 +1255,10 @@ static int win_create(qb_ctx* ctx, const qb_geom* g, int32_t op, int32_t nwin, c
   *out = w;
   return QB_OK;
 fail:
+  // both streams: the first LU panel or the LU's lookahead may still be
+  // running on the second one when a later step fails
   (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream_aux);
   qb_tables_free(ctx, &tab);
   if (rc == QB_OK) rc = QB_E_HIP;
   win_free(w);
@@ -1580,6 +1583,7 @@ static int win_moments(qb_window* w, int32_t r, const double* V, double* A0, dou
 fail:
   ctx->bix = nullptr;
   (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream_aux);
   qb_dev_free(ctx, actd, nq * sizeof(int));
   qb_dev_free(ctx, Vd, nv * sizeof(cd));
   qb_dev_free(ctx, X, (size_t)nq * nv * sizeof(cd));
